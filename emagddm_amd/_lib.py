@@ -1,0 +1,91 @@
+"""ctypes binding of libemagddm_fetidp.so (the C ABI in include/emagddm_fetidp.h).
+
+The shared library is built in-tree by ``emagddm_amd/csrc/Makefile``.  There is
+no fallback: if the library is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemagddm_fetidp.so")
+
+SUCCESS, NUMERICAL_ERROR, DIVERGED, MAX_ITER_REACHED, INVALID_INPUT = range(5)
+STATUS_NAMES = {0: "SUCCESS", 1: "NUMERICAL_ERROR", 2: "DIVERGED", 3: "MAX_ITER_REACHED",
+                4: "INVALID_INPUT"}
+FP64, FP32_REFINE = 0, 1
+
+
+class Config(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("max_iterations", C.c_int32),
+                ("tolerance", C.c_double), ("dual_tolerance", C.c_double),
+                ("max_refinements", C.c_int32), ("verbose", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("refinements", C.c_int32),
+                ("reserved", C.c_int32), ("residual_norm", C.c_double),
+                ("dual_residual", C.c_double), ("initial_dual_residual", C.c_double),
+                ("factor_ms", C.c_double), ("solve_ms", C.c_double), ("symbolic_ms", C.c_double)]
+
+
+EXPORTS = [
+    "fetidp_config_default", "fetidp_create", "fetidp_setup", "fetidp_set_materials",
+    "fetidp_set_rhs_tet_J", "fetidp_set_rhs_free", "fetidp_factor", "fetidp_solve",
+    "fetidp_get_sizes", "fetidp_get_topology", "fetidp_get_global_nnz",
+    "fetidp_get_global_system", "fetidp_apply_dual_operator", "fetidp_apply_preconditioner",
+    "fetidp_get_dual_rhs", "fetidp_get_lambda", "fetidp_get_history",
+    "fetidp_get_coarse_matrix", "fetidp_last_error", "fetidp_destroy", "emag_nedelec_integrate",
+]
+
+_lib = None
+
+
+def load():
+    """Load the in-tree library once; raise loudly if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C emagddm_amd/csrc` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    lp = C.POINTER(C.c_int64)
+    lib.fetidp_config_default.argtypes = [C.POINTER(Config)]
+    lib.fetidp_config_default.restype = None
+    lib.fetidp_create.argtypes = [C.POINTER(Config)]
+    lib.fetidp_create.restype = P
+    lib.fetidp_setup.argtypes = [P, C.c_int64, dp, C.c_int64, ip, ip, C.c_int32, dp, dp, ip,
+                                 C.c_int64, ip]
+    lib.fetidp_set_materials.argtypes = [P, C.c_int32, dp, dp]
+    lib.fetidp_set_rhs_tet_J.argtypes = [P, dp]
+    lib.fetidp_set_rhs_free.argtypes = [P, dp]
+    lib.fetidp_factor.argtypes = [P]
+    lib.fetidp_solve.argtypes = [P, dp, dp, C.POINTER(Stats)]
+    lib.fetidp_get_sizes.argtypes = [P, lp]
+    lib.fetidp_get_topology.argtypes = [P, ip, ip, ip, ip]
+    lib.fetidp_get_global_nnz.argtypes = [P]
+    lib.fetidp_get_global_nnz.restype = C.c_int64
+    lib.fetidp_get_global_system.argtypes = [P, ip, ip, dp, dp]
+    lib.fetidp_apply_dual_operator.argtypes = [P, dp, dp]
+    lib.fetidp_apply_preconditioner.argtypes = [P, dp, dp]
+    lib.fetidp_get_dual_rhs.argtypes = [P, dp]
+    lib.fetidp_get_lambda.argtypes = [P, dp]
+    lib.fetidp_get_history.argtypes = [P, dp, C.c_int32]
+    lib.fetidp_get_coarse_matrix.argtypes = [P, dp]
+    lib.fetidp_last_error.argtypes = [P]
+    lib.fetidp_last_error.restype = C.c_char_p
+    lib.fetidp_destroy.argtypes = [P]
+    lib.fetidp_destroy.restype = None
+    lib.emag_nedelec_integrate.argtypes = [C.c_int64, P, P, P, P, P, P, P, P, P, P]
+    for name in EXPORTS:
+        fn = getattr(lib, name)
+        if fn.restype is C.c_int:  # default
+            fn.restype = C.c_int32
+    _lib = lib
+    return lib

@@ -1,0 +1,780 @@
+// FETI-DP solver orchestration behind the C ABI (include/emagddm_fetidp.h).
+//
+// Host C++ sequences the HIP kernels of kernels.hip; all numeric work runs on
+// the GPU.  The reference has no FETI-DP implementation: the solver takes the
+// place of FETIDPContext (src/fetidp/fetidp_context.cpp:19-35) and follows the
+// EMLinearSolverBase contract (include/numeric/em_linear_solver.h:76-236):
+// statuses instead of exceptions, factor once / solve many.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "emagddm_fetidp.h"
+#include "kernels.h"
+#include "plan.h"
+#include "topology.h"
+
+using namespace emag;
+
+namespace {
+
+#define HIPCHECK(x)                                                                       \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      throw std::runtime_error(std::string(#x) + " failed: " + hipGetErrorString(e_));    \
+  } while (0)
+
+struct InputError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct NumericError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    n = count;
+    if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+  }
+  void upload(const std::vector<T>& v) {
+    alloc(v.size());
+    if (n) HIPCHECK(hipMemcpy(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void upload(const T* src, size_t count) {
+    alloc(count);
+    if (n) HIPCHECK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void zero(hipStream_t st) {
+    if (n) HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), st));
+  }
+};
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct fetidp_solver {
+  fetidp_config cfg{};
+  std::string err;
+  Topology topo;
+  Plans plans;
+  bool setup_done = false, factored = false, rhs_from_J = false, rhs_set = false;
+  hipStream_t st = nullptr;
+  int32_t n_mat = 0;
+  double symbolic_ms = 0.0, last_factor_ms = 0.0, upload_ms = 0.0;
+  bool device_ready = false;
+  std::vector<double> h_xyz_, h_nu_, h_kappa_;
+  std::vector<int32_t> h_t2n_, h_mat_;
+
+  DBuf<double> xyz, nu, kappa, jq;
+  DBuf<int32_t> t2n, mat;
+  DBuf<double> Ae, Fe;
+  DBuf<int64_t> pool_target, pool_ptr;
+  DBuf<int32_t> pool_src;
+  DBuf<int64_t> rhs_target, rhs_ptr;
+  DBuf<int32_t> rhs_src;
+  DBuf<int32_t> csr_rp, csr_col, csr_src;
+  DBuf<int64_t> csr_ptr;
+  DBuf<double> csr_val;
+  DBuf<int64_t> coarse_target, coarse_ptr, coarse_srcpos;
+  DBuf<TileWork> local_items, coarse_items;
+  DBuf<int64_t> local_base, coarse_base;
+  DBuf<DevSub> devsub;
+  DBuf<int32_t> vec_dof;
+  DBuf<double> vec_w;
+  DBuf<int32_t> d_lam;
+  DBuf<double> d_sgn, d_sgnD, d_half, rho;
+  DBuf<int64_t> lam_pos, rho_pos;
+  DBuf<int32_t> c_loc;
+  DBuf<int64_t> c_ptr, c_ypos, c_vpos;
+  DBuf<int8_t> free_kind;
+  DBuf<int64_t> free_a;
+  DBuf<double> pool64, inv64, W64, P64;
+  DBuf<float> pool32, inv32, W32, P32;
+  DBuf<int> fail;
+  DBuf<double> cpool, cinv, sinv;
+  DBuf<int> cfail;
+  DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc;
+  DBuf<double> lam, r, z, p, q, dv, ubar, x, xtot, rglob, partial;
+  std::vector<double> hist;
+  double last_dual = 0.0, init_dual = 0.0;
+
+  ~fetidp_solver() {
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  bool fp32() const { return cfg.precision == FETIDP_FP32_REFINE; }
+
+  // ---------------------------------------------------------------- setup
+  void setup(int64_t n_nodes, const double* h_xyz, int64_t n_tets, const int32_t* h_t2n,
+             const int32_t* h_mat, int32_t nmat, const double* h_nu, const double* h_kappa,
+             const int32_t* h_sub, int64_t n_pec, const int32_t* h_pec) {
+    if (n_nodes <= 0 || n_tets <= 0 || !h_xyz || !h_t2n || !h_mat || !h_nu || !h_kappa || !h_sub)
+      throw InputError("setup: null or empty mesh input");
+    if (n_pec < 0 || (n_pec > 0 && !h_pec)) throw InputError("setup: bad PEC input");
+    if (nmat <= 0) throw InputError("setup: n_mat must be positive");
+    for (int64_t t = 0; t < n_tets; ++t)
+      if (h_mat[t] < 0 || h_mat[t] >= nmat) throw InputError("setup: material id out of range");
+    for (int32_t m = 0; m < nmat; ++m)
+      if (!(h_nu[m] > 0.0) || !(h_kappa[m] >= 0.0)) throw InputError("setup: need nu > 0, kappa >= 0");
+    const double t0 = now_ms();
+    setup_done = device_ready = factored = rhs_set = false;
+    MeshInput in{n_nodes, n_tets, h_t2n, h_sub, n_pec, h_pec};
+    std::string e = build_topology(in, topo);
+    if (!e.empty()) throw InputError("setup: " + e);
+    build_plans(topo, plans);
+    n_mat = nmat;
+    h_xyz_.assign(h_xyz, h_xyz + 3 * n_nodes);
+    h_t2n_.assign(h_t2n, h_t2n + 4 * n_tets);
+    h_mat_.assign(h_mat, h_mat + n_tets);
+    h_nu_.assign(h_nu, h_nu + nmat);
+    h_kappa_.assign(h_kappa, h_kappa + nmat);
+    setup_done = true;
+    symbolic_ms = now_ms() - t0;
+  }
+
+  // Device allocations and uploads, once per setup (first GPU-touching call).
+  void ensure_device() {
+    if (!setup_done) throw InputError("call fetidp_setup first");
+    if (device_ready) return;
+    const double t0 = now_ms();
+    if (!st) HIPCHECK(hipStreamCreate(&st));
+    xyz.upload(h_xyz_);
+    t2n.upload(h_t2n_);
+    mat.upload(h_mat_);
+    nu.upload(h_nu_);
+    kappa.upload(h_kappa_);
+    const int64_t nt = topo.n_tets;
+    Ae.alloc(size_t(21 * nt + 1));
+    {
+      const double one = 1.0;
+      HIPCHECK(hipMemcpy(Ae.p + 21 * nt, &one, sizeof(double), hipMemcpyHostToDevice));
+    }
+    Fe.alloc(size_t(6 * nt));
+    pool_target.upload(plans.pool.target);
+    pool_ptr.upload(plans.pool.ptr);
+    pool_src.upload(plans.pool.src);
+    rhs_target.upload(plans.rhs_global.target);
+    rhs_ptr.upload(plans.rhs_global.ptr);
+    rhs_src.upload(plans.rhs_global.src);
+    csr_rp.upload(plans.csr.row_ptr);
+    csr_col.upload(plans.csr.col);
+    csr_ptr.upload(plans.csr.ptr);
+    csr_src.upload(plans.csr.src);
+    csr_val.alloc(plans.csr.col.size());
+    coarse_target.upload(plans.coarse.target);
+    coarse_ptr.upload(plans.coarse.ptr);
+    coarse_srcpos.upload(plans.coarse.srcpos);
+    local_items.upload(plans.local_sched.items);
+    coarse_items.upload(plans.coarse_sched.items);
+    {
+      std::vector<int64_t> base(topo.n_sub);
+      for (int s = 0; s < topo.n_sub; ++s) base[s] = topo.sub[s].pool_off;
+      local_base.upload(base);
+      coarse_base.upload(std::vector<int64_t>{0});
+    }
+    devsub.upload(plans.devsub);
+    vec_dof.upload(plans.vec_dof);
+    vec_w.upload(plans.vec_w);
+    d_lam.upload(topo.d_lam);
+    d_sgn.upload(topo.d_sgn);
+    d_sgnD.alloc(topo.dvec_elems);
+    d_half.alloc(topo.dvec_elems);
+    rho.alloc(topo.dvec_elems);
+    lam_pos.upload(topo.lam_pos);
+    rho_pos.upload(plans.rho_pos);
+    c_loc.upload(topo.c_loc);
+    c_ptr.upload(topo.c_gather_ptr);
+    c_ypos.upload(plans.c_ypos);
+    c_vpos.upload(plans.c_vpos);
+    free_kind.upload(topo.free_kind);
+    free_a.upload(topo.free_src_a);
+    fail.alloc(topo.n_sub);
+    cfail.alloc(1);
+    const int64_t nf = topo.n_free, nl = topo.n_lam, nc = std::max<int64_t>(topo.n_c, 1);
+    f_glob.alloc(nf);
+    f_cur.alloc(nf);
+    vec.alloc(topo.vec_elems);
+    vout.alloc(topo.vec_elems);
+    U.alloc(topo.vec_elems);
+    ybuf.alloc(topo.y_elems);
+    dbuf.alloc(topo.dvec_elems);
+    fc.alloc(nc);
+    uc.alloc(nc);
+    gc.alloc(nc);
+    for (DBuf<double>* b : {&lam, &r, &z, &p, &q, &dv, &ubar}) b->alloc(std::max<int64_t>(nl, 1));
+    x.alloc(nf);
+    xtot.alloc(nf);
+    rglob.alloc(nf);
+    partial.alloc(3 * kDotBlocks);
+    cpool.alloc(plans.coarse_pool_elems);
+    cinv.alloc(kTileElems);
+    sinv.alloc(size_t(topo.n_c) * size_t(std::max<int64_t>(topo.n_c, 1)));
+    HIPCHECK(hipStreamSynchronize(st));
+    device_ready = true;
+    upload_ms = now_ms() - t0;
+  }
+
+  void set_materials(int32_t nmat, const double* h_nu, const double* h_kappa) {
+    if (!setup_done) throw InputError("set_materials before setup");
+    if (nmat != n_mat || !h_nu || !h_kappa) throw InputError("set_materials: n_mat mismatch");
+    for (int32_t m = 0; m < nmat; ++m)
+      if (!(h_nu[m] > 0.0) || !(h_kappa[m] >= 0.0)) throw InputError("set_materials: need nu > 0, kappa >= 0");
+    h_nu_.assign(h_nu, h_nu + nmat);
+    h_kappa_.assign(h_kappa, h_kappa + nmat);
+    if (device_ready) {
+      nu.upload(h_nu_);
+      kappa.upload(h_kappa_);
+    }
+    factored = false;
+  }
+
+  void set_rhs_J(const double* h_jq) {
+    if (!setup_done) throw InputError("set_rhs before setup");
+    if (!h_jq) throw InputError("set_rhs: null J");
+    ensure_device();
+    jq.upload(h_jq, size_t(12 * topo.n_tets));
+    rhs_from_J = true;
+    rhs_set = true;
+  }
+
+  void set_rhs_free(const double* h_f) {
+    if (!setup_done) throw InputError("set_rhs before setup");
+    if (!h_f) throw InputError("set_rhs: null f");
+    ensure_device();
+    HIPCHECK(hipMemcpy(f_glob.p, h_f, sizeof(double) * topo.n_free, hipMemcpyHostToDevice));
+    rhs_from_J = false;
+    rhs_set = true;
+  }
+
+  // ---------------------------------------------------------------- factor
+  template <typename T>
+  void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& base,
+                    T* pool, T* inv, int* failp, T* P) {
+    for (const TileLaunch& L : S.launches) {
+      if (L.mode < 0) {
+        launch_extract_P<T>(topo.n_sub, plans.max_ndp, devsub.p, pool, P, st);
+        continue;
+      }
+      launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, pool, inv, failp, st);
+    }
+  }
+
+  template <typename T>
+  void factor_impl(DBuf<T>& pool, DBuf<T>& inv, DBuf<T>& W, DBuf<T>& P) {
+    const int64_t nt = topo.n_tets;
+    // element matrices A_e = K_e + M_e (and F_e when the RHS is given by J)
+    launch_integrate(nt, xyz.p, t2n.p, mat.p, nu.p, kappa.p, rhs_from_J ? jq.p : nullptr, 1, Ae.p,
+                     nullptr, Fe.p, st);
+    // global CSR values (FP64) for the residual check
+    launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st);
+    if (rhs_from_J)
+      launch_assemble_f64(plans.rhs_global.size(), rhs_target.p, rhs_ptr.p, rhs_src.p, Fe.p, f_glob.p, st);
+    // subdomain pools
+    pool.alloc(size_t(topo.pool_elems));
+    inv.alloc(size_t(topo.n_sub) * kTileElems);
+    W.alloc(size_t(topo.w_elems));
+    P.alloc(size_t(topo.p_elems));
+    pool.zero(st);
+    if constexpr (sizeof(T) == 8)
+      launch_assemble_f64(plans.pool.size(), pool_target.p, pool_ptr.p, pool_src.p, Ae.p, pool.p, st);
+    else
+      launch_assemble_f32(plans.pool.size(), pool_target.p, pool_ptr.p, pool_src.p, Ae.p, pool.p, st);
+    launch_scaling(topo.dvec_elems, topo.n_lam, rho_pos.p, pool.p, sizeof(T) == 4, rho.p, lam_pos.p,
+                   d_sgn.p, d_sgnD.p, d_half.p, st);
+    fail.zero(st);
+    run_schedule<T>(plans.local_sched, local_items, local_base, pool.p, inv.p, fail.p, P.p);
+    launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, W.p, st);
+    // coarse problem in FP64
+    cpool.zero(st);
+    launch_gather_pool<T>(plans.coarse.size(), coarse_target.p, coarse_ptr.p, coarse_srcpos.p,
+                          pool.p, cpool.p, st);
+    cfail.zero(st);
+    run_schedule<double>(plans.coarse_sched, coarse_items, coarse_base, cpool.p, cinv.p, cfail.p, nullptr);
+    launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, sinv.p, st);
+    HIPCHECK(hipStreamSynchronize(st));
+    std::vector<int> hf(topo.n_sub);
+    HIPCHECK(hipMemcpy(hf.data(), fail.p, sizeof(int) * topo.n_sub, hipMemcpyDeviceToHost));
+    int hc = 0;
+    HIPCHECK(hipMemcpy(&hc, cfail.p, sizeof(int), hipMemcpyDeviceToHost));
+    for (int s = 0; s < topo.n_sub; ++s)
+      if (hf[s]) throw NumericError("subdomain " + std::to_string(s) + ": K_rr not positive definite");
+    if (hc) throw NumericError("coarse Schur complement not positive definite");
+  }
+
+  double factor() {
+    if (!setup_done) throw InputError("factor before setup");
+    if (!rhs_set) throw InputError("factor needs a right-hand side (set_rhs_*) first");
+    ensure_device();
+    factored = false;
+    const double t0 = now_ms();
+    if (fp32()) {
+      pool64.release(); inv64.release(); W64.release(); P64.release();
+      factor_impl<float>(pool32, inv32, W32, P32);
+    } else {
+      pool32.release(); inv32.release(); W32.release(); P32.release();
+      factor_impl<double>(pool64, inv64, W64, P64);
+    }
+    factored = true;
+    return now_ms() - t0;
+  }
+
+  // ---------------------------------------------------------------- operators
+  double dots(int npairs, const double* const* a, const double* const* b, int64_t n, double* out) {
+    launch_dots(n, npairs, a, b, partial.p, st);
+    double h[3 * kDotBlocks];
+    HIPCHECK(hipMemcpyAsync(h, partial.p, sizeof(double) * npairs * kDotBlocks, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int k = 0; k < npairs; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < kDotBlocks; ++i) s += h[k * kDotBlocks + i];
+      out[k] = s;
+    }
+    return out[0];
+  }
+
+  template <typename T>
+  void apply_F(const DBuf<T>& W, const double* pin, double* qout) {
+    launch_gemv_dual<T>(topo.n_sub, plans.max_wrows, devsub.p, W.p, 0, d_lam.p, d_sgn.p, pin, ybuf.p, st, plans.max_ndp);
+    launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
+    launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, gc.p, uc.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
+    launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
+  }
+
+  template <typename T>
+  void apply_M(const DBuf<T>& P, const double* rin, double* zout) {
+    launch_gemv_dual<T>(topo.n_sub, plans.max_ndp, devsub.p, P.p, 1, d_lam.p, d_sgnD.p, rin, dbuf.p, st, plans.max_ndp);
+    launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgnD.p, dbuf.p, zout, st);
+  }
+
+  // condense the global free-dof RHS `fsrc` into the dual RHS dv (and fc, a_s)
+  template <typename T>
+  void condense(const DBuf<T>& pool, const DBuf<T>& W, const double* fsrc) {
+    launch_split_rhs(topo.vec_elems, vec_dof.p, vec_w.p, fsrc, vec.p, st);
+    launch_forward<T>(topo.n_sub, devsub.p, pool.p, vec.p, st);
+    launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
+    launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, fc.p, uc.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, 1.0, dbuf.p, st);
+    launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, dv.p, st);
+  }
+
+  // recover the global free-dof solution from lambda into xout
+  template <typename T>
+  void recover(const DBuf<T>& pool, const DBuf<T>& W, double* xout) {
+    launch_gemv_dual<T>(topo.n_sub, plans.max_wrows, devsub.p, W.p, 0, d_lam.p, d_sgn.p, lam.p, ybuf.p, st, plans.max_ndp);
+    launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
+    launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, gc.p, uc.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, 1.0, dbuf.p, st);
+    launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
+    launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
+    launch_interior<T>(topo.n_sub, devsub.p, pool.p, vec.p, U.p, vout.p, st);
+    launch_gather_global(topo.n_free, free_kind.p, free_a.p, vout.p, ubar.p, uc.p, xout, st);
+  }
+
+  // ||fref - K xin|| / ||fref||, residual vector left in rglob
+  double global_residual(const double* xin, const double* fref) {
+    launch_csr_residual(topo.n_free, csr_rp.p, csr_col.p, csr_val.p, xin, fref, rglob.p, st);
+    const double* a[2] = {rglob.p, fref};
+    double o[2];
+    dots(2, a, a, topo.n_free, o);
+    return o[1] > 0 ? std::sqrt(o[0] / o[1]) : 0.0;
+  }
+
+  // PCPG on F lambda = d from the current lambda until ||r||/||d|| < tol.
+  // `fresh` recomputes r, z, p from lambda; otherwise continues the recurrence.
+  template <typename T>
+  int pcpg(const DBuf<T>& W, const DBuf<T>& P, double tol, int max_it, bool fresh, int& its,
+           double& dn) {
+    const int64_t nl = topo.n_lam;
+    double o[3];
+    if (fresh) {
+      const double* a1[1] = {dv.p};
+      dn = std::sqrt(dots(1, a1, a1, nl, o));
+      if (dn < 1e-300) {
+        last_dual = init_dual = 0.0;
+        return FETIDP_SUCCESS;
+      }
+      apply_F<T>(W, lam.p, q.p);
+      HIPCHECK(hipMemcpyAsync(r.p, dv.p, sizeof(double) * nl, hipMemcpyDeviceToDevice, st));
+      launch_axpy(nl, -1.0, q.p, r.p, st);
+      apply_M<T>(P, r.p, z.p);
+      const double* a[2] = {r.p, r.p};
+      const double* b[2] = {r.p, z.p};
+      dots(2, a, b, nl, o);
+      last_dual = init_dual = std::sqrt(o[0]) / dn;
+      hist.push_back(last_dual);
+      rz_ = o[1];
+      HIPCHECK(hipMemcpyAsync(p.p, z.p, sizeof(double) * nl, hipMemcpyDeviceToDevice, st));
+      prev_ = last_dual;
+    }
+    if (last_dual < tol) return FETIDP_SUCCESS;
+    for (int k = 0; k < max_it; ++k) {
+      apply_F<T>(W, p.p, q.p);
+      const double* a1[1] = {p.p};
+      const double* b1[1] = {q.p};
+      const double pq = dots(1, a1, b1, nl, o);
+      if (!(std::fabs(pq) >= 1e-300)) return FETIDP_NUMERICAL_ERROR;
+      const double alpha = rz_ / pq;
+      launch_axpy2(nl, alpha, p.p, q.p, lam.p, r.p, st);
+      apply_M<T>(P, r.p, z.p);
+      const double* a[2] = {r.p, r.p};
+      const double* b[2] = {r.p, z.p};
+      dots(2, a, b, nl, o);
+      ++its;
+      last_dual = std::sqrt(o[0]) / dn;
+      hist.push_back(last_dual);
+      if (!std::isfinite(last_dual)) return FETIDP_NUMERICAL_ERROR;
+      if (last_dual < tol) return FETIDP_SUCCESS;
+      if (its > 10 && last_dual > prev_ * 1000.0) return FETIDP_DIVERGED;
+      const double beta = o[1] / rz_;
+      rz_ = o[1];
+      prev_ = last_dual;
+      launch_xpby(nl, z.p, beta, p.p, st);
+    }
+    return FETIDP_MAX_ITER_REACHED;
+  }
+  double rz_ = 0.0, prev_ = 0.0;
+
+  template <typename T>
+  int solve_impl(const DBuf<T>& pool, const DBuf<T>& W, const DBuf<T>& P, const double* h_lam0,
+                 fetidp_stats& S) {
+    const int64_t nl = topo.n_lam, nf = topo.n_free;
+    hist.clear();
+    int its = 0, status = FETIDP_SUCCESS;
+    double dn = 0.0, res = 0.0;
+    if (!fp32()) {
+      condense<T>(pool, W, f_glob.p);
+      if (h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
+      else lam.zero(st);
+      double tol = cfg.dual_tolerance;
+      bool fresh = true;
+      for (int round = 0; round < 6; ++round) {
+        status = pcpg<T>(W, P, tol, cfg.max_iterations - its, fresh, its, dn);
+        fresh = false;
+        recover<T>(pool, W, x.p);
+        res = global_residual(x.p, f_glob.p);
+        if (status != FETIDP_SUCCESS || res <= cfg.tolerance) break;
+        tol *= 0.1;
+      }
+      S.refinements = 0;
+      HIPCHECK(hipMemcpyAsync(xtot.p, x.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
+    } else {
+      // FP64 iterative refinement around the FP32 FETI-DP solve
+      xtot.zero(st);
+      HIPCHECK(hipMemcpyAsync(f_cur.p, f_glob.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
+      int k = 0;
+      res = 1.0;
+      for (k = 0; k < cfg.max_refinements; ++k) {
+        condense<T>(pool, W, f_cur.p);
+        if (k == 0 && h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
+        else lam.zero(st);
+        status = pcpg<T>(W, P, cfg.dual_tolerance, cfg.max_iterations - its, true, its, dn);
+        if (status == FETIDP_NUMERICAL_ERROR || status == FETIDP_DIVERGED) break;
+        recover<T>(pool, W, x.p);
+        launch_axpy(nf, 1.0, x.p, xtot.p, st);
+        res = global_residual(xtot.p, f_glob.p);
+        HIPCHECK(hipMemcpyAsync(f_cur.p, rglob.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
+        if (res <= cfg.tolerance) { status = FETIDP_SUCCESS; break; }
+        if (its >= cfg.max_iterations) { status = FETIDP_MAX_ITER_REACHED; break; }
+      }
+      if (res > cfg.tolerance && status == FETIDP_SUCCESS) status = FETIDP_MAX_ITER_REACHED;
+      S.refinements = k + 1;
+    }
+    if (status == FETIDP_SUCCESS && res > cfg.tolerance) status = FETIDP_MAX_ITER_REACHED;
+    S.iterations = its;
+    S.residual_norm = res;
+    S.dual_residual = last_dual;
+    S.initial_dual_residual = init_dual;
+    return status;
+  }
+
+  int solve(const double* h_lam0, double* h_x, fetidp_stats& S) {
+    if (!factored) throw InputError("solve before factor");
+    const double t0 = now_ms();
+    int status = fp32() ? solve_impl<float>(pool32, W32, P32, h_lam0, S)
+                        : solve_impl<double>(pool64, W64, P64, h_lam0, S);
+    if (h_x) HIPCHECK(hipMemcpyAsync(h_x, xtot.p, sizeof(double) * topo.n_free, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    S.solve_ms = now_ms() - t0;
+    S.status = status;
+    return status;
+  }
+
+  template <typename F64, typename F32>
+  void with_ops(F64 f64, F32 f32) {
+    if (!factored) throw InputError("operators need fetidp_factor first");
+    if (fp32()) f32(); else f64();
+  }
+};
+
+namespace {
+int32_t guard(fetidp_handle h, const std::function<int32_t()>& fn) {
+  if (!h) return FETIDP_INVALID_INPUT;
+  try {
+    h->err.clear();
+    return fn();
+  } catch (const InputError& e) {
+    h->err = e.what();
+    return FETIDP_INVALID_INPUT;
+  } catch (const NumericError& e) {
+    h->err = e.what();
+    return FETIDP_NUMERICAL_ERROR;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return FETIDP_NUMERICAL_ERROR;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void fetidp_config_default(fetidp_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->precision = FETIDP_FP64;
+  cfg->max_iterations = 1000;
+  cfg->tolerance = 1e-8;
+  cfg->dual_tolerance = 1e-10;
+  cfg->max_refinements = 8;
+  cfg->verbose = 0;
+}
+
+fetidp_handle fetidp_create(const fetidp_config* cfg) {
+  try {
+    auto* h = new fetidp_solver();
+    if (cfg) h->cfg = *cfg;
+    else fetidp_config_default(&h->cfg);
+    if (h->cfg.precision != FETIDP_FP64 && h->cfg.precision != FETIDP_FP32_REFINE) {
+      delete h;
+      return nullptr;
+    }
+    if (h->cfg.max_iterations <= 0) h->cfg.max_iterations = 1000;
+    if (!(h->cfg.tolerance > 0)) h->cfg.tolerance = 1e-8;
+    if (!(h->cfg.dual_tolerance > 0)) h->cfg.dual_tolerance = 1e-10;
+    if (h->cfg.max_refinements <= 0) h->cfg.max_refinements = 8;
+    return h;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+int32_t fetidp_setup(fetidp_handle h, int64_t n_nodes, const double* xyz, int64_t n_tets,
+                     const int32_t* tet2node, const int32_t* tet_material, int32_t n_mat,
+                     const double* nu, const double* kappa, const int32_t* tet_subdomain,
+                     int64_t n_pec, const int32_t* pec_edge_nodes) {
+  return guard(h, [&] {
+    h->setup(n_nodes, xyz, n_tets, tet2node, tet_material, n_mat, nu, kappa, tet_subdomain, n_pec,
+             pec_edge_nodes);
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_set_materials(fetidp_handle h, int32_t n_mat, const double* nu, const double* kappa) {
+  return guard(h, [&] {
+    h->set_materials(n_mat, nu, kappa);
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_set_rhs_tet_J(fetidp_handle h, const double* jq) {
+  return guard(h, [&] {
+    h->set_rhs_J(jq);
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_set_rhs_free(fetidp_handle h, const double* f) {
+  return guard(h, [&] {
+    h->set_rhs_free(f);
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_factor(fetidp_handle h) {
+  return guard(h, [&] {
+    h->last_factor_ms = h->factor();
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_solve(fetidp_handle h, const double* lambda0, double* x_free, fetidp_stats* stats) {
+  fetidp_stats S{};
+  int32_t rc = guard(h, [&] { return int32_t(h->solve(lambda0, x_free, S)); });
+  if (h) {
+    S.status = rc;
+    S.factor_ms = h->last_factor_ms;
+    S.symbolic_ms = h->symbolic_ms;
+  }
+  if (stats) *stats = S;
+  return rc;
+}
+
+int32_t fetidp_get_sizes(fetidp_handle h, int64_t* out) {
+  return guard(h, [&] {
+    if (!h->setup_done || !out) throw InputError("get_sizes before setup");
+    const Topology& T = h->topo;
+    out[0] = T.n_edges;
+    out[1] = T.n_free;
+    out[2] = T.n_lam;
+    out[3] = T.n_c;
+    out[4] = T.n_sub;
+    out[5] = T.n_tets;
+    out[6] = T.pool_elems;
+    out[7] = T.w_elems + T.p_elems;
+    out[8] = h->plans.coarse_ncp;
+    out[9] = h->plans.max_m;
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_get_topology(fetidp_handle h, int32_t* tet_free, int32_t* edge_nodes,
+                            int32_t* lam_free, int32_t* primal_free) {
+  return guard(h, [&] {
+    if (!h->setup_done) throw InputError("get_topology before setup");
+    const Topology& T = h->topo;
+    if (tet_free) std::memcpy(tet_free, T.tet_free.data(), sizeof(int32_t) * T.tet_free.size());
+    if (edge_nodes) std::memcpy(edge_nodes, T.edge_nodes.data(), sizeof(int32_t) * T.edge_nodes.size());
+    if (lam_free) std::memcpy(lam_free, T.free_of_lam.data(), sizeof(int32_t) * T.free_of_lam.size());
+    if (primal_free) std::memcpy(primal_free, T.free_of_c.data(), sizeof(int32_t) * T.free_of_c.size());
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int64_t fetidp_get_global_nnz(fetidp_handle h) {
+  if (!h || !h->setup_done) return -1;
+  return int64_t(h->plans.csr.col.size());
+}
+
+int32_t fetidp_get_global_system(fetidp_handle h, int32_t* row_ptr, int32_t* col, double* val,
+                                 double* f) {
+  return guard(h, [&] {
+    if (!h->factored) throw InputError("get_global_system needs fetidp_factor first");
+    const CsrPlan& C = h->plans.csr;
+    if (row_ptr) std::memcpy(row_ptr, C.row_ptr.data(), sizeof(int32_t) * C.row_ptr.size());
+    if (col) std::memcpy(col, C.col.data(), sizeof(int32_t) * C.col.size());
+    if (val) HIPCHECK(hipMemcpy(val, h->csr_val.p, sizeof(double) * C.col.size(), hipMemcpyDeviceToHost));
+    if (f) HIPCHECK(hipMemcpy(f, h->f_glob.p, sizeof(double) * h->topo.n_free, hipMemcpyDeviceToHost));
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+static int32_t dual_io(fetidp_handle h, const double* in, double* out, int which) {
+  return guard(h, [&] {
+    if (!in || !out) throw InputError("null vector");
+    const int64_t nl = h->topo.n_lam;
+    h->with_ops(
+        [&] {
+          HIPCHECK(hipMemcpy(h->p.p, in, sizeof(double) * nl, hipMemcpyHostToDevice));
+          if (which == 0) h->apply_F<double>(h->W64, h->p.p, h->q.p);
+          else h->apply_M<double>(h->P64, h->p.p, h->q.p);
+        },
+        [&] {
+          HIPCHECK(hipMemcpy(h->p.p, in, sizeof(double) * nl, hipMemcpyHostToDevice));
+          if (which == 0) h->apply_F<float>(h->W32, h->p.p, h->q.p);
+          else h->apply_M<float>(h->P32, h->p.p, h->q.p);
+        });
+    HIPCHECK(hipStreamSynchronize(h->st));
+    HIPCHECK(hipMemcpy(out, h->q.p, sizeof(double) * nl, hipMemcpyDeviceToHost));
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_apply_dual_operator(fetidp_handle h, const double* p, double* q) { return dual_io(h, p, q, 0); }
+int32_t fetidp_apply_preconditioner(fetidp_handle h, const double* r, double* z) { return dual_io(h, r, z, 1); }
+
+int32_t fetidp_get_dual_rhs(fetidp_handle h, double* d) {
+  return guard(h, [&] {
+    if (!d) throw InputError("null vector");
+    h->with_ops([&] { h->condense<double>(h->pool64, h->W64, h->f_glob.p); },
+                [&] { h->condense<float>(h->pool32, h->W32, h->f_glob.p); });
+    HIPCHECK(hipStreamSynchronize(h->st));
+    HIPCHECK(hipMemcpy(d, h->dv.p, sizeof(double) * h->topo.n_lam, hipMemcpyDeviceToHost));
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_get_lambda(fetidp_handle h, double* lam) {
+  return guard(h, [&] {
+    if (!lam || !h->setup_done) throw InputError("get_lambda: bad call");
+    HIPCHECK(hipMemcpy(lam, h->lam.p, sizeof(double) * h->topo.n_lam, hipMemcpyDeviceToHost));
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_get_history(fetidp_handle h, double* hist, int32_t cap) {
+  if (!h) return -1;
+  const int32_t n = int32_t(h->hist.size());
+  for (int32_t i = 0; i < n && i < cap && hist; ++i) hist[i] = h->hist[i];
+  return n;
+}
+
+int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S) {
+  return guard(h, [&] {
+    if (!h->factored || !S) throw InputError("get_coarse_matrix needs fetidp_factor first");
+    const int64_t nc = h->topo.n_c, ncp = h->plans.coarse_ncp;
+    // the coarse pool keeps L after the factorization; rebuild S~ by re-gathering
+    DBuf<double> tmp;
+    tmp.alloc(h->plans.coarse_pool_elems);
+    tmp.zero(h->st);
+    if (h->fp32())
+      launch_gather_pool<float>(h->plans.coarse.size(), h->coarse_target.p, h->coarse_ptr.p,
+                                h->coarse_srcpos.p, h->pool32.p, tmp.p, h->st);
+    else
+      launch_gather_pool<double>(h->plans.coarse.size(), h->coarse_target.p, h->coarse_ptr.p,
+                                 h->coarse_srcpos.p, h->pool64.p, tmp.p, h->st);
+    HIPCHECK(hipStreamSynchronize(h->st));
+    std::vector<double> pool(h->plans.coarse_pool_elems);
+    HIPCHECK(hipMemcpy(pool.data(), tmp.p, sizeof(double) * pool.size(), hipMemcpyDeviceToHost));
+    for (int64_t a = 0; a < nc; ++a)
+      for (int64_t b = 0; b < nc; ++b) {
+        const int64_t hi = std::max(a, b), lo = std::min(a, b);
+        S[a * nc + b] = pool[pool_pos(0, hi, lo)];
+      }
+    (void)ncp;
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+const char* fetidp_last_error(fetidp_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+void fetidp_destroy(fetidp_handle h) { delete h; }
+
+int32_t emag_nedelec_integrate(int64_t n_tets, const double* d_xyz, const int32_t* d_tet2node,
+                               const int32_t* d_tet_material, const double* d_nu,
+                               const double* d_kappa, const double* d_jq, double* d_Ke,
+                               double* d_Me, double* d_Fe, void* stream) {
+  if (n_tets <= 0 || !d_xyz || !d_tet2node || !d_tet_material || !d_nu || !d_kappa || !d_Ke || !d_Me)
+    return FETIDP_INVALID_INPUT;
+  if (d_jq && !d_Fe) return FETIDP_INVALID_INPUT;
+  launch_integrate(n_tets, d_xyz, d_tet2node, d_tet_material, d_nu, d_kappa, d_jq, 0, d_Ke, d_Me,
+                   d_Fe, static_cast<hipStream_t>(stream));
+  return hipGetLastError() == hipSuccess ? FETIDP_SUCCESS : FETIDP_NUMERICAL_ERROR;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Launch wrappers for the CDNA4 kernels of the FETI-DP Nedelec solver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "topology.h"
+
+namespace emag {
+
+
+// Element integration: per tet the packed upper triangle (21) of
+// K_e (+ M_e when mode == 1) and of M_e (mode 0), and F_e (6) if jq != null.
+// Output arrays are structure-of-arrays: entry k of tet t at [k * nt + t].
+void launch_integrate(int64_t nt, const double* xyz, const int32_t* t2n, const int32_t* mat,
+                      const double* nu, const double* kappa, const double* jq, int mode,
+                      double* outA, double* outM, double* outF, hipStream_t st);
+
+// out[target[i]] = sum_{c in ptr[i]..ptr[i+1]} (+-) elem[src[c] >> 1]  (sign = src & 1)
+void launch_assemble_f64(int64_t n, const int64_t* target, const int64_t* ptr, const int32_t* src,
+                         const double* elem, double* out, hipStream_t st);
+void launch_assemble_f32(int64_t n, const int64_t* target, const int64_t* ptr, const int32_t* src,
+                         const double* elem, float* out, hipStream_t st);
+
+// out[target[i]] = sum_{c} pool[srcpos[c]]  (coarse assembly from factor pools)
+template <typename T>
+void launch_gather_pool(int64_t n, const int64_t* target, const int64_t* ptr, const int64_t* srcpos,
+                        const T* pool, double* out, hipStream_t st);
+
+// mode 0: Schur update list; 1: column step below the diagonal (update + trsm with
+// the inverse diagonal tile); 2: column step diagonal (update + Cholesky + inverse).
+template <typename T>
+void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base, T* pool,
+                      T* inv, int* fail, hipStream_t st);
+
+template <typename T>
+void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, T* P, hipStream_t st);
+template <typename T>
+void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool, T* W,
+                      hipStream_t st);
+void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, double* sinv, hipStream_t st);
+
+// Forward solve y = L_rr^-1 v_r in place; then v_c <- v_c - L_cr y, v_a <- L_a r y.
+template <typename T>
+void launch_forward(int ns, const DevSub* subs, const T* pool, double* vec, hipStream_t st);
+// Interior solve: out_i = L_ii^-T (y_i - L_{D,c; i}^T U) with y = vec (forward result).
+template <typename T>
+void launch_interior(int ns, const DevSub* subs, const T* pool, const double* vec, const double* U,
+                     double* out, hipStream_t st);
+// rho[slot] = pool diagonal at each D slot; stiffness scaling weights and 0.5 means.
+void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void* pool, int is_f32,
+                    double* rho, const int64_t* lam_pos, const double* sgn, double* sgnD,
+                    double* half, hipStream_t st);
+
+// y_s = M_s x_s with x_s[k] = scale(slot) * v[lam[slot]], rows_s = ndp (+ ncp if with_c)
+template <typename T>
+void launch_gemv_dual(int ns, int max_rows, const DevSub* subs, const T* M, int which,
+                      const int32_t* d_lam, const double* d_scale, const double* v, double* y,
+                      hipStream_t st, int max_ndp);
+// out[slot] = alpha*y[...] + beta*(Phi u)[k] + gamma*vec[o_a + k]
+template <typename T>
+void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* y,
+                         const int32_t* c_loc, const double* uc, const double* vec, double alpha,
+                         double beta, double gamma, double* out, hipStream_t st);
+// out[l] = sum over the two sides of scale[slot] * w[slot]
+void launch_lambda_scatter(int64_t nl, const int64_t* lam_pos, const double* scale, const double* w,
+                           double* out, hipStream_t st);
+void launch_coarse_gather(int64_t nc, const int64_t* ptr, const int64_t* pos, const double* src,
+                          const double* add, double* out, hipStream_t st);
+void launch_dense_gemv(int64_t n, int64_t ld, const double* A, const double* x, double* y,
+                       hipStream_t st);
+void launch_fill_U(int ns, int max_m, const DevSub* subs, const int32_t* d_lam, const double* ubar,
+                   const int32_t* c_loc, const double* uc, double* U, hipStream_t st);
+void launch_split_rhs(int64_t n, const int32_t* vdof, const double* vw, const double* f, double* vec,
+                      hipStream_t st);
+void launch_gather_global(int64_t nf, const int8_t* kind, const int64_t* a, const double* vec,
+                          const double* ubar, const double* uc, double* x, hipStream_t st);
+void launch_csr_residual(int64_t n, const int32_t* rp, const int32_t* col, const double* val,
+                         const double* x, const double* f, double* r, hipStream_t st);
+// partial[b] = sum over block b of a[i]*b[i] for up to 3 pairs; nblk partial sums each
+constexpr int kDotBlocks = 256;
+void launch_dots(int64_t n, int npairs, const double* const* a, const double* const* b,
+                 double* partial, hipStream_t st);
+void launch_axpy2(int64_t n, double alpha, const double* p, const double* q, double* lam, double* r,
+                  hipStream_t st);
+void launch_xpby(int64_t n, const double* z, double beta, double* p, hipStream_t st);
+void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t st);
+
+}  // namespace emag

@@ -1,0 +1,898 @@
+// CDNA4 (gfx950) kernels of the FETI-DP Nedelec solver.
+//
+// Data layout in HBM
+//  * element data: structure-of-arrays, entry k of tet t at [k * nt + t]
+//    (coalesced per-thread-per-tet writes); one trailing 1.0 constant.
+//  * factor pool: per matrix, the lower triangle of 64x64 tiles, tile (I,J)
+//    (I >= J) at base + (I(I+1)/2 + J) * 4096, each tile column-major.
+//  * iteration operators W_s = [Z; Phi^T] and P_s = S_DD: row-major dense.
+//  * vectors: FP64 always.
+//
+// MFMA lane layout used (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32):
+//   A operand  lane l -> A[l % 16][l / 16]           (16 x 4)
+//   B operand  lane l -> B[l / 16][l % 16]           (4 x 16)
+//   D          lane l, reg r -> D[4 * (l / 16) + r][l % 16]
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace emag {
+
+__device__ __forceinline__ int64_t tile_off(int I, int J) {
+  return ((int64_t)I * (I + 1) / 2 + J) * (int64_t)kTileElems;
+}
+
+// ---------------------------------------------------------------------------
+// Element integration (closed form, FP64)
+// ---------------------------------------------------------------------------
+__constant__ int c_ref_edge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+__global__ __launch_bounds__(256) void k_integrate(int64_t nt, const double* __restrict__ xyz,
+                                                   const int32_t* __restrict__ t2n,
+                                                   const int32_t* __restrict__ mat,
+                                                   const double* __restrict__ nu,
+                                                   const double* __restrict__ kappa,
+                                                   const double* __restrict__ jq, int mode,
+                                                   double* __restrict__ outA,
+                                                   double* __restrict__ outM,
+                                                   double* __restrict__ outF) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int4 nd = reinterpret_cast<const int4*>(t2n)[t];
+  const int ids[4] = {nd.x, nd.y, nd.z, nd.w};
+  double X[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) X[a][d] = xyz[3 * (int64_t)ids[a] + d];
+  // columns of J: c_k = x_k - x_3; rows of J^-1: (c1 x c2, c2 x c0, c0 x c1) / det
+  double c[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[k][d] = X[k][d] - X[3][d];
+  double G[4][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double* u = c[(k + 1) % 3];
+    const double* v = c[(k + 2) % 3];
+    G[k][0] = u[1] * v[2] - u[2] * v[1];
+    G[k][1] = u[2] * v[0] - u[0] * v[2];
+    G[k][2] = u[0] * v[1] - u[1] * v[0];
+  }
+  const double det = c[0][0] * G[0][0] + c[0][1] * G[0][1] + c[0][2] * G[0][2];
+  const double idet = 1.0 / det;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) G[k][d] *= idet;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) G[3][d] = -(G[0][d] + G[1][d] + G[2][d]);
+  const double vol = fabs(det) * (1.0 / 6.0);
+  const int m = mat[t];
+  const double ka = kappa[m] * vol * (1.0 / 20.0);
+  const double kn = nu[m] * vol;
+  double cu[6][3];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const double* ga = G[c_ref_edge[e][0]];
+    const double* gb = G[c_ref_edge[e][1]];
+    cu[e][0] = 2.0 * (gb[1] * ga[2] - gb[2] * ga[1]);
+    cu[e][1] = 2.0 * (gb[2] * ga[0] - gb[0] * ga[2]);
+    cu[e][2] = 2.0 * (gb[0] * ga[1] - gb[1] * ga[0]);
+  }
+  double g[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) g[p][q] = G[p][0] * G[q][0] + G[p][1] * G[q][1] + G[p][2] * G[q][2];
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+#pragma unroll
+    for (int j = i; j < 6; ++j, ++k) {
+      const int a = c_ref_edge[i][0], b = c_ref_edge[i][1];
+      const int cc = c_ref_edge[j][0], d = c_ref_edge[j][1];
+      const double kij = kn * (cu[i][0] * cu[j][0] + cu[i][1] * cu[j][1] + cu[i][2] * cu[j][2]);
+      const double mij = ka * ((1 + (b == d)) * g[a][cc] - (1 + (b == cc)) * g[a][d] -
+                               (1 + (a == d)) * g[b][cc] + (1 + (a == cc)) * g[b][d]);
+      if (mode == 1) {
+        outA[k * nt + t] = kij + mij;
+      } else {
+        outA[k * nt + t] = kij;
+        outM[k * nt + t] = mij;
+      }
+    }
+  }
+  if (jq != nullptr) {
+    const double qa = 0.1381966011250105151795413165634361882280;  // (5 - sqrt 5) / 20
+    const double qb = 0.5854101966249684544613760503096914353161;  // (5 + 3 sqrt 5) / 20
+    const double* J = jq + 12 * t;
+    double F[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double jx = J[3 * q], jy = J[3 * q + 1], jz = J[3 * q + 2];
+      double gj[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) gj[p] = G[p][0] * jx + G[p][1] * jy + G[p][2] * jz;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) {
+        const int a = c_ref_edge[e][0], b = c_ref_edge[e][1];
+        const double la = (a == q) ? qb : qa, lb = (b == q) ? qb : qa;
+        F[e] += lb * gj[a] - la * gj[b];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 6; ++e) outF[e * nt + t] = F[e] * (vol * 0.25);
+  }
+}
+
+void launch_integrate(int64_t nt, const double* xyz, const int32_t* t2n, const int32_t* mat,
+                      const double* nu, const double* kappa, const double* jq, int mode,
+                      double* outA, double* outM, double* outF, hipStream_t st) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_integrate, dim3((nt + 255) / 256), dim3(256), 0, st, nt, xyz, t2n, mat, nu,
+                     kappa, jq, mode, outA, outM, outF);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic assembly: each output entry sums its contribution list in order
+// ---------------------------------------------------------------------------
+template <typename TO>
+__global__ __launch_bounds__(256) void k_assemble(int64_t n, const int64_t* __restrict__ target,
+                                                  const int64_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ src,
+                                                  const double* __restrict__ elem, TO* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int64_t c = ptr[i]; c < ptr[i + 1]; ++c) {
+    const int32_t code = src[c];
+    const double v = elem[(uint32_t)code >> 1];
+    s += (code & 1) ? -v : v;
+  }
+  out[target ? target[i] : i] = (TO)s;
+}
+
+void launch_assemble_f64(int64_t n, const int64_t* target, const int64_t* ptr, const int32_t* src,
+                         const double* elem, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_assemble<double>, dim3((n + 255) / 256), dim3(256), 0, st, n, target, ptr,
+                     src, elem, out);
+}
+void launch_assemble_f32(int64_t n, const int64_t* target, const int64_t* ptr, const int32_t* src,
+                         const double* elem, float* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_assemble<float>, dim3((n + 255) / 256), dim3(256), 0, st, n, target, ptr,
+                     src, elem, out);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_pool(int64_t n, const int64_t* __restrict__ target,
+                                                     const int64_t* __restrict__ ptr,
+                                                     const int64_t* __restrict__ srcpos,
+                                                     const T* __restrict__ pool, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int64_t c = ptr[i]; c < ptr[i + 1]; ++c) {
+    const int64_t p = srcpos[c];
+    s += p < 0 ? 1.0 : (double)pool[p];
+  }
+  out[target[i]] = s;
+}
+
+template <typename T>
+void launch_gather_pool(int64_t n, const int64_t* target, const int64_t* ptr, const int64_t* srcpos,
+                        const T* pool, double* out, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_pool<T>, dim3((n + 255) / 256), dim3(256), 0, st, n, target, ptr,
+                     srcpos, pool, out);
+}
+template void launch_gather_pool<double>(int64_t, const int64_t*, const int64_t*, const int64_t*,
+                                         const double*, double*, hipStream_t);
+template void launch_gather_pool<float>(int64_t, const int64_t*, const int64_t*, const int64_t*,
+                                        const float*, double*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Batched tile Cholesky on MFMA
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kLd = 65;  // padded LDS leading dimension for a 64x64 tile
+
+// acc (32x32 per wave, 2x2 MFMA blocks) += A_p,k * B_k,q over one 64-deep tile,
+// A operand read from Ap (column-major 64x64, element (p,k) at k*lda + p),
+// B operand from Bq (element (q,k) at k*ldb + q).
+template <typename T, int LDA, int LDB>
+__device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[2][2], const T* Ap,
+                                         const T* Bq, int pb0, int qb0, int lr, int lk) {
+#pragma unroll 4
+  for (int kk = 0; kk < 64; kk += 4) {
+    const int kc = kk + lk;
+    const T a0 = Ap[kc * LDA + pb0 + lr];
+    const T a1 = Ap[kc * LDA + pb0 + 16 + lr];
+    const T b0 = Bq[kc * LDB + qb0 + lr];
+    const T b1 = Bq[kc * LDB + qb0 + 16 + lr];
+    acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+  }
+}
+
+// MODE 0: C_IJ -= sum_K L_IK L_JK^T                     (Schur updates)
+// MODE 1: same, then X = C L_JJ^-T with the inverse tile (column step, I > J)
+// MODE 2: same, then Cholesky of the diagonal tile and its inverse (I == J)
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict__ work,
+                                                     const int64_t* __restrict__ base,
+                                                     T* __restrict__ pool, T* __restrict__ inv,
+                                                     int* __restrict__ fail) {
+  using acc_t = typename Mfma<T>::acc_t;
+  const TileWork w = work[blockIdx.x];
+  const int64_t b = base[w.mat];
+  T* C = pool + b + tile_off(w.I, w.J);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  // D = C^T: rows p of D are columns j of C, columns q of D are rows i of C.
+  const int pb0 = (wave >> 1) * 32, qb0 = (wave & 1) * 32;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = acc_t{0, 0, 0, 0};
+  for (int K = w.k0; K < w.k1; ++K)
+    tile_mma<T, 64, 64>(acc, pool + b + tile_off(w.J, K), pool + b + tile_off(w.I, K), pb0, qb0, lr, lk);
+
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = pb0 + 16 * x + 4 * lk + r, i = qb0 + 16 * y + lr;
+          C[j * 64 + i] -= acc[x][y][r];
+        }
+  } else {
+    __shared__ T s[64 * kLd];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = pb0 + 16 * x + 4 * lk + r, i = qb0 + 16 * y + lr;
+          s[j * kLd + i] = C[j * 64 + i] - acc[x][y][r];
+        }
+    __syncthreads();
+    if constexpr (MODE == 1) {
+      // X[i][j] = sum_k C'[i][k] Linv[j][k]:  D2[p=j][q=i], A op = Linv (p,k), B op = C' (q,k)
+      const T* Li = inv + (int64_t)w.mat * kTileElems;
+      acc_t a2[2][2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) a2[x][y] = acc_t{0, 0, 0, 0};
+      tile_mma<T, 64, kLd>(a2, Li, s, pb0, qb0, lr, lk);
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int j = pb0 + 16 * x + 4 * lk + r, i = qb0 + 16 * y + lr;
+            C[j * 64 + i] = a2[x][y][r];
+          }
+    } else {
+      // Cholesky of the 64x64 tile in LDS (lower), right-looking, 256 threads
+      __shared__ T si[64 * kLd];
+      int bad = 0;
+      for (int k = 0; k < 64; ++k) {
+        __syncthreads();
+        T piv = s[k * kLd + k];
+        T d;
+        if (!(piv > T(0)) || !(piv < T(3.0e38))) {
+          bad = 1;
+          d = T(1);
+        } else {
+          d = sqrt(piv);
+        }
+        __syncthreads();
+        if (tid == 0) s[k * kLd + k] = d;
+        for (int i = k + 1 + tid; i < 64; i += 256) s[k * kLd + i] /= d;
+        __syncthreads();
+        const int i = lane;
+        for (int j = k + 1 + wave; j < 64; j += 4)
+          if (i >= j) s[j * kLd + i] -= s[k * kLd + i] * s[k * kLd + j];
+      }
+      __syncthreads();
+      if (bad && tid == 0) atomicOr(fail + w.mat, 1);
+      // inverse of the lower factor, one column per thread of wave 0
+      if (tid < 64) {
+        const int cidx = tid;
+        for (int i = 0; i < cidx; ++i) si[cidx * kLd + i] = T(0);
+        T y = T(1) / s[cidx * kLd + cidx];
+        si[cidx * kLd + cidx] = y;
+        for (int i = cidx + 1; i < 64; ++i) {
+          T acc1 = T(0);
+          for (int k = cidx; k < i; ++k) acc1 += s[k * kLd + i] * si[cidx * kLd + k];
+          si[cidx * kLd + i] = -acc1 / s[i * kLd + i];
+        }
+      }
+      __syncthreads();
+      T* Li = inv + (int64_t)w.mat * kTileElems;
+      for (int e = tid; e < kTileElems; e += 256) {
+        const int i = e & 63, j = e >> 6;
+        C[e] = (i >= j) ? s[j * kLd + i] : T(0);
+        Li[e] = si[j * kLd + i];
+      }
+    }
+  }
+}
+
+template <typename T>
+void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base, T* pool,
+                      T* inv, int* fail, hipStream_t st) {
+  if (n <= 0) return;
+  if (mode == 0)
+    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, mat_base, pool, inv, fail);
+  else if (mode == 1)
+    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, mat_base, pool, inv, fail);
+  else
+    hipLaunchKernelGGL((k_tile_update<T, 2>), dim3(n), dim3(256), 0, st, w, mat_base, pool, inv, fail);
+}
+template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, double*,
+                                       double*, int*, hipStream_t);
+template void launch_tile_mode<float>(int, const TileWork*, int64_t, const int64_t*, float*, float*,
+                                      int*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Operator extraction after the partial factorization
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T pool_sym(const T* P, int a, int b) {
+  const int hi = a > b ? a : b, lo = a > b ? b : a;
+  return P[tile_off(hi >> 6, lo >> 6) + (lo & 63) * 64 + (hi & 63)];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_extract_P(const DevSub* __restrict__ subs,
+                                                   const T* __restrict__ pool, T* __restrict__ P) {
+  const DevSub S = subs[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)S.ndp * S.ndp) return;
+  const int a = int(e / S.ndp), b = int(e % S.ndp);
+  P[S.p_off + e] = pool_sym(pool + S.pool_off, S.o_d + a, S.o_d + b);
+}
+
+template <typename T>
+void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, T* P, hipStream_t st) {
+  const int64_t n = (int64_t)max_ndp * max_ndp;
+  if (n <= 0 || ns <= 0) return;
+  hipLaunchKernelGGL(k_extract_P<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, P);
+}
+template void launch_extract_P<double>(int, int, const DevSub*, const double*, double*, hipStream_t);
+template void launch_extract_P<float>(int, int, const DevSub*, const float*, float*, hipStream_t);
+
+// W = [Z; Phi^T] from the trailing block: Z = -A(D',D'), Phi = -A(D',c)
+template <typename T>
+__global__ __launch_bounds__(256) void k_extract_W(const DevSub* __restrict__ subs,
+                                                   const T* __restrict__ pool, T* __restrict__ W) {
+  const DevSub S = subs[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rows = S.ndp + S.ncp;
+  if (e >= (int64_t)rows * S.ndp) return;
+  const int row = int(e / S.ndp), k = int(e % S.ndp);
+  const T* P = pool + S.pool_off;
+  T v;
+  if (row < S.ndp)
+    v = -pool_sym(P, S.o_a + row, S.o_a + k);
+  else
+    v = -pool_sym(P, S.o_a + k, S.o_c + (row - S.ndp));
+  W[S.w_off + e] = v;
+}
+
+template <typename T>
+void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool, T* W,
+                      hipStream_t st) {
+  const int64_t n = (int64_t)max_rows * max_ndp;
+  if (n <= 0 || ns <= 0) return;
+  hipLaunchKernelGGL(k_extract_W<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, W);
+}
+template void launch_extract_W<double>(int, int, int, const DevSub*, const double*, double*,
+                                       hipStream_t);
+template void launch_extract_W<float>(int, int, int, const DevSub*, const float*, float*, hipStream_t);
+
+__global__ __launch_bounds__(256) void k_extract_sinv(int64_t nc, int64_t ncp,
+                                                      const double* __restrict__ cpool,
+                                                      double* __restrict__ sinv) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nc * nc) return;
+  const int a = int(e / nc), b = int(e % nc);
+  sinv[e] = -pool_sym(cpool, int(ncp) + a, int(ncp) + b);
+}
+
+void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, double* sinv, hipStream_t st) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_extract_sinv, dim3((nc * nc + 255) / 256), dim3(256), 0, st, nc, ncp, cpool, sinv);
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves with the stored partial factor (one workgroup per subdomain)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_forward(const DevSub* __restrict__ subs,
+                                                 const T* __restrict__ pool, double* __restrict__ vec) {
+  const DevSub S = subs[blockIdx.x];
+  const T* P = pool + S.pool_off;
+  double* v = vec + S.vec_off;
+  __shared__ double part[4][64];
+  __shared__ T dg[64 * kLd];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int J = 0; J < S.Tr; ++J) {
+    double acc = 0.0;
+    for (int K = wv; K < J; K += 4) {
+      const T* L = P + tile_off(J, K);
+      const double* y = v + K * 64;
+#pragma unroll 8
+      for (int c = 0; c < 64; ++c) acc += (double)L[c * 64 + lane] * y[c];
+    }
+    part[wv][lane] = acc;
+    const T* D = P + tile_off(J, J);
+    for (int e = tid; e < kTileElems; e += 256) dg[(e >> 6) * kLd + (e & 63)] = D[e];
+    __syncthreads();
+    if (wv == 0) {
+      double t = v[J * 64 + lane] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
+      double y = 0.0;
+      for (int c = 0; c < 64; ++c) {
+        const double yc = __shfl(t, c) / (double)dg[c * kLd + c];
+        if (lane == c) y = yc;
+        if (lane > c) t -= (double)dg[c * kLd + lane] * yc;
+      }
+      v[J * 64 + lane] = y;
+    }
+    __syncthreads();
+  }
+  // trailing rows: c rows get f_c - L_cr y, D' rows get L_D'r y
+  for (int I = S.Tr + wv; I < S.T; I += 4) {
+    const int k0 = I >= S.Tc ? S.Ti : 0;
+    double acc = 0.0;
+    for (int K = k0; K < S.Tr; ++K) {
+      const T* L = P + tile_off(I, K);
+      const double* y = v + K * 64;
+#pragma unroll 8
+      for (int c = 0; c < 64; ++c) acc += (double)L[c * 64 + lane] * y[c];
+    }
+    if (I < S.Tc)
+      v[I * 64 + lane] -= acc;
+    else
+      v[I * 64 + lane] = acc;
+  }
+}
+
+template <typename T>
+void launch_forward(int ns, const DevSub* subs, const T* pool, double* vec, hipStream_t st) {
+  if (ns <= 0) return;
+  hipLaunchKernelGGL(k_forward<T>, dim3(ns), dim3(256), 0, st, subs, pool, vec);
+}
+template void launch_forward<double>(int, const DevSub*, const double*, double*, hipStream_t);
+template void launch_forward<float>(int, const DevSub*, const float*, double*, hipStream_t);
+
+// sum over column c of tile L (64x64 col-major) times u (64): thread t -> column t>>2,
+// rows (t&3)*16 .. +16; reduced across the 4 lanes sharing a column.
+template <typename T>
+__device__ __forceinline__ double colsum_part(const T* L, const double* u, int tid) {
+  const int c = tid >> 2, r0 = (tid & 3) * 16;
+  double s = 0.0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) s += (double)L[c * 64 + r0 + r] * u[r0 + r];
+  return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_interior(const DevSub* __restrict__ subs,
+                                                  const T* __restrict__ pool,
+                                                  const double* __restrict__ vec,
+                                                  const double* __restrict__ U,
+                                                  double* __restrict__ out) {
+  const DevSub S = subs[blockIdx.x];
+  const T* P = pool + S.pool_off;
+  const double* yv = vec + S.vec_off;
+  double* v = out + S.vec_off;
+  const double* u = U + S.vec_off;
+  __shared__ T dg[64 * kLd];
+  __shared__ double tv[64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // t_K = y_K - sum_{I in [Ti, Tc)} L_IK^T U_I
+  for (int K = 0; K < S.Ti; ++K) {
+    double s = 0.0;
+    for (int I = S.Ti; I < S.Tc; ++I) s += colsum_part(P + tile_off(I, K), u + I * 64, tid);
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if ((tid & 3) == 0) v[K * 64 + (tid >> 2)] = yv[K * 64 + (tid >> 2)] - s;
+  }
+  __syncthreads();
+  // backward substitution with L_ii^T
+  for (int J = S.Ti - 1; J >= 0; --J) {
+    double s = 0.0;
+    for (int K = J + 1; K < S.Ti; ++K) s += colsum_part(P + tile_off(K, J), v + K * 64, tid);
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if ((tid & 3) == 0) tv[tid >> 2] = s;
+    const T* D = P + tile_off(J, J);
+    for (int e = tid; e < kTileElems; e += 256) dg[(e >> 6) * kLd + (e & 63)] = D[e];
+    __syncthreads();
+    if (wv == 0) {
+      double t = v[J * 64 + lane] - tv[lane];
+      double x = 0.0;
+      for (int c = 63; c >= 0; --c) {
+        const double xc = __shfl(t, c) / (double)dg[c * kLd + c];
+        if (lane == c) x = xc;
+        if (lane < c) t -= (double)dg[lane * kLd + c] * xc;  // L[c][lane]
+      }
+      v[J * 64 + lane] = x;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+void launch_interior(int ns, const DevSub* subs, const T* pool, const double* vec, const double* U,
+                     double* out, hipStream_t st) {
+  if (ns <= 0) return;
+  hipLaunchKernelGGL(k_interior<T>, dim3(ns), dim3(256), 0, st, subs, pool, vec, U, out);
+}
+template void launch_interior<double>(int, const DevSub*, const double*, const double*,
+                                      const double*, double*, hipStream_t);
+template void launch_interior<float>(int, const DevSub*, const float*, const double*, const double*,
+                                     double*, hipStream_t);
+
+__global__ __launch_bounds__(256) void k_rho(int64_t n, const int64_t* __restrict__ pos,
+                                             const void* __restrict__ pool, int is_f32,
+                                             double* __restrict__ rho) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = pos[i];
+  double v = 0.0;
+  if (p >= 0) v = is_f32 ? (double)((const float*)pool)[p] : ((const double*)pool)[p];
+  rho[i] = v;
+}
+
+// Stiffness scaling: the D slot of subdomain s for a multiplier shared with t is
+// weighted rho_t / (rho_s + rho_t); sgnD = sign * weight, half = 0.5 on real slots.
+__global__ __launch_bounds__(256) void k_scaling(int64_t nl, const int64_t* __restrict__ lam_pos,
+                                                 const double* __restrict__ rho,
+                                                 const double* __restrict__ sgn,
+                                                 double* __restrict__ sgnD, double* __restrict__ half) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  const int64_t p0 = lam_pos[2 * l], p1 = lam_pos[2 * l + 1];
+  const double r0 = rho[p0], r1 = rho[p1];
+  sgnD[p0] = sgn[p0] * (r1 / (r0 + r1));
+  sgnD[p1] = sgn[p1] * (r0 / (r0 + r1));
+  half[p0] = 0.5;
+  half[p1] = 0.5;
+}
+
+void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void* pool, int is_f32,
+                    double* rho, const int64_t* lam_pos, const double* sgn, double* sgnD,
+                    double* half, hipStream_t st) {
+  if (ndv > 0)
+    hipLaunchKernelGGL(k_rho, dim3((ndv + 255) / 256), dim3(256), 0, st, ndv, rho_pos, pool, is_f32, rho);
+  if (nl > 0)
+    hipLaunchKernelGGL(k_scaling, dim3((nl + 255) / 256), dim3(256), 0, st, nl, lam_pos, rho, sgn,
+                       sgnD, half);
+}
+
+// ---------------------------------------------------------------------------
+// PCPG operators
+// ---------------------------------------------------------------------------
+constexpr int kGemvRows = 64;
+
+// which = 0: M = W_s ((ndp+ncp) x ndp), out at y_off; which = 1: M = P_s (ndp x ndp), out at dvec_off
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_dual(const DevSub* __restrict__ subs,
+                                                   const T* __restrict__ M, int which,
+                                                   const int32_t* __restrict__ d_lam,
+                                                   const double* __restrict__ d_scale,
+                                                   const double* __restrict__ v, double* __restrict__ y) {
+  extern __shared__ double xs[];
+  const DevSub S = subs[blockIdx.y];
+  const int rows = which == 0 ? S.ndp + S.ncp : S.ndp;
+  const int r0 = blockIdx.x * kGemvRows;
+  if (r0 >= rows) return;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int k = tid; k < S.ndp; k += 256) {
+    const int64_t slot = S.dvec_off + k;
+    const int32_t l = d_lam[slot];
+    xs[k] = l >= 0 ? d_scale[slot] * v[l] : 0.0;
+  }
+  __syncthreads();
+  const T* base = M + (which == 0 ? S.w_off : S.p_off);
+  const int ncol = S.nd;
+  for (int rr = wv; rr < kGemvRows; rr += 4) {
+    const int row = r0 + rr;
+    if (row >= rows) break;
+    const T* a = base + (int64_t)row * S.ndp;
+    double s = 0.0;
+    for (int k = lane; k < ncol; k += 64) s += (double)a[k] * xs[k];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) y[(which == 0 ? S.y_off : S.dvec_off) + row] = s;
+  }
+}
+
+template <typename T>
+void launch_gemv_dual(int ns, int max_rows, const DevSub* subs, const T* M, int which,
+                      const int32_t* d_lam, const double* d_scale, const double* v, double* y,
+                      hipStream_t st, int max_ndp) {
+  if (ns <= 0 || max_rows <= 0) return;
+  hipLaunchKernelGGL(k_gemv_dual<T>, dim3((max_rows + kGemvRows - 1) / kGemvRows, ns), dim3(256),
+                     max_ndp * sizeof(double), st, subs, M, which, d_lam, d_scale, v, y);
+}
+template void launch_gemv_dual<double>(int, int, const DevSub*, const double*, int, const int32_t*,
+                                       const double*, const double*, double*, hipStream_t, int);
+template void launch_gemv_dual<float>(int, int, const DevSub*, const float*, int, const int32_t*,
+                                      const double*, const double*, double*, hipStream_t, int);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__ subs,
+                                                      const T* __restrict__ W,
+                                                      const double* __restrict__ y,
+                                                      const int32_t* __restrict__ c_loc,
+                                                      const double* __restrict__ uc,
+                                                      const double* __restrict__ vec, double alpha,
+                                                      double beta, double gamma,
+                                                      double* __restrict__ out) {
+  const DevSub S = subs[blockIdx.y];
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= S.ndp) return;
+  double r = 0.0;
+  if (k < S.nd) {
+    double s = 0.0;
+    const T* ph = W + S.w_off + (int64_t)S.ndp * S.ndp;  // Phi^T rows
+    for (int c = 0; c < S.nc; ++c) s += (double)ph[(int64_t)c * S.ndp + k] * uc[c_loc[S.cvec_off + c]];
+    r = beta * s;
+    if (alpha != 0.0) r += alpha * y[S.y_off + k];
+    if (gamma != 0.0) r += gamma * vec[S.vec_off + S.o_a + k];
+  }
+  out[S.dvec_off + k] = r;
+}
+
+template <typename T>
+void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* y,
+                         const int32_t* c_loc, const double* uc, const double* vec, double alpha,
+                         double beta, double gamma, double* out, hipStream_t st) {
+  if (ns <= 0 || max_ndp <= 0) return;
+  hipLaunchKernelGGL(k_dual_correct<T>, dim3((max_ndp + 255) / 256, ns), dim3(256), 0, st, subs, W,
+                     y, c_loc, uc, vec, alpha, beta, gamma, out);
+}
+template void launch_dual_correct<double>(int, int, const DevSub*, const double*, const double*,
+                                          const int32_t*, const double*, const double*, double,
+                                          double, double, double*, hipStream_t);
+template void launch_dual_correct<float>(int, int, const DevSub*, const float*, const double*,
+                                         const int32_t*, const double*, const double*, double, double,
+                                         double, double*, hipStream_t);
+
+__global__ __launch_bounds__(256) void k_lambda_scatter(int64_t nl, const int64_t* __restrict__ pos,
+                                                        const double* __restrict__ scale,
+                                                        const double* __restrict__ w,
+                                                        double* __restrict__ out) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  const int64_t p0 = pos[2 * l], p1 = pos[2 * l + 1];
+  out[l] = scale[p0] * w[p0] + scale[p1] * w[p1];
+}
+
+void launch_lambda_scatter(int64_t nl, const int64_t* lam_pos, const double* scale, const double* w,
+                           double* out, hipStream_t st) {
+  if (nl <= 0) return;
+  hipLaunchKernelGGL(k_lambda_scatter, dim3((nl + 255) / 256), dim3(256), 0, st, nl, lam_pos, scale,
+                     w, out);
+}
+
+__global__ __launch_bounds__(256) void k_coarse_gather(int64_t nc, const int64_t* __restrict__ ptr,
+                                                       const int64_t* __restrict__ pos,
+                                                       const double* __restrict__ src,
+                                                       const double* __restrict__ add,
+                                                       double* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double s = add ? add[c] : 0.0;
+  for (int64_t j = ptr[c]; j < ptr[c + 1]; ++j) s += src[pos[j]];
+  out[c] = s;
+}
+
+void launch_coarse_gather(int64_t nc, const int64_t* ptr, const int64_t* pos, const double* src,
+                          const double* add, double* out, hipStream_t st) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_coarse_gather, dim3((nc + 255) / 256), dim3(256), 0, st, nc, ptr, pos, src,
+                     add, out);
+}
+
+__global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, int64_t ld, const double* __restrict__ A,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ y) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double* a = A + row * ld;
+  double s = 0.0;
+  for (int64_t k = lane; k < n; k += 64) s += a[k] * x[k];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) y[row] = s;
+}
+
+void launch_dense_gemv(int64_t n, int64_t ld, const double* A, const double* x, double* y,
+                       hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_dense_gemv, dim3((n + 3) / 4), dim3(256), 0, st, n, ld, A, x, y);
+}
+
+__global__ __launch_bounds__(256) void k_fill_U(const DevSub* __restrict__ subs,
+                                                const int32_t* __restrict__ d_lam,
+                                                const double* __restrict__ ubar,
+                                                const int32_t* __restrict__ c_loc,
+                                                const double* __restrict__ uc, double* __restrict__ U) {
+  const DevSub S = subs[blockIdx.y];
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < S.ndp) {
+    const int32_t l = d_lam[S.dvec_off + k];
+    U[S.vec_off + S.o_d + k] = l >= 0 ? ubar[l] : 0.0;
+  }
+  if (k < S.ncp) {
+    const int32_t c = c_loc[S.cvec_off + k];
+    U[S.vec_off + S.o_c + k] = c >= 0 ? uc[c] : 0.0;
+  }
+}
+
+void launch_fill_U(int ns, int max_m, const DevSub* subs, const int32_t* d_lam, const double* ubar,
+                   const int32_t* c_loc, const double* uc, double* U, hipStream_t st) {
+  if (ns <= 0 || max_m <= 0) return;
+  hipLaunchKernelGGL(k_fill_U, dim3((max_m + 255) / 256, ns), dim3(256), 0, st, subs, d_lam, ubar,
+                     c_loc, uc, U);
+}
+
+__global__ __launch_bounds__(256) void k_split_rhs(int64_t n, const int32_t* __restrict__ vdof,
+                                                   const double* __restrict__ vw,
+                                                   const double* __restrict__ f, double* __restrict__ vec) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t d = vdof[i];
+  vec[i] = d >= 0 ? f[d] * vw[i] : 0.0;
+}
+
+void launch_split_rhs(int64_t n, const int32_t* vdof, const double* vw, const double* f, double* vec,
+                      hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_split_rhs, dim3((n + 255) / 256), dim3(256), 0, st, n, vdof, vw, f, vec);
+}
+
+__global__ __launch_bounds__(256) void k_gather_global(int64_t nf, const int8_t* __restrict__ kind,
+                                                       const int64_t* __restrict__ a,
+                                                       const double* __restrict__ vec,
+                                                       const double* __restrict__ ubar,
+                                                       const double* __restrict__ uc,
+                                                       double* __restrict__ x) {
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nf) return;
+  const int k = kind[d];
+  const int64_t s = a[d];
+  x[d] = k == 0 ? vec[s] : (k == 1 ? ubar[s] : uc[s]);
+}
+
+void launch_gather_global(int64_t nf, const int8_t* kind, const int64_t* a, const double* vec,
+                          const double* ubar, const double* uc, double* x, hipStream_t st) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(k_gather_global, dim3((nf + 255) / 256), dim3(256), 0, st, nf, kind, a, vec,
+                     ubar, uc, x);
+}
+
+__global__ __launch_bounds__(256) void k_csr_residual(int64_t n, const int32_t* __restrict__ rp,
+                                                      const int32_t* __restrict__ col,
+                                                      const double* __restrict__ val,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ f,
+                                                      double* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int32_t j = rp[i]; j < rp[i + 1]; ++j) s += val[j] * x[col[j]];
+  r[i] = (f ? f[i] : 0.0) - s;
+}
+
+void launch_csr_residual(int64_t n, const int32_t* rp, const int32_t* col, const double* val,
+                         const double* x, const double* f, double* r, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_csr_residual, dim3((n + 255) / 256), dim3(256), 0, st, n, rp, col, val, x, f, r);
+}
+
+struct DotArgs {
+  const double* a[3];
+  const double* b[3];
+};
+
+__global__ __launch_bounds__(256) void k_dots(int64_t n, int np, DotArgs args, double* __restrict__ partial) {
+  __shared__ double red[3][256];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    for (int p = 0; p < np; ++p) s[p] += args.a[p][i] * args.b[p][i];
+  for (int p = 0; p < np; ++p) red[p][threadIdx.x] = s[p];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int p = 0; p < np; ++p) red[p][threadIdx.x] += red[p][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int p = 0; p < np; ++p) partial[p * kDotBlocks + blockIdx.x] = red[p][0];
+}
+
+void launch_dots(int64_t n, int npairs, const double* const* a, const double* const* b,
+                 double* partial, hipStream_t st) {
+  DotArgs args{};
+  for (int p = 0; p < npairs; ++p) {
+    args.a[p] = a[p];
+    args.b[p] = b[p];
+  }
+  hipLaunchKernelGGL(k_dots, dim3(kDotBlocks), dim3(256), 0, st, n, npairs, args, partial);
+}
+
+__global__ __launch_bounds__(256) void k_axpy2(int64_t n, double alpha, const double* __restrict__ p,
+                                               const double* __restrict__ q, double* __restrict__ lam,
+                                               double* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  lam[i] += alpha * p[i];
+  r[i] -= alpha * q[i];
+}
+
+void launch_axpy2(int64_t n, double alpha, const double* p, const double* q, double* lam, double* r,
+                  hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy2, dim3((n + 255) / 256), dim3(256), 0, st, n, alpha, p, q, lam, r);
+}
+
+__global__ __launch_bounds__(256) void k_xpby(int64_t n, const double* __restrict__ z, double beta,
+                                              double* __restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = z[i] + beta * p[i];
+}
+
+void launch_xpby(int64_t n, const double* z, double beta, double* p, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_xpby, dim3((n + 255) / 256), dim3(256), 0, st, n, z, beta, p);
+}
+
+__global__ __launch_bounds__(256) void k_axpy(int64_t n, double alpha, const double* __restrict__ x,
+                                              double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] += alpha * x[i];
+}
+
+void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, st, n, alpha, x, y);
+}
+
+}  // namespace emag

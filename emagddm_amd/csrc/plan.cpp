@@ -1,0 +1,324 @@
+// Assembly plans and the batched factorization schedule (see plan.h).
+#include "plan.h"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+namespace emag {
+
+int packed_index(int a, int b) {
+  if (a > b) std::swap(a, b);
+  int k = 0;
+  for (int i = 0; i < a; ++i) k += 6 - i;
+  return k + (b - a);
+}
+
+namespace {
+
+// Group emissions (target, code) by target, keeping emission order inside a group.
+void group_emissions(std::vector<uint64_t>& tgt, const std::vector<int32_t>& code, uint64_t max_t,
+                     std::vector<int64_t>& target, std::vector<int64_t>& ptr,
+                     std::vector<int32_t>& src) {
+  std::vector<uint32_t> idx(tgt.size());
+  for (size_t i = 0; i < idx.size(); ++i) idx[i] = uint32_t(i);
+  radix_sort_pairs(tgt, idx, max_t);
+  target.clear();
+  ptr.clear();
+  src.resize(tgt.size());
+  for (size_t i = 0; i < tgt.size(); ++i) {
+    if (i == 0 || tgt[i] != tgt[i - 1]) {
+      target.push_back(int64_t(tgt[i]));
+      ptr.push_back(int64_t(i));
+    }
+    src[i] = code[idx[i]];
+  }
+  ptr.push_back(int64_t(tgt.size()));
+}
+
+int32_t enc(int64_t elem_index, int sign) {
+  return int32_t((elem_index << 1) | (sign < 0 ? 1 : 0));
+}
+
+void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
+  if (list.empty()) return;
+  TileLaunch L{mode, int64_t(S.items.size()), int64_t(list.size())};
+  S.items.insert(S.items.end(), list.begin(), list.end());
+  S.launches.push_back(L);
+  for (const TileWork& w : list) S.flops += 2.0 * kTile * kTile * kTile * double(w.k1 - w.k0);
+}
+
+}  // namespace
+
+void build_plans(const Topology& T, Plans& P) {
+  P = Plans();
+  const int64_t nt = T.n_tets, ns = T.n_sub, nf = T.n_free;
+  const int64_t one_code = enc(21 * nt, 1);  // trailing constant 1.0 of the element array
+
+  // local index of every free dof inside its subdomain's padded layout
+  std::vector<int32_t> loc(nf, -1);
+
+  // ---- subdomain pools ------------------------------------------------------
+  {
+    std::vector<uint64_t> tgt;
+    std::vector<int32_t> code;
+    tgt.reserve(size_t(nt) * 26);
+    code.reserve(size_t(nt) * 26);
+    P.vec_dof.assign(T.vec_elems, -1);
+    P.vec_w.assign(T.vec_elems, 0.0);
+    P.rho_pos.assign(T.dvec_elems, -1);
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& L = T.sub[s];
+      const int32_t* dofs = &T.sub_dof[T.sub_dof_ptr[s]];
+      const int n = L.ni + L.nd + L.nc;
+      auto padded = [&](int k) { return k < L.ni ? k : (k < L.ni + L.nd ? L.o_d + (k - L.ni) : L.o_c + (k - L.ni - L.nd)); };
+      for (int k = 0; k < n; ++k) {
+        loc[dofs[k]] = padded(k);
+        P.vec_dof[L.vec_off + padded(k)] = dofs[k];
+        P.vec_w[L.vec_off + padded(k)] = 1.0 / T.free_mult[dofs[k]];
+      }
+      for (int k = 0; k < L.nd; ++k) P.rho_pos[L.dvec_off + k] = pool_pos(L.pool_off, L.o_d + k, L.o_d + k);
+      for (int64_t j = T.sub_tet_ptr[s]; j < T.sub_tet_ptr[s + 1]; ++j) {
+        const int64_t t = T.sub_tet[j];
+        for (int a = 0; a < 6; ++a) {
+          const int32_t da = T.tet_free[6 * t + a];
+          if (da < 0) continue;
+          const int la = loc[da];
+          for (int b = 0; b < 6; ++b) {
+            const int32_t db = T.tet_free[6 * t + b];
+            if (db < 0) continue;
+            const int lb = loc[db];
+            if (la < lb) continue;
+            const int32_t c = enc(int64_t(packed_index(a, b)) * nt + t, T.tet_sign[6 * t + a] * T.tet_sign[6 * t + b]);
+            tgt.push_back(uint64_t(pool_pos(L.pool_off, la, lb)));
+            code.push_back(c);
+            if (la != lb && la / kTile == lb / kTile) {
+              tgt.push_back(uint64_t(L.pool_off + tile_offset(la / kTile, la / kTile) + (la % kTile) * kTile + lb % kTile));
+              code.push_back(c);
+            }
+          }
+        }
+      }
+      auto diag_one = [&](int r) {
+        tgt.push_back(uint64_t(pool_pos(L.pool_off, r, r)));
+        code.push_back(int32_t(one_code));
+      };
+      for (int r = L.ni; r < L.nip; ++r) diag_one(r);
+      for (int r = L.o_d + L.nd; r < L.o_c; ++r) diag_one(r);
+      for (int r = L.o_c + L.nc; r < L.o_a; ++r) diag_one(r);
+      for (int k = 0; k < L.nd; ++k) {
+        tgt.push_back(uint64_t(pool_pos(L.pool_off, L.o_a + k, L.o_d + k)));
+        code.push_back(int32_t(one_code));
+      }
+      for (int k = 0; k < n; ++k) loc[dofs[k]] = -1;
+    }
+    group_emissions(tgt, code, uint64_t(T.pool_elems), P.pool.target, P.pool.ptr, P.pool.src);
+  }
+
+  // ---- global CSR and RHS ---------------------------------------------------
+  {
+    std::vector<uint64_t> tgt;
+    std::vector<int32_t> code;
+    tgt.reserve(size_t(nt) * 36);
+    code.reserve(size_t(nt) * 36);
+    for (int64_t t = 0; t < nt; ++t)
+      for (int a = 0; a < 6; ++a) {
+        const int32_t da = T.tet_free[6 * t + a];
+        if (da < 0) continue;
+        for (int b = 0; b < 6; ++b) {
+          const int32_t db = T.tet_free[6 * t + b];
+          if (db < 0) continue;
+          tgt.push_back(uint64_t(da) * uint64_t(nf) + uint64_t(db));
+          code.push_back(enc(int64_t(packed_index(a, b)) * nt + t, T.tet_sign[6 * t + a] * T.tet_sign[6 * t + b]));
+        }
+      }
+    std::vector<int64_t> keys;
+    group_emissions(tgt, code, uint64_t(nf) * uint64_t(nf), keys, P.csr.ptr, P.csr.src);
+    P.csr.row_ptr.assign(nf + 1, 0);
+    P.csr.col.resize(keys.size());
+    for (size_t i = 0; i < keys.size(); ++i) {
+      P.csr.row_ptr[keys[i] / nf + 1]++;
+      P.csr.col[i] = int32_t(keys[i] % nf);
+    }
+    for (int64_t r = 0; r < nf; ++r) P.csr.row_ptr[r + 1] += P.csr.row_ptr[r];
+    if (keys.size() >= (size_t(1) << 31)) throw std::runtime_error("global CSR too large for int32");
+
+    tgt.clear();
+    code.clear();
+    for (int64_t t = 0; t < nt; ++t)
+      for (int a = 0; a < 6; ++a) {
+        const int32_t da = T.tet_free[6 * t + a];
+        if (da < 0) continue;
+        tgt.push_back(uint64_t(da));
+        code.push_back(enc(int64_t(a) * nt + t, T.tet_sign[6 * t + a]));
+      }
+    group_emissions(tgt, code, uint64_t(nf), P.rhs_global.target, P.rhs_global.ptr, P.rhs_global.src);
+  }
+
+  // ---- coarse pool: [[S~, I], [I, 0]], padded ---------------------------------
+  {
+    const int64_t nc = T.n_c;
+    const int64_t ncp = round_up(std::max<int64_t>(nc, 1), kTile);
+    P.coarse_ncp = ncp;
+    const int64_t Tc = 2 * ncp / kTile;
+    P.coarse_pool_elems = Tc * (Tc + 1) / 2 * kTileElems;
+    std::vector<uint64_t> tgt;
+    std::vector<int64_t> srcpos;
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& L = T.sub[s];
+      for (int ka = 0; ka < L.nc; ++ka) {
+        const int64_t Ca = T.c_loc[L.cvec_off + ka];
+        for (int kb = 0; kb <= ka; ++kb) {
+          const int64_t Cb = T.c_loc[L.cvec_off + kb];
+          if (Ca < Cb) throw std::runtime_error("primal order not monotone");
+          const int64_t sp = pool_pos(L.pool_off, L.o_c + ka, L.o_c + kb);
+          tgt.push_back(uint64_t(pool_pos(0, Ca, Cb)));
+          srcpos.push_back(sp);
+          if (Ca != Cb && Ca / kTile == Cb / kTile) {
+            tgt.push_back(uint64_t(tile_offset(Ca / kTile, Ca / kTile) + (Ca % kTile) * kTile + Cb % kTile));
+            srcpos.push_back(sp);
+          }
+        }
+      }
+    }
+    for (int64_t r = nc; r < ncp; ++r) {
+      tgt.push_back(uint64_t(pool_pos(0, r, r)));
+      srcpos.push_back(-1);
+    }
+    for (int64_t k = 0; k < nc; ++k) {
+      tgt.push_back(uint64_t(pool_pos(0, ncp + k, k)));
+      srcpos.push_back(-1);
+    }
+    std::vector<uint32_t> idx(tgt.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = uint32_t(i);
+    radix_sort_pairs(tgt, idx, uint64_t(P.coarse_pool_elems));
+    for (size_t i = 0; i < tgt.size(); ++i) {
+      if (i == 0 || tgt[i] != tgt[i - 1]) {
+        P.coarse.target.push_back(int64_t(tgt[i]));
+        P.coarse.ptr.push_back(int64_t(i));
+      }
+      P.coarse.srcpos.push_back(srcpos[idx[i]]);
+    }
+    P.coarse.ptr.push_back(int64_t(tgt.size()));
+
+    // coarse schedule (single matrix, eliminate the first ncp/64 tile columns)
+    const int Tcc = int(ncp / kTile), Ta = int(Tc);
+    TileSchedule& S = P.coarse_sched;
+    for (int j = 0; j < Tcc; ++j) {
+      add_tiles(S, 2, {TileWork{0, j, j, 0, j, 0}});
+      std::vector<TileWork> below;
+      for (int I = j + 1; I < Ta; ++I) below.push_back(TileWork{0, I, j, 0, j, 0});
+      add_tiles(S, 1, below);
+    }
+    std::vector<TileWork> schur;
+    for (int J = Tcc; J < Ta; ++J)
+      for (int I = J; I < Ta; ++I) schur.push_back(TileWork{0, I, J, 0, Tcc, 0});
+    add_tiles(S, 0, schur);
+  }
+
+  // ---- coarse gather positions ----------------------------------------------
+  P.c_ypos.resize(T.c_gather_slot.size());
+  P.c_vpos.resize(T.c_gather_slot.size());
+  {
+    std::vector<int32_t> sub_of_cslot(T.cvec_elems);
+    for (int64_t s = 0; s < ns; ++s)
+      for (int k = 0; k < T.sub[s].ncp; ++k) sub_of_cslot[T.sub[s].cvec_off + k] = int32_t(s);
+    for (size_t j = 0; j < T.c_gather_slot.size(); ++j) {
+      const int64_t slot = T.c_gather_slot[j];
+      const SubLayout& L = T.sub[sub_of_cslot[slot]];
+      const int64_t k = slot - L.cvec_off;
+      P.c_ypos[j] = L.y_off + L.ndp + k;
+      P.c_vpos[j] = L.vec_off + L.o_c + k;
+    }
+  }
+
+  // ---- local factorization schedule -------------------------------------------
+  {
+    TileSchedule& S = P.local_sched;
+    int maxTi = 0, maxTd = 0;
+    for (const SubLayout& L : T.sub) {
+      maxTi = std::max(maxTi, L.Ti);
+      maxTd = std::max(maxTd, L.Tr - L.Ti);
+    }
+    // phase 1: interior columns; rows of D' tiles are structurally zero here
+    for (int j = 0; j < maxTi; ++j) {
+      std::vector<TileWork> diag, below;
+      for (int64_t s = 0; s < ns; ++s) {
+        const SubLayout& L = T.sub[s];
+        if (j >= L.Ti) continue;
+        diag.push_back(TileWork{int32_t(s), j, j, 0, j, 0});
+        for (int I = j + 1; I < L.Tc; ++I) below.push_back(TileWork{int32_t(s), I, j, 0, j, 0});
+      }
+      add_tiles(S, 2, diag);
+      add_tiles(S, 1, below);
+    }
+    std::vector<TileWork> schur1;
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& L = T.sub[s];
+      if (L.Ti == 0) continue;
+      for (int J = L.Ti; J < L.Tc; ++J)
+        for (int I = J; I < L.Tc; ++I) schur1.push_back(TileWork{int32_t(s), I, J, 0, L.Ti, 0});
+    }
+    add_tiles(S, 0, schur1);
+    S.launches.push_back(TileLaunch{-1, 0, 0});  // snapshot S_DD for the preconditioner
+    // phase 2: dual columns
+    for (int j = 0; j < maxTd; ++j) {
+      std::vector<TileWork> diag, below;
+      for (int64_t s = 0; s < ns; ++s) {
+        const SubLayout& L = T.sub[s];
+        const int col = L.Ti + j;
+        if (col >= L.Tr) continue;
+        diag.push_back(TileWork{int32_t(s), col, col, L.Ti, col, 0});
+        for (int I = col + 1; I < L.T; ++I) below.push_back(TileWork{int32_t(s), I, col, L.Ti, col, 0});
+      }
+      add_tiles(S, 2, diag);
+      add_tiles(S, 1, below);
+    }
+    std::vector<TileWork> schur2;
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& L = T.sub[s];
+      if (L.Tr == L.Ti) continue;
+      for (int J = L.Tr; J < L.T; ++J)
+        for (int I = J; I < L.T; ++I) schur2.push_back(TileWork{int32_t(s), I, J, L.Ti, L.Tr, 0});
+    }
+    add_tiles(S, 0, schur2);
+  }
+
+  // ---- bounds validation of every index the kernels will use ------------------
+  auto check = [](bool ok, const char* what) {
+    if (!ok) throw std::runtime_error(std::string("plan out of range: ") + what);
+  };
+  for (int64_t t : P.pool.target) check(t >= 0 && t < T.pool_elems, "pool target");
+  for (int64_t t : P.coarse.target) check(t >= 0 && t < P.coarse_pool_elems, "coarse target");
+  for (int64_t sp : P.coarse.srcpos) check(sp < T.pool_elems, "coarse source");
+  for (const TileWork& w : P.local_sched.items) {
+    const SubLayout& L = T.sub[w.mat];
+    check(w.mat >= 0 && w.mat < ns && w.J <= w.I && w.I < L.T && w.k0 >= 0 && w.k1 <= w.J,
+          "local tile");
+  }
+  {
+    const int Ta = int(2 * P.coarse_ncp / kTile);
+    for (const TileWork& w : P.coarse_sched.items)
+      check(w.mat == 0 && w.J <= w.I && w.I < Ta && w.k0 >= 0 && w.k1 <= w.J, "coarse tile");
+  }
+  for (int64_t y : P.c_ypos) check(y >= 0 && y < T.y_elems, "coarse y position");
+  for (int64_t v : P.c_vpos) check(v >= 0 && v < T.vec_elems, "coarse v position");
+
+  // ---- device layout table ----------------------------------------------------
+  P.devsub.resize(ns);
+  for (int64_t s = 0; s < ns; ++s) {
+    const SubLayout& L = T.sub[s];
+    DevSub& D = P.devsub[s];
+    D.nd = L.nd; D.nc = L.nc; D.ndp = L.ndp; D.ncp = L.ncp;
+    D.o_d = L.o_d; D.o_c = L.o_c; D.o_a = L.o_a; D.m = L.m;
+    D.T = L.T; D.Ti = L.Ti; D.Tr = L.Tr; D.Tc = L.Tc;
+    D.pool_off = L.pool_off; D.vec_off = L.vec_off; D.dvec_off = L.dvec_off;
+    D.w_off = L.w_off; D.p_off = L.p_off; D.y_off = L.y_off; D.cvec_off = L.cvec_off;
+    P.max_ndp = std::max(P.max_ndp, L.ndp);
+    P.max_ncp = std::max(P.max_ncp, L.ncp);
+    P.max_m = std::max(P.max_m, L.m);
+    P.max_wrows = std::max(P.max_wrows, L.ndp + L.ncp);
+  }
+}
+
+}  // namespace emag

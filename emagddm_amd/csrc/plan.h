@@ -1,0 +1,72 @@
+// Assembly plans and the batched factorization schedule (host side).
+//
+// Every GPU reduction in the solver is written as "each output sums its own
+// contribution list in a fixed order", so results are bitwise repeatable.
+// The plans below are those lists, built once per mesh/partition.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "topology.h"
+
+namespace emag {
+
+// out[target[i]] = sum_{c in [ptr[i], ptr[i+1])} +-elem[src[c] >> 1]
+struct NzPlan {
+  std::vector<int64_t> target, ptr;
+  std::vector<int32_t> src;  // (element index << 1) | negative
+  int64_t size() const { return int64_t(target.size()); }
+};
+
+// out[target[i]] = sum_c (srcpos[c] < 0 ? 1 : pool[srcpos[c]])
+struct PoolPlan {
+  std::vector<int64_t> target, ptr, srcpos;
+  int64_t size() const { return int64_t(target.size()); }
+};
+
+struct CsrPlan {
+  std::vector<int32_t> row_ptr, col;
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> src;
+};
+
+// A run of the batched tile factorization: list of launches over `items`.
+//   mode 0/1/2 as launch_tile_mode; mode -1: snapshot the Dirichlet Schur blocks.
+struct TileLaunch {
+  int mode;
+  int64_t off, n;
+};
+struct TileSchedule {
+  std::vector<TileWork> items;
+  std::vector<TileLaunch> launches;
+  double flops = 0.0;  // algorithmic flops of the tile updates
+};
+
+struct Plans {
+  NzPlan pool;        // element matrices -> subdomain factor pools (augmented, tiled)
+  NzPlan rhs_global;  // element F -> global free-dof RHS
+  CsrPlan csr;        // element matrices -> global free-dof CSR
+  PoolPlan coarse;    // subdomain trailing blocks -> coarse augmented pool
+  std::vector<int32_t> vec_dof;  // local-vector slot -> free dof (-1 none)
+  std::vector<double> vec_w;     // 1 / multiplicity (RHS split weights)
+  std::vector<int64_t> c_ypos;   // per coarse gather entry: position in W-output space
+  std::vector<int64_t> c_vpos;   // per coarse gather entry: position in local-vector space
+  std::vector<int64_t> rho_pos;  // per D slot: pool position of its diagonal (-1 padding)
+  int64_t coarse_ncp = 0;        // padded coarse order
+  int64_t coarse_pool_elems = 0;
+  TileSchedule local_sched, coarse_sched;
+  std::vector<DevSub> devsub;
+  int max_ndp = 0, max_ncp = 0, max_m = 0, max_wrows = 0;
+};
+
+// Packed index of (a, b), a <= b, in the 21-entry upper triangle of a 6x6.
+int packed_index(int a, int b);
+
+void build_plans(const Topology& T, Plans& P);
+
+inline int64_t tile_offset(int64_t I, int64_t J) { return (I * (I + 1) / 2 + J) * kTileElems; }
+inline int64_t pool_pos(int64_t base, int64_t r, int64_t c) {  // r >= c
+  return base + tile_offset(r / kTile, c / kTile) + (c % kTile) * kTile + (r % kTile);
+}
+
+}  // namespace emag

@@ -1,0 +1,150 @@
+/*
+ * emagddm_fetidp.h -- C ABI of the MI355X FETI-DP solver for lowest-order
+ * Nedelec (edge) curl-curl problems  curl(nu curl A) + kappa A = J  on tets.
+ *
+ * Plain pointers and sizes only.  Unless a function says "device pointer",
+ * every array argument is host memory; the handle owns all device memory.
+ * Calls on one handle must be serialized (as EMLinearSolverBase: "set_matrix()
+ * and solve() need external synchronization", include/numeric/
+ * em_linear_solver.h:76-90).
+ *
+ * Status codes map 1:1 onto numeric::SolverStatus
+ * (include/numeric/em_linear_solver.h:26-32).
+ */
+#ifndef EMAGDDM_FETIDP_H
+#define EMAGDDM_FETIDP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fetidp_status {
+  FETIDP_SUCCESS = 0,          /* SolverStatus::SUCCESS */
+  FETIDP_NUMERICAL_ERROR = 1,  /* SolverStatus::NUMERICAL_ERROR (non-SPD block, breakdown) */
+  FETIDP_DIVERGED = 2,         /* SolverStatus::DIVERGED */
+  FETIDP_MAX_ITER_REACHED = 3, /* SolverStatus::MAX_ITER_REACHED */
+  FETIDP_INVALID_INPUT = 4     /* SolverStatus::INVALID_INPUT */
+};
+
+enum fetidp_precision {
+  FETIDP_FP64 = 0,        /* FP64 factors and operators on MFMA */
+  FETIDP_FP32_REFINE = 1  /* FP32 factors/operators on MFMA + FP64 iterative refinement */
+};
+
+/* Solver configuration.  Defaults (fetidp_config_default) mirror
+ * IterativeSolverConfig{tolerance = 1e-8, max_iterations = 1000}
+ * (include/numeric/iterative_solvers.h:59-75). */
+typedef struct fetidp_config {
+  int32_t precision;        /* enum fetidp_precision */
+  int32_t max_iterations;   /* PCPG iteration cap per solve */
+  double tolerance;         /* target ||f - K u|| / ||f|| on the assembled free-dof system */
+  double dual_tolerance;    /* first PCPG stop ||r||/||d||; tightened x0.1 until tolerance holds */
+  int32_t max_refinements;  /* FP32 mode: outer FP64 refinement steps */
+  int32_t verbose;
+} fetidp_config;
+
+typedef struct fetidp_stats {
+  int32_t status;
+  int32_t iterations;       /* total PCPG iterations (all refinement steps) */
+  int32_t refinements;      /* outer refinement steps taken */
+  int32_t reserved;
+  double residual_norm;     /* final ||f - K u||_2 / ||f||_2 (FP64, assembled system) */
+  double dual_residual;     /* final PCPG ||r|| / ||d|| */
+  double initial_dual_residual; /* ||d - F lambda0|| / ||d|| */
+  double factor_ms;         /* numeric setup: integration, assembly, factorization, coarse */
+  double solve_ms;          /* RHS condensation, PCPG, recovery, residual check */
+  double symbolic_ms;       /* host topology + plans + upload (fetidp_setup) */
+} fetidp_stats;
+
+typedef struct fetidp_solver* fetidp_handle;
+
+void fetidp_config_default(fetidp_config* cfg);
+
+/* Replaces EMSolverFactory::create_solver for the new FETIDP solver type
+ * (include/numeric/em_solver_factory.h:213-279). NULL on failure. */
+fetidp_handle fetidp_create(const fetidp_config* cfg);
+
+/* Symbolic setup.  Mesh, materials, partition and PEC edges; replaces the
+ * EMMeshData -> GlobalEdgeIDGenerator -> EMDOFManager chain
+ * (src/fe_em/global_edge_id_generator.cpp:30-110, src/fe_em/em_dof_manager.cpp:29-65)
+ * and FETIDPContext::initialize (src/fetidp/fetidp_context.cpp:19-35).
+ *   xyz[3*n_nodes], tet2node[4*n_tets] (Nedelec TET4 local node order),
+ *   tet_material[n_tets] in [0,n_mat), nu[n_mat] = 1/mu, kappa[n_mat] (sigma role),
+ *   tet_subdomain[n_tets] >= 0, pec_edge_nodes[2*n_pec]: DIRICHLET VECTOR_EDGE_ONLY
+ *   edge node pairs (EMBoundaryMarker, include/fe_em/em_mesh_data.hpp:166-176). */
+int32_t fetidp_setup(fetidp_handle h, int64_t n_nodes, const double* xyz, int64_t n_tets,
+                     const int32_t* tet2node, const int32_t* tet_material, int32_t n_mat,
+                     const double* nu, const double* kappa, const int32_t* tet_subdomain,
+                     int64_t n_pec, const int32_t* pec_edge_nodes);
+
+/* Replace material values (same n_mat) without redoing the symbolic setup. */
+int32_t fetidp_set_materials(fetidp_handle h, int32_t n_mat, const double* nu, const double* kappa);
+
+/* Source J sampled at the 4-point rule nodes of every tet, jq[12*n_tets]
+ * (point q has barycentric weight (5+3sqrt5)/20 on local node q, (5-sqrt5)/20
+ * on the others); F_e(i) = sum_q |T|/4 W_i(x_q).J_q.  Replaces the zero
+ * MagneticVector3DIntegrator::computeSourceVector
+ * (src/numeric/magnetic_vector_3d_integrator.cpp:343-364). */
+int32_t fetidp_set_rhs_tet_J(fetidp_handle h, const double* jq);
+
+/* Assembled free-dof right-hand side f[n_free] (replaces set_rhs by J). */
+int32_t fetidp_set_rhs_free(fetidp_handle h, const double* f);
+
+/* Numeric setup on the GPU: element integration, subdomain assembly, batched
+ * partial Cholesky (MFMA), local dual operators, coarse Schur complement and
+ * its factorization.  Analogue of set_matrix (include/numeric/em_linear_solver.h:110-130). */
+int32_t fetidp_factor(fetidp_handle h);
+
+/* PCPG solve; lambda0[n_lambda] may be NULL (start from 0).  x_free[n_free]
+ * (may be NULL) receives the solution in EMDOFManager free-dof order.
+ * Analogue of EMLinearSolverBase::solve -> SolverResult
+ * (include/numeric/em_linear_solver.h:53-61,132-160). */
+int32_t fetidp_solve(fetidp_handle h, const double* lambda0, double* x_free, fetidp_stats* stats);
+
+/* Sizes: out[0..9] = n_edges, n_free, n_lambda, n_primal, n_sub, n_tets,
+ * pool elements, operator elements (W+P), coarse padded order, max local order. */
+int32_t fetidp_get_sizes(fetidp_handle h, int64_t* out);
+
+/* Topology readback: tet_free[6*n_tets] (free dof per local edge, -1 PEC, in
+ * ElementGeometry order), edge_nodes[2*n_edges], lam_free[n_lambda],
+ * primal_free[n_primal].  Any pointer may be NULL. */
+int32_t fetidp_get_topology(fetidp_handle h, int32_t* tet_free, int32_t* edge_nodes,
+                            int32_t* lam_free, int32_t* primal_free);
+
+/* Global free-dof CSR of K + M and the assembled RHS (after fetidp_factor /
+ * a set_rhs call): row_ptr[n_free+1], col[nnz], val[nnz], f[n_free]. */
+int64_t fetidp_get_global_nnz(fetidp_handle h);
+int32_t fetidp_get_global_system(fetidp_handle h, int32_t* row_ptr, int32_t* col, double* val,
+                                 double* f);
+
+/* Dual-space operators after fetidp_factor and a RHS: q = F p, z = M^-1 r,
+ * d (dual right-hand side), last lambda.  Vectors have n_lambda entries. */
+int32_t fetidp_apply_dual_operator(fetidp_handle h, const double* p, double* q);
+int32_t fetidp_apply_preconditioner(fetidp_handle h, const double* r, double* z);
+int32_t fetidp_get_dual_rhs(fetidp_handle h, double* d);
+int32_t fetidp_get_lambda(fetidp_handle h, double* lam);
+/* Per-iteration PCPG dual residual history of the last solve (up to cap). */
+int32_t fetidp_get_history(fetidp_handle h, double* hist, int32_t cap);
+/* Coarse Schur complement S~ (n_primal^2, row-major) after fetidp_factor. */
+int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S);
+
+const char* fetidp_last_error(fetidp_handle h);
+void fetidp_destroy(fetidp_handle h);
+
+/* Stand-alone batched element kernel (device pointers, HIP stream or NULL):
+ * K_e, M_e packed upper triangles (21 per tet, structure-of-arrays: entry k of
+ * tet t at [k*n_tets + t]) and F_e (6 per tet, same layout; jq may be NULL).
+ * Replaces MagneticVector3DIntegrator::computeAllMatrices
+ * (src/numeric/magnetic_vector_3d_integrator.cpp:104-135) with D1-D4 corrected. */
+int32_t emag_nedelec_integrate(int64_t n_tets, const double* d_xyz, const int32_t* d_tet2node,
+                               const int32_t* d_tet_material, const double* d_nu,
+                               const double* d_kappa, const double* d_jq, double* d_Ke,
+                               double* d_Me, double* d_Fe, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMAGDDM_FETIDP_H */

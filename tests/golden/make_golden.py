@@ -1,0 +1,63 @@
+"""Writes golden vectors from the reference's own solvers (oracle/_ref).
+
+Run in the build container only (it needs oracle/_ref/libref_solvers.so, built
+from /root/reference by oracle/ref/Makefile).  The fixtures are data: the
+seeded problem's input checksums and the reference solvers' outputs.
+
+    python tests/golden/make_golden.py
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from emagddm_amd import workloads as W  # noqa: E402
+from oracle import assembly  # noqa: E402
+
+
+def ref_lib():
+    return C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libref_solvers.so"))
+
+
+def run(lib, p, cg_max=5000):
+    asm = assembly.assemble(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.pec_pairs, p.jq)
+    A, f = asm["A"], asm["f"]
+    n = A.shape[0]
+    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    x = np.zeros(n)
+    fm, sm = C.c_double(), C.c_double()
+    st = lib.ref_solve_llt(n, ip(rp), ip(col), dp(val), dp(f), dp(x), C.byref(fm), C.byref(sm))
+    assert st == 0
+    xc = np.zeros(n)
+    it, res = C.c_int(), C.c_double()
+    st2 = lib.ref_solve_cg(n, ip(rp), ip(col), dp(val), dp(f), C.c_double(1e-8), cg_max, dp(xc),
+                           C.byref(it), C.byref(res), C.byref(sm))
+    return dict(x_llt=x, x_cg=xc, cg_status=st2, cg_iterations=it.value, cg_residual=res.value,
+                n_free=n, nnz=A.nnz, f_norm=np.linalg.norm(f), a_sum=A.data.sum(),
+                jq_sum=p.jq.sum(), xyz_sum=p.xyz.sum())
+
+
+def main():
+    lib = ref_lib()
+    cases = {
+        "micro4": W.make_problem(4, 2, seed=3, name="micro"),
+        "micro8c": W.make_problem(8, 2, materials="contrast", seed=5, name="micro_c"),
+        "cube16": W.make_workload("cube16"),
+    }
+    for name, p in cases.items():
+        out = run(lib, p)
+        if out["n_free"] > 10000:
+            del out["x_cg"]  # keep the large fixture small: the LLT solution pins it
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+        print(name, out["n_free"], out["cg_iterations"], out["cg_residual"])
+
+
+if __name__ == "__main__":
+    main()

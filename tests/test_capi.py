@@ -1,0 +1,113 @@
+"""C ABI: library loads, exports every declared symbol, host-side setup is
+bit-exact with the oracle's topology, error paths return statuses (CPU only)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from emagddm_amd import FETIDPSolver, _lib, workloads as W
+from oracle import topology
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "emagddm_fetidp.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(fetidp_[A-Za-z_0-9]+|emag_[A-Za-z_0-9]+)\s*\(", text))
+
+
+def test_library_exports_all_declared_symbols():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert names == set(_lib.EXPORTS)
+
+
+def test_config_default_mirrors_iterative_config():
+    lib = _lib.load()
+    cfg = _lib.Config()
+    lib.fetidp_config_default(C.byref(cfg))
+    assert cfg.tolerance == 1e-8 and cfg.max_iterations == 1000 and cfg.precision == 0
+
+
+@pytest.mark.parametrize("n,parts,jitter", [(4, 2, None), (12, 3, None), (16, 2, None),
+                                            (20, 3, (5, 8))])
+def test_setup_topology_matches_oracle(n, parts, jitter):
+    p = W.make_problem(n, parts, jitter=jitter, seed=n)
+    s = FETIDPSolver().setup_problem(p)
+    t = s.topology()
+    t2e, pairs = topology.edge_ids(p.tet2node)
+    free = topology.free_numbering(pairs, p.pec_pairs)
+    assert np.array_equal(t["edge_nodes"], pairs)
+    assert np.array_equal(t["tet_free"], free[t2e])
+    nf = int(free.max()) + 1
+    mult, _, _ = topology.classify(free[t2e], p.tet_subdomain, nf)
+    assert np.array_equal(t["lam_free"], np.flatnonzero(mult == 2))
+    assert np.array_equal(t["primal_free"], np.flatnonzero(mult >= 3))
+    assert s.sizes["n_free"] == nf and s.sizes["n_sub"] == parts ** 3
+
+
+def test_setup_shuffled_tets_and_unsorted_nodes():
+    # non-ascending local node order and shuffled tets: ids follow first appearance
+    p = W.make_problem(5, 1, seed=0)
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(p.n_tets)
+    t2n = p.tet2node[perm].copy()
+    for t in range(0, t2n.shape[0], 3):   # odd permutation flips orientation
+        t2n[t, [0, 1]] = t2n[t, [1, 0]]
+    s = FETIDPSolver()
+    s.setup(p.xyz, t2n, p.tet_material[perm], p.nu, p.kappa, np.zeros(p.n_tets, np.int32),
+            p.pec_pairs[:, ::-1])
+    t = s.topology()
+    t2e, pairs = topology.edge_ids(t2n)
+    free = topology.free_numbering(pairs, p.pec_pairs)
+    assert np.array_equal(t["tet_free"], free[t2e])
+
+
+def test_error_paths_return_status():
+    lib = _lib.load()
+    h = lib.fetidp_create(None)
+    assert h
+    st = _lib.Stats()
+    # solve / factor before setup -> INVALID_INPUT, message set, no exception
+    assert lib.fetidp_factor(h) == _lib.INVALID_INPUT
+    assert lib.fetidp_solve(h, None, None, C.byref(st)) == _lib.INVALID_INPUT
+    assert st.status == _lib.INVALID_INPUT
+    assert b"setup" in lib.fetidp_last_error(h) or b"factor" in lib.fetidp_last_error(h)
+    # bad mesh: node index out of range
+    xyz = np.zeros((4, 3))
+    t2n = np.array([[0, 1, 2, 9]], dtype=np.int32)
+    one = np.array([0], dtype=np.int32)
+    nu = np.array([1.0])
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    rc = lib.fetidp_setup(h, 4, dp(xyz), 1, ip(t2n), ip(one), 1, dp(nu), dp(nu), ip(one), 0, None)
+    assert rc == _lib.INVALID_INPUT
+    # negative reluctivity
+    t2n = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    neg = np.array([-1.0])
+    rc = lib.fetidp_setup(h, 4, dp(xyz), 1, ip(t2n), ip(one), 1, dp(neg), dp(nu), ip(one), 0, None)
+    assert rc == _lib.INVALID_INPUT
+    # material id out of range
+    two = np.array([2], dtype=np.int32)
+    rc = lib.fetidp_setup(h, 4, dp(xyz), 1, ip(t2n), ip(two), 1, dp(nu), dp(nu), ip(one), 0, None)
+    assert rc == _lib.INVALID_INPUT
+    # invalid precision -> create fails
+    cfg = _lib.Config()
+    lib.fetidp_config_default(C.byref(cfg))
+    cfg.precision = 7
+    assert not lib.fetidp_create(C.byref(cfg))
+    lib.fetidp_destroy(h)
+    assert lib.fetidp_solve(None, None, None, None) == _lib.INVALID_INPUT
+
+
+def test_set_materials_requires_same_count():
+    p = W.make_problem(4, 2, seed=0)
+    s = FETIDPSolver().setup_problem(p)
+    with pytest.raises(RuntimeError, match="INVALID_INPUT"):
+        s.set_materials(np.ones(3), np.ones(3))
