@@ -20,7 +20,8 @@ FP64, FP32_REFINE = 0, 1
 class Config(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_iterations", C.c_int32),
                 ("tolerance", C.c_double), ("dual_tolerance", C.c_double),
-                ("max_refinements", C.c_int32), ("verbose", C.c_int32)]
+                ("max_refinements", C.c_int32), ("verbose", C.c_int32),
+                ("inner_tolerance", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -36,7 +37,10 @@ EXPORTS = [
     "fetidp_get_sizes", "fetidp_get_topology", "fetidp_get_global_nnz",
     "fetidp_get_global_system", "fetidp_apply_dual_operator", "fetidp_apply_preconditioner",
     "fetidp_get_dual_rhs", "fetidp_get_lambda", "fetidp_get_history",
-    "fetidp_get_coarse_matrix", "fetidp_last_error", "fetidp_destroy", "emag_nedelec_integrate",
+    "fetidp_get_refinement_history",
+    "fetidp_get_coarse_matrix", "fetidp_get_profile", "fetidp_reset_profile",
+    "fetidp_last_error", "fetidp_destroy", "emag_nedelec_integrate", "emag_tile_cholesky_test",
+    "emag_mfma_layout_probe",
 ]
 
 _lib = None
@@ -77,12 +81,18 @@ def load():
     lib.fetidp_get_dual_rhs.argtypes = [P, dp]
     lib.fetidp_get_lambda.argtypes = [P, dp]
     lib.fetidp_get_history.argtypes = [P, dp, C.c_int32]
+    lib.fetidp_get_refinement_history.argtypes = [P, dp, C.c_int32]
     lib.fetidp_get_coarse_matrix.argtypes = [P, dp]
+    lib.fetidp_get_profile.argtypes = [P, dp, C.c_int32]
+    lib.fetidp_reset_profile.argtypes = [P]
+    lib.fetidp_reset_profile.restype = None
     lib.fetidp_last_error.argtypes = [P]
     lib.fetidp_last_error.restype = C.c_char_p
     lib.fetidp_destroy.argtypes = [P]
     lib.fetidp_destroy.restype = None
     lib.emag_nedelec_integrate.argtypes = [C.c_int64, P, P, P, P, P, P, P, P, P, P]
+    lib.emag_tile_cholesky_test.argtypes = [C.c_int32, C.c_int64, C.c_int64, dp, dp]
+    lib.emag_mfma_layout_probe.argtypes = [C.c_int32, dp]
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is C.c_int:  # default

@@ -120,11 +120,59 @@ struct fetidp_solver {
   DBuf<double> cpool, cinv, sinv;
   DBuf<int> cfail;
   DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc;
-  DBuf<double> lam, r, z, p, q, dv, ubar, x, xtot, rglob, partial;
-  std::vector<double> hist;
+  DBuf<double> lam, r, z, p, q, dv, ubar, x, xtot, rglob, partial, pf, qf;
+  std::vector<double> hist, ref_hist;
   double last_dual = 0.0, init_dual = 0.0;
 
+  // ---- optional event profiling (cfg.verbose & 2) ---------------------------
+  // slot k accumulates milliseconds of one kernel class; see fetidp_get_profile.
+  static constexpr int kProfSlots = 24;
+  double prof_ms[kProfSlots] = {};
+  double prof_cnt[kProfSlots] = {};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  bool profiling() const { return (cfg.verbose & 2) != 0; }
+  hipEvent_t prof_event() {
+    if (!prof_pool.empty()) {
+      hipEvent_t e = prof_pool.back();
+      prof_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+  }
+  template <typename Fn>
+  void timed(int slot, Fn&& fn) {
+    if (!profiling()) {
+      fn();
+      return;
+    }
+    hipEvent_t a = prof_event(), b = prof_event();
+    HIPCHECK(hipEventRecord(a, st));
+    fn();
+    HIPCHECK(hipEventRecord(b, st));
+    prof_pending.push_back({slot, {a, b}});
+  }
+  void prof_collect() {
+    for (auto& pe : prof_pending) {
+      HIPCHECK(hipEventSynchronize(pe.second.second));
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+      prof_ms[pe.first] += ms;
+      prof_cnt[pe.first] += 1;
+      prof_pool.push_back(pe.second.first);
+      prof_pool.push_back(pe.second.second);
+    }
+    prof_pending.clear();
+  }
+
   ~fetidp_solver() {
+    for (auto& pe : prof_pending) {
+      (void)hipEventDestroy(pe.second.first);
+      (void)hipEventDestroy(pe.second.second);
+    }
+    for (hipEvent_t e : prof_pool) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
   }
 
@@ -230,6 +278,8 @@ struct fetidp_solver {
     for (DBuf<double>* b : {&lam, &r, &z, &p, &q, &dv, &ubar}) b->alloc(std::max<int64_t>(nl, 1));
     x.alloc(nf);
     xtot.alloc(nf);
+    pf.alloc(nf);
+    qf.alloc(nf);
     rglob.alloc(nf);
     partial.alloc(3 * kDotBlocks);
     cpool.alloc(plans.coarse_pool_elems);
@@ -276,12 +326,13 @@ struct fetidp_solver {
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& base,
                     T* pool, T* inv, int* failp, T* P) {
+    const int slot0 = P ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     for (const TileLaunch& L : S.launches) {
       if (L.mode < 0) {
-        launch_extract_P<T>(topo.n_sub, plans.max_ndp, devsub.p, pool, P, st);
+        timed(5, [&] { launch_extract_P<T>(topo.n_sub, plans.max_ndp, devsub.p, pool, P, st); });
         continue;
       }
-      launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, pool, inv, failp, st);
+      timed(slot0 + L.mode, [&] { launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, pool, inv, failp, st); });
     }
   }
 
@@ -289,8 +340,10 @@ struct fetidp_solver {
   void factor_impl(DBuf<T>& pool, DBuf<T>& inv, DBuf<T>& W, DBuf<T>& P) {
     const int64_t nt = topo.n_tets;
     // element matrices A_e = K_e + M_e (and F_e when the RHS is given by J)
-    launch_integrate(nt, xyz.p, t2n.p, mat.p, nu.p, kappa.p, rhs_from_J ? jq.p : nullptr, 1, Ae.p,
-                     nullptr, Fe.p, st);
+    timed(0, [&] {
+      launch_integrate(nt, xyz.p, t2n.p, mat.p, nu.p, kappa.p, rhs_from_J ? jq.p : nullptr, 1, Ae.p,
+                       nullptr, Fe.p, st);
+    });
     // global CSR values (FP64) for the residual check
     launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st);
     if (rhs_from_J)
@@ -301,15 +354,17 @@ struct fetidp_solver {
     W.alloc(size_t(topo.w_elems));
     P.alloc(size_t(topo.p_elems));
     pool.zero(st);
-    if constexpr (sizeof(T) == 8)
-      launch_assemble_f64(plans.pool.size(), pool_target.p, pool_ptr.p, pool_src.p, Ae.p, pool.p, st);
-    else
-      launch_assemble_f32(plans.pool.size(), pool_target.p, pool_ptr.p, pool_src.p, Ae.p, pool.p, st);
+    timed(1, [&] {
+      if constexpr (sizeof(T) == 8)
+        launch_assemble_f64(plans.pool.size(), pool_target.p, pool_ptr.p, pool_src.p, Ae.p, pool.p, st);
+      else
+        launch_assemble_f32(plans.pool.size(), pool_target.p, pool_ptr.p, pool_src.p, Ae.p, pool.p, st);
+    });
     launch_scaling(topo.dvec_elems, topo.n_lam, rho_pos.p, pool.p, sizeof(T) == 4, rho.p, lam_pos.p,
                    d_sgn.p, d_sgnD.p, d_half.p, st);
     fail.zero(st);
     run_schedule<T>(plans.local_sched, local_items, local_base, pool.p, inv.p, fail.p, P.p);
-    launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, W.p, st);
+    timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, W.p, st); });
     // coarse problem in FP64
     cpool.zero(st);
     launch_gather_pool<T>(plans.coarse.size(), coarse_target.p, coarse_ptr.p, coarse_srcpos.p,
@@ -318,6 +373,7 @@ struct fetidp_solver {
     run_schedule<double>(plans.coarse_sched, coarse_items, coarse_base, cpool.p, cinv.p, cfail.p, nullptr);
     launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, sinv.p, st);
     HIPCHECK(hipStreamSynchronize(st));
+    prof_collect();
     std::vector<int> hf(topo.n_sub);
     HIPCHECK(hipMemcpy(hf.data(), fail.p, sizeof(int) * topo.n_sub, hipMemcpyDeviceToHost));
     int hc = 0;
@@ -360,16 +416,19 @@ struct fetidp_solver {
 
   template <typename T>
   void apply_F(const DBuf<T>& W, const double* pin, double* qout) {
-    launch_gemv_dual<T>(topo.n_sub, plans.max_wrows, devsub.p, W.p, 0, d_lam.p, d_sgn.p, pin, ybuf.p, st, plans.max_ndp);
+    timed(9, [&] { launch_gemv_dual<T>(topo.n_sub, plans.max_wrows, devsub.p, W.p, 0, d_lam.p, d_sgn.p, pin, ybuf.p, st, plans.max_ndp); });
+    timed(10, [&] { apply_F_rest<T>(W, qout); });
+  }
+  template <typename T>
+  void apply_F_rest(const DBuf<T>& W, double* qout) {
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
     launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, gc.p, uc.p, st);
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
-
   template <typename T>
   void apply_M(const DBuf<T>& P, const double* rin, double* zout) {
-    launch_gemv_dual<T>(topo.n_sub, plans.max_ndp, devsub.p, P.p, 1, d_lam.p, d_sgnD.p, rin, dbuf.p, st, plans.max_ndp);
+    timed(11, [&] { launch_gemv_dual<T>(topo.n_sub, plans.max_ndp, devsub.p, P.p, 1, d_lam.p, d_sgnD.p, rin, dbuf.p, st, plans.max_ndp); });
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgnD.p, dbuf.p, zout, st);
   }
 
@@ -450,12 +509,13 @@ struct fetidp_solver {
       last_dual = std::sqrt(o[0]) / dn;
       hist.push_back(last_dual);
       if (!std::isfinite(last_dual)) return FETIDP_NUMERICAL_ERROR;
-      if (last_dual < tol) return FETIDP_SUCCESS;
       if (its > 10 && last_dual > prev_ * 1000.0) return FETIDP_DIVERGED;
+      // finish the recurrence before any return so a later call can resume it
       const double beta = o[1] / rz_;
       rz_ = o[1];
       prev_ = last_dual;
       launch_xpby(nl, z.p, beta, p.p, st);
+      if (last_dual < tol) return FETIDP_SUCCESS;
     }
     return FETIDP_MAX_ITER_REACHED;
   }
@@ -466,6 +526,7 @@ struct fetidp_solver {
                  fetidp_stats& S) {
     const int64_t nl = topo.n_lam, nf = topo.n_free;
     hist.clear();
+    ref_hist.clear();
     int its = 0, status = FETIDP_SUCCESS;
     double dn = 0.0, res = 0.0;
     if (!fp32()) {
@@ -479,30 +540,56 @@ struct fetidp_solver {
         fresh = false;
         recover<T>(pool, W, x.p);
         res = global_residual(x.p, f_glob.p);
+        ref_hist.push_back(res);
         if (status != FETIDP_SUCCESS || res <= cfg.tolerance) break;
         tol *= 0.1;
       }
       S.refinements = 0;
       HIPCHECK(hipMemcpyAsync(xtot.p, x.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
     } else {
-      // FP64 iterative refinement around the FP32 FETI-DP solve
+      // FP64 flexible CG on the assembled system, preconditioned by the FP32
+      // FETI-DP solve (inner PCPG to cfg.inner_tolerance): the outer recurrence,
+      // inner products and residual are FP64, the subdomain work is FP32.
       xtot.zero(st);
       HIPCHECK(hipMemcpyAsync(f_cur.p, f_glob.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
+      double o[3];
+      const double* af[1] = {f_glob.p};
+      const double fn2 = dots(1, af, af, nf, o);
       int k = 0;
       res = 1.0;
+      double pq_prev = 0.0;
       for (k = 0; k < cfg.max_refinements; ++k) {
         condense<T>(pool, W, f_cur.p);
         if (k == 0 && h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
         else lam.zero(st);
-        status = pcpg<T>(W, P, cfg.dual_tolerance, cfg.max_iterations - its, true, its, dn);
+        status = pcpg<T>(W, P, cfg.inner_tolerance, cfg.max_iterations - its, true, its, dn);
         if (status == FETIDP_NUMERICAL_ERROR || status == FETIDP_DIVERGED) break;
-        recover<T>(pool, W, x.p);
-        launch_axpy(nf, 1.0, x.p, xtot.p, st);
-        res = global_residual(xtot.p, f_glob.p);
-        HIPCHECK(hipMemcpyAsync(f_cur.p, rglob.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
-        if (res <= cfg.tolerance) { status = FETIDP_SUCCESS; break; }
+        recover<T>(pool, W, x.p);                      // z = M32^-1 r
+        if (k == 0) {
+          HIPCHECK(hipMemcpyAsync(pf.p, x.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
+        } else {
+          const double* a1[1] = {x.p};
+          const double* b1[1] = {qf.p};
+          const double beta = -dots(1, a1, b1, nf, o) / pq_prev;
+          launch_xpby(nf, x.p, beta, pf.p, st);       // p = z + beta p
+        }
+        launch_csr_spmv(nf, csr_rp.p, csr_col.p, csr_val.p, pf.p, qf.p, st);  // q = K p
+        const double* a2[2] = {pf.p, pf.p};
+        const double* b2[2] = {f_cur.p, qf.p};
+        dots(2, a2, b2, nf, o);
+        pq_prev = o[1];
+        if (!(pq_prev > 0.0)) { status = FETIDP_NUMERICAL_ERROR; break; }
+        const double alpha = o[0] / pq_prev;
+        launch_axpy2(nf, alpha, pf.p, qf.p, xtot.p, f_cur.p, st);  // x += a p, r -= a q
+        const double* a3[1] = {f_cur.p};
+        res = std::sqrt(dots(1, a3, a3, nf, o) / fn2);
+        ref_hist.push_back(res);
+        if (res <= 0.5 * cfg.tolerance) { status = FETIDP_SUCCESS; break; }
         if (its >= cfg.max_iterations) { status = FETIDP_MAX_ITER_REACHED; break; }
       }
+      // the recursive residual is confirmed with the true FP64 residual
+      res = global_residual(xtot.p, f_glob.p);
+      ref_hist.push_back(res);
       if (res > cfg.tolerance && status == FETIDP_SUCCESS) status = FETIDP_MAX_ITER_REACHED;
       S.refinements = k + 1;
     }
@@ -521,6 +608,7 @@ struct fetidp_solver {
                         : solve_impl<double>(pool64, W64, P64, h_lam0, S);
     if (h_x) HIPCHECK(hipMemcpyAsync(h_x, xtot.p, sizeof(double) * topo.n_free, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    prof_collect();
     S.solve_ms = now_ms() - t0;
     S.status = status;
     return status;
@@ -552,6 +640,39 @@ int32_t guard(fetidp_handle h, const std::function<int32_t()>& fn) {
 }
 }  // namespace
 
+template <typename T>
+static int32_t tile_cholesky_test(int64_t m, int64_t pc, const double* A, double* out) {
+  const int Ta = int(m / kTile), Tp = int(pc / kTile);
+  const int64_t elems = int64_t(Ta) * (Ta + 1) / 2 * kTileElems;
+  std::vector<T> pool(elems, T(0));
+  for (int64_t i = 0; i < m; ++i)
+    for (int64_t j = 0; j <= i; ++j) {
+      pool[pool_pos(0, i, j)] = T(A[i * m + j]);
+      if (i / kTile == j / kTile) pool[tile_offset(i / kTile, i / kTile) + (i % kTile) * kTile + j % kTile] = T(A[i * m + j]);
+    }
+  TileSchedule S = single_matrix_schedule(Tp, Ta);
+  DBuf<T> dpool, dinv;
+  DBuf<TileWork> items;
+  DBuf<int64_t> base;
+  DBuf<int> fail;
+  dpool.upload(pool);
+  dinv.alloc(kTileElems);
+  items.upload(S.items);
+  base.upload(std::vector<int64_t>{0});
+  fail.alloc(1);
+  HIPCHECK(hipMemset(fail.p, 0, sizeof(int)));
+  for (const TileLaunch& L : S.launches)
+    launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, dpool.p, dinv.p, fail.p, nullptr);
+  HIPCHECK(hipDeviceSynchronize());
+  HIPCHECK(hipMemcpy(pool.data(), dpool.p, sizeof(T) * elems, hipMemcpyDeviceToHost));
+  int hf = 0;
+  HIPCHECK(hipMemcpy(&hf, fail.p, sizeof(int), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < m; ++i)
+    for (int64_t j = 0; j < m; ++j) out[i * m + j] = j <= i ? double(pool[pool_pos(0, i, j)]) : 0.0;
+  return hf ? FETIDP_NUMERICAL_ERROR : FETIDP_SUCCESS;
+}
+
+
 extern "C" {
 
 void fetidp_config_default(fetidp_config* cfg) {
@@ -561,8 +682,9 @@ void fetidp_config_default(fetidp_config* cfg) {
   cfg->max_iterations = 1000;
   cfg->tolerance = 1e-8;
   cfg->dual_tolerance = 1e-10;
-  cfg->max_refinements = 8;
+  cfg->max_refinements = 12;
   cfg->verbose = 0;
+  cfg->inner_tolerance = 1e-2;
 }
 
 fetidp_handle fetidp_create(const fetidp_config* cfg) {
@@ -577,7 +699,8 @@ fetidp_handle fetidp_create(const fetidp_config* cfg) {
     if (h->cfg.max_iterations <= 0) h->cfg.max_iterations = 1000;
     if (!(h->cfg.tolerance > 0)) h->cfg.tolerance = 1e-8;
     if (!(h->cfg.dual_tolerance > 0)) h->cfg.dual_tolerance = 1e-10;
-    if (h->cfg.max_refinements <= 0) h->cfg.max_refinements = 8;
+    if (h->cfg.max_refinements <= 0) h->cfg.max_refinements = 12;
+    if (!(h->cfg.inner_tolerance > 0)) h->cfg.inner_tolerance = 1e-2;
     return h;
   } catch (...) {
     return nullptr;
@@ -734,6 +857,13 @@ int32_t fetidp_get_history(fetidp_handle h, double* hist, int32_t cap) {
   return n;
 }
 
+int32_t fetidp_get_refinement_history(fetidp_handle h, double* hist, int32_t cap) {
+  if (!h) return -1;
+  const int32_t n = int32_t(h->ref_hist.size());
+  for (int32_t i = 0; i < n && i < cap && hist; ++i) hist[i] = h->ref_hist[i];
+  return n;
+}
+
 int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S) {
   return guard(h, [&] {
     if (!h->factored || !S) throw InputError("get_coarse_matrix needs fetidp_factor first");
@@ -761,9 +891,63 @@ int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S) {
   });
 }
 
+int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
+  if (!h || !out) return -1;
+  const int n = fetidp_solver::kProfSlots;
+  int k = 0;
+  for (int i = 0; i < n && k + 1 < cap; ++i) {
+    out[k++] = h->prof_ms[i];
+    out[k++] = h->prof_cnt[i];
+  }
+  const TileSchedule* sch[2] = {&h->plans.local_sched, &h->plans.coarse_sched};
+  for (int q = 0; q < 2; ++q)
+    for (int mode = 0; mode < 3; ++mode) {
+      double fl = 0.0;
+      for (const TileLaunch& L : sch[q]->launches)
+        if (L.mode == mode)
+          for (int64_t i = 0; i < L.n; ++i) {
+            const TileWork& w = sch[q]->items[L.off + i];
+            fl += 2.0 * kTile * kTile * kTile * double(w.k1 - w.k0) + (mode == 1 ? 2.0 * kTile * kTile * kTile : 0.0);
+          }
+      if (k < cap) out[k++] = fl;
+    }
+  return k;
+}
+
+void fetidp_reset_profile(fetidp_handle h) {
+  if (!h) return;
+  for (int i = 0; i < fetidp_solver::kProfSlots; ++i) h->prof_ms[i] = h->prof_cnt[i] = 0.0;
+}
+
 const char* fetidp_last_error(fetidp_handle h) { return h ? h->err.c_str() : "null handle"; }
 
 void fetidp_destroy(fetidp_handle h) { delete h; }
+
+int32_t emag_mfma_layout_probe(int32_t precision, double* out) {
+  if (!out) return FETIDP_INVALID_INPUT;
+  try {
+    DBuf<double> d;
+    d.alloc(512);
+    launch_mfma_probe(precision == FETIDP_FP32_REFINE, d.p);
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipMemcpy(out, d.p, sizeof(double) * 512, hipMemcpyDeviceToHost));
+    return FETIDP_SUCCESS;
+  } catch (...) {
+    return FETIDP_NUMERICAL_ERROR;
+  }
+}
+
+int32_t emag_tile_cholesky_test(int32_t precision, int64_t m, int64_t p_cols, const double* A,
+                                double* out) {
+  if (m <= 0 || m % kTile || p_cols < 0 || p_cols % kTile || p_cols > m || !A || !out)
+    return FETIDP_INVALID_INPUT;
+  try {
+    return precision == FETIDP_FP32_REFINE ? tile_cholesky_test<float>(m, p_cols, A, out)
+                                           : tile_cholesky_test<double>(m, p_cols, A, out);
+  } catch (...) {
+    return FETIDP_NUMERICAL_ERROR;
+  }
+}
 
 int32_t emag_nedelec_integrate(int64_t n_tets, const double* d_xyz, const int32_t* d_tet2node,
                                const int32_t* d_tet_material, const double* d_nu,

@@ -32,6 +32,9 @@ template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base, T* pool,
                       T* inv, int* fail, hipStream_t st);
 
+// MFMA lane-layout probe (one wave, 512 doubles of output).
+void launch_mfma_probe(int is_f32, double* out);
+
 template <typename T>
 void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, T* P, hipStream_t st);
 template <typename T>
@@ -76,6 +79,8 @@ void launch_gather_global(int64_t nf, const int8_t* kind, const int64_t* a, cons
                           const double* ubar, const double* uc, double* x, hipStream_t st);
 void launch_csr_residual(int64_t n, const int32_t* rp, const int32_t* col, const double* val,
                          const double* x, const double* f, double* r, hipStream_t st);
+void launch_csr_spmv(int64_t n, const int32_t* rp, const int32_t* col, const double* val,
+                     const double* x, double* y, hipStream_t st);
 // partial[b] = sum over block b of a[i]*b[i] for up to 3 pairs; nblk partial sums each
 constexpr int kDotBlocks = 256;
 void launch_dots(int64_t n, int npairs, const double* const* a, const double* const* b,

@@ -11,7 +11,8 @@
 // MFMA lane layout used (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32):
 //   A operand  lane l -> A[l % 16][l / 16]           (16 x 4)
 //   B operand  lane l -> B[l / 16][l % 16]           (4 x 16)
-//   D          lane l, reg r -> D[4 * (l / 16) + r][l % 16]
+//   D          lane l, reg r -> D[Mfma<T>::row(l / 16, r)][l % 16]
+//              (f64: row l/16 + 4r; f32: row 4(l/16) + r -- see Mfma<T>)
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -199,12 +200,16 @@ template void launch_gather_pool<float>(int64_t, const int64_t*, const int64_t*,
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Mfma;
+// row(lk, r): accumulator row of register r in lane group lk = lane / 16 (the
+// column is lane % 16).  Measured on gfx950 by emag_mfma_layout_probe:
+// f64 interleaves rows across lane groups, f32 keeps 4 consecutive rows per lane.
 template <>
 struct Mfma<double> {
   typedef double acc_t __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -212,9 +217,35 @@ struct Mfma<float> {
   static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
 constexpr int kLd = 65;  // padded LDS leading dimension for a 64x64 tile
+
+// Lane-layout probe of the 16x16x4 MFMA builtins: with A[p][0] = p (A[p][k>0] = 0)
+// and B[0][q] = 1 the product holds D[p][q] = p, with A[p][0] = 1 and B[0][q] = q
+// it holds q; each lane writes its 4 accumulator values for both products.
+// out: [2 products][64 lanes][4 regs], one wave, all indices in bounds.
+template <typename T>
+__global__ void k_mfma_probe(double* out) {
+  using acc_t = typename Mfma<T>::acc_t;
+  const int l = threadIdx.x;
+  const int lr = l & 15, lk = l >> 4;
+  acc_t z = acc_t{0, 0, 0, 0};
+  acc_t d0 = Mfma<T>::op(lk == 0 ? T(lr) : T(0), lk == 0 ? T(1) : T(0), z);
+  acc_t d1 = Mfma<T>::op(lk == 0 ? T(1) : T(0), lk == 0 ? T(lr) : T(0), z);
+  for (int r = 0; r < 4; ++r) {
+    out[(0 * 64 + l) * 4 + r] = (double)d0[r];
+    out[(1 * 64 + l) * 4 + r] = (double)d1[r];
+  }
+}
+
+void launch_mfma_probe(int is_f32, double* out) {
+  if (is_f32)
+    hipLaunchKernelGGL(k_mfma_probe<float>, dim3(1), dim3(64), 0, nullptr, out);
+  else
+    hipLaunchKernelGGL(k_mfma_probe<double>, dim3(1), dim3(64), 0, nullptr, out);
+}
 
 // acc (32x32 per wave, 2x2 MFMA blocks) += A_p,k * B_k,q over one 64-deep tile,
 // A operand read from Ap (column-major 64x64, element (p,k) at k*lda + p),
@@ -267,7 +298,7 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
       for (int y = 0; y < 2; ++y)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int j = pb0 + 16 * x + 4 * lk + r, i = qb0 + 16 * y + lr;
+          const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
           C[j * 64 + i] -= acc[x][y][r];
         }
   } else {
@@ -278,7 +309,7 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
       for (int y = 0; y < 2; ++y)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int j = pb0 + 16 * x + 4 * lk + r, i = qb0 + 16 * y + lr;
+          const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
           s[j * kLd + i] = C[j * 64 + i] - acc[x][y][r];
         }
     __syncthreads();
@@ -297,7 +328,7 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
         for (int y = 0; y < 2; ++y)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int j = pb0 + 16 * x + 4 * lk + r, i = qb0 + 16 * y + lr;
+            const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
             C[j * 64 + i] = a2[x][y][r];
           }
     } else {
@@ -823,6 +854,23 @@ void launch_csr_residual(int64_t n, const int32_t* rp, const int32_t* col, const
                          const double* x, const double* f, double* r, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_csr_residual, dim3((n + 255) / 256), dim3(256), 0, st, n, rp, col, val, x, f, r);
+}
+
+__global__ __launch_bounds__(256) void k_csr_spmv(int64_t n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col,
+                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int32_t j = rp[i]; j < rp[i + 1]; ++j) s += val[j] * x[col[j]];
+  y[i] = s;
+}
+
+void launch_csr_spmv(int64_t n, const int32_t* rp, const int32_t* col, const double* val,
+                     const double* x, double* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_csr_spmv, dim3((n + 255) / 256), dim3(256), 0, st, n, rp, col, val, x, y);
 }
 
 struct DotArgs {

@@ -50,6 +50,21 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
 
 }  // namespace
 
+TileSchedule single_matrix_schedule(int Tp, int Ta) {
+  TileSchedule S;
+  for (int j = 0; j < Tp; ++j) {
+    add_tiles(S, 2, {TileWork{0, j, j, 0, j, 0}});
+    std::vector<TileWork> below;
+    for (int I = j + 1; I < Ta; ++I) below.push_back(TileWork{0, I, j, 0, j, 0});
+    add_tiles(S, 1, below);
+  }
+  std::vector<TileWork> schur;
+  for (int J = Tp; J < Ta; ++J)
+    for (int I = J; I < Ta; ++I) schur.push_back(TileWork{0, I, J, 0, Tp, 0});
+  add_tiles(S, 0, schur);
+  return S;
+}
+
 void build_plans(const Topology& T, Plans& P) {
   P = Plans();
   const int64_t nt = T.n_tets, ns = T.n_sub, nf = T.n_free;
@@ -202,18 +217,7 @@ void build_plans(const Topology& T, Plans& P) {
     P.coarse.ptr.push_back(int64_t(tgt.size()));
 
     // coarse schedule (single matrix, eliminate the first ncp/64 tile columns)
-    const int Tcc = int(ncp / kTile), Ta = int(Tc);
-    TileSchedule& S = P.coarse_sched;
-    for (int j = 0; j < Tcc; ++j) {
-      add_tiles(S, 2, {TileWork{0, j, j, 0, j, 0}});
-      std::vector<TileWork> below;
-      for (int I = j + 1; I < Ta; ++I) below.push_back(TileWork{0, I, j, 0, j, 0});
-      add_tiles(S, 1, below);
-    }
-    std::vector<TileWork> schur;
-    for (int J = Tcc; J < Ta; ++J)
-      for (int I = J; I < Ta; ++I) schur.push_back(TileWork{0, I, J, 0, Tcc, 0});
-    add_tiles(S, 0, schur);
+    P.coarse_sched = single_matrix_schedule(int(ncp / kTile), int(Tc));
   }
 
   // ---- coarse gather positions ----------------------------------------------

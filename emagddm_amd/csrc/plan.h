@@ -64,6 +64,10 @@ int packed_index(int a, int b);
 
 void build_plans(const Topology& T, Plans& P);
 
+// Left-looking schedule of one matrix of Ta tile rows: eliminate the first Tp
+// tile columns, then update the trailing (Ta-Tp)^2 block with the Schur term.
+TileSchedule single_matrix_schedule(int Tp, int Ta);
+
 inline int64_t tile_offset(int64_t I, int64_t J) { return (I * (I + 1) / 2 + J) * kTileElems; }
 inline int64_t pool_pos(int64_t base, int64_t r, int64_t c) {  // r >= c
   return base + tile_offset(r / kTile, c / kTile) + (c % kTile) * kTile + (r % kTile);

@@ -60,7 +60,8 @@ class FETIDPSolver:
     """MI355X FETI-DP solver for curl(nu curl A) + kappa A = J on Nedelec tets."""
 
     def __init__(self, precision="fp64", tolerance=1e-8, dual_tolerance=1e-10,
-                 max_iterations=1000, max_refinements=8):
+                 max_iterations=1000, max_refinements=12, profile=False,
+                 inner_tolerance=1e-2):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.fetidp_config_default(C.byref(cfg))
@@ -69,6 +70,8 @@ class FETIDPSolver:
         cfg.dual_tolerance = dual_tolerance
         cfg.max_iterations = max_iterations
         cfg.max_refinements = max_refinements
+        cfg.verbose = 2 if profile else 0
+        cfg.inner_tolerance = inner_tolerance
         self.precision = precision
         self._h = self._lib.fetidp_create(C.byref(cfg))
         if not self._h:
@@ -201,6 +204,30 @@ class FETIDPSolver:
         n = self._lib.fetidp_get_history(self._h, None, 0)
         h = np.zeros(max(n, 1))
         self._lib.fetidp_get_history(self._h, _dp(h), n)
+        return h[:n]
+
+    PROFILE_SLOTS = ["integrate", "pool_assembly", "tile_schur", "tile_column", "tile_diag",
+                     "extract", "coarse_schur", "coarse_column", "coarse_diag", "dual_gemv",
+                     "dual_rest", "precond_gemv"]
+
+    def profile(self):
+        """{kernel class: (ms, launches)} plus tile flops, from event timing."""
+        buf = np.zeros(64)
+        n = self._lib.fetidp_get_profile(self._h, _dp(buf), 64)
+        out = {name: (float(buf[2 * i]), int(buf[2 * i + 1]))
+               for i, name in enumerate(self.PROFILE_SLOTS)}
+        fl = buf[48:54] if n >= 54 else np.zeros(6)
+        out["flops"] = dict(zip(["tile_schur", "tile_column", "tile_diag", "coarse_schur",
+                                 "coarse_column", "coarse_diag"], (float(v) for v in fl)))
+        return out
+
+    def reset_profile(self):
+        self._lib.fetidp_reset_profile(self._h)
+
+    def refinement_history(self):
+        n = self._lib.fetidp_get_refinement_history(self._h, None, 0)
+        h = np.zeros(max(n, 1))
+        self._lib.fetidp_get_refinement_history(self._h, _dp(h), n)
         return h[:n]
 
     def coarse_matrix(self):

@@ -30,7 +30,8 @@ enum fetidp_status {
 
 enum fetidp_precision {
   FETIDP_FP64 = 0,        /* FP64 factors and operators on MFMA */
-  FETIDP_FP32_REFINE = 1  /* FP32 factors/operators on MFMA + FP64 iterative refinement */
+  FETIDP_FP32_REFINE = 1  /* FP32 factors/operators on MFMA inside FP64 refinement
+                             (flexible CG on the assembled system) */
 };
 
 /* Solver configuration.  Defaults (fetidp_config_default) mirror
@@ -41,8 +42,9 @@ typedef struct fetidp_config {
   int32_t max_iterations;   /* PCPG iteration cap per solve */
   double tolerance;         /* target ||f - K u|| / ||f|| on the assembled free-dof system */
   double dual_tolerance;    /* first PCPG stop ||r||/||d||; tightened x0.1 until tolerance holds */
-  int32_t max_refinements;  /* FP32 mode: outer FP64 refinement steps */
-  int32_t verbose;
+  int32_t max_refinements;  /* FP32 mode: outer FP64 flexible-CG steps */
+  int32_t verbose;          /* bit 1 (value 2): record per-kernel-class event timings */
+  double inner_tolerance;   /* FP32 mode: PCPG stop ||r||/||d|| of each refinement step */
 } fetidp_config;
 
 typedef struct fetidp_stats {
@@ -127,8 +129,18 @@ int32_t fetidp_get_dual_rhs(fetidp_handle h, double* d);
 int32_t fetidp_get_lambda(fetidp_handle h, double* lam);
 /* Per-iteration PCPG dual residual history of the last solve (up to cap). */
 int32_t fetidp_get_history(fetidp_handle h, double* hist, int32_t cap);
+/* Global relative residual after each recovery / refinement step of the last solve. */
+int32_t fetidp_get_refinement_history(fetidp_handle h, double* hist, int32_t cap);
 /* Coarse Schur complement S~ (n_primal^2, row-major) after fetidp_factor. */
 int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S);
+
+/* Event profile when cfg.verbose & 2: out[2k], out[2k+1] = milliseconds and
+ * launch count of kernel class k (0 integrate, 1 pool assembly, 2..4 local tile
+ * modes 0..2, 5 operator extraction, 6..8 coarse tile modes 0..2, 9 dual GEMV
+ * of F, 10 rest of F, 11 preconditioner GEMV), then the algorithmic flops of
+ * local modes 0..2 and coarse modes 0..2.  Returns the number of values written. */
+int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap);
+void fetidp_reset_profile(fetidp_handle h);
 
 const char* fetidp_last_error(fetidp_handle h);
 void fetidp_destroy(fetidp_handle h);
@@ -142,6 +154,20 @@ int32_t emag_nedelec_integrate(int64_t n_tets, const double* d_xyz, const int32_
                                const int32_t* d_tet_material, const double* d_nu,
                                const double* d_kappa, const double* d_jq, double* d_Ke,
                                double* d_Me, double* d_Fe, void* stream);
+
+/* Self-test of the batched tile factorization on one dense SPD matrix (host
+ * arrays, row-major m x m, m and p_cols multiples of 64): eliminates the first
+ * p_cols columns with the same MFMA kernels and schedule shape the solver uses
+ * and returns the lower triangle: L in the eliminated columns, the Schur
+ * complement A22 - L21 L21^T in the trailing block.  No reference counterpart. */
+int32_t emag_tile_cholesky_test(int32_t precision, int64_t m, int64_t p_cols, const double* A,
+                                double* out);
+
+/* Known-answer probe of the 16x16x4 MFMA accumulator lane layout the tile
+ * kernels rely on: out[512] = [2 products][64 lanes][4 registers] holding the
+ * row index p, then the column index q, of each accumulator element.  No
+ * reference counterpart (device self-test). */
+int32_t emag_mfma_layout_probe(int32_t precision, double* out);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,9 @@ solver implements, in the plainest dense form, for small meshes:
   primal (c, kept continuous through the coarse problem).  Lambdas and
   primals are numbered in ascending free-dof order; local dofs are ordered
   (i, Delta, c), each block ascending in free-dof index.
-* Local K_s = sum over the subdomain's tets of s_a s_b (K_e + M_e)(a, b).
+* Local K_s = sum over the subdomain's tets of s_a s_b (K_e + M_e)(a, b);
+  local f_s = global f divided by the edge's subdomain multiplicity (any split
+  with sum_s f_s = f gives the same u; this one fixes lambda).
 * Z_s = [K_rr^-1]_DD, Phi_s = [K_rr^-1 K_rc]_D, S~_s = K_cc - K_cr K_rr^-1 K_rc,
   Dirichlet Schur P_s = K_DD - K_Di K_ii^-1 K_iD.
 * F = sum B Z B^T + sum B Phi R S~^-1 R^T Phi^T B^T,
@@ -77,8 +79,8 @@ class DenseFETIDP:
             cc2 = np.tile(ml, (1, 6))
             m2 = (rr >= 0) & (cc2 >= 0)
             np.add.at(K, (rr[m2], cc2[m2]), Ae[tets].reshape(-1, 36)[m2])
-            fl = np.zeros(nl)
-            np.add.at(fl, ml[ok], Fe[tets][ok])
+            # right-hand side split: f_s = f / multiplicity on every shared edge
+            fl = self.f[order] / mult[order]
             ni, nd = ii.size, dd.size
             self.subs.append(dict(order=order, ni=ni, nd=nd, nc=cc.size, K=K, f=fl,
                                   lam=self.lam_of[dd], cidx=self.c_of[cc], dofs_d=dd))
