@@ -61,7 +61,8 @@ def run_variant(prob, precision, steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     r = results[-1]
-    ok = all(x.status == SolverStatus.SUCCESS and x.residual_norm <= 1e-8 for x in results)
+    ok = all(x.status == SolverStatus.SUCCESS and x.residual_norm <= 1e-8
+             and not x.fell_back_to_fp64 for x in results)
     prof = s.profile()
     return dict(solver=s, result=r, ok=ok, s_per_solve=(t1 - t0) / steps,
                 factor_ms=r.factor_time_ms, solve_ms=r.solve_time_ms,
@@ -138,11 +139,21 @@ def main():
     prob = workloads.make_workload(a.workload)
     variants = {}
     for prec in a.variants.split(","):
-        v = run_variant(prob, prec, a.steps, a.warmup)
+        try:
+            v = run_variant(prob, prec, a.steps, a.warmup)
+        except RuntimeError as e:
+            print(f"[bench] {prec}: failed: {e}", file=sys.stderr, flush=True)
+            continue
         variants[prec] = v
         print(f"[bench] {prec}: {v['s_per_solve']:.4f} s/solve its={v['iterations']} "
               f"ref={v['refinements']} res={v['residual']:.2e} ok={v['ok']}", file=sys.stderr,
               flush=True)
+        prof = {k: (round(ms / a.steps, 3), n // a.steps) for k, (ms, n) in
+                ((k, v["profile"][k]) for k in v["solver"].PROFILE_SLOTS)}
+        print(f"[bench] {prec} per-step event ms (ms, launches): {prof}", file=sys.stderr,
+              flush=True)
+        print(f"[bench] {prec} refinement residuals: {v['solver'].refinement_history()}",
+              file=sys.stderr, flush=True)
     good = {k: v for k, v in variants.items() if v["ok"]}
     best = min(good, key=lambda k: good[k]["s_per_solve"]) if good else "fp64"
     vb = variants[best]
@@ -166,7 +177,8 @@ def main():
         "variants": {k: dict(s_per_solve=round(v["s_per_solve"], 5),
                              factor_ms=round(v["factor_ms"], 2), solve_ms=round(v["solve_ms"], 2),
                              pcpg_iterations=v["iterations"], refinements=v["refinements"],
-                             residual=v["residual"], reached_1e8=v["ok"])
+                             residual=v["residual"], reached_1e8=v["ok"],
+                             fell_back_to_fp64=v["result"].fell_back_to_fp64)
                      for k, v in variants.items()},
         "speedup_vs_plain_fp64": (round(variants["fp64"]["s_per_solve"] / vb["s_per_solve"], 3)
                                   if "fp64" in variants else None),

@@ -26,7 +26,7 @@ class Config(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("refinements", C.c_int32),
-                ("reserved", C.c_int32), ("residual_norm", C.c_double),
+                ("fell_back", C.c_int32), ("residual_norm", C.c_double),
                 ("dual_residual", C.c_double), ("initial_dual_residual", C.c_double),
                 ("factor_ms", C.c_double), ("solve_ms", C.c_double), ("symbolic_ms", C.c_double)]
 

@@ -102,8 +102,8 @@ struct fetidp_solver {
   DBuf<int64_t> csr_ptr;
   DBuf<double> csr_val;
   DBuf<int64_t> coarse_target, coarse_ptr, coarse_srcpos;
-  DBuf<TileWork> local_items, coarse_items;
-  DBuf<int64_t> local_base, coarse_base;
+  DBuf<TileWork> local_items, local_items_ll, coarse_items;
+  DBuf<int64_t> local_base, coarse_base, local_inv_base, coarse_inv_base;
   DBuf<DevSub> devsub;
   DBuf<int32_t> vec_dof;
   DBuf<double> vec_w;
@@ -119,14 +119,14 @@ struct fetidp_solver {
   DBuf<int> fail;
   DBuf<double> cpool, cinv, sinv;
   DBuf<int> cfail;
-  DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc;
+  DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc, yvec, tbuf;
   DBuf<double> lam, r, z, p, q, dv, ubar, x, xtot, rglob, partial, pf, qf;
   std::vector<double> hist, ref_hist;
   double last_dual = 0.0, init_dual = 0.0;
 
   // ---- optional event profiling (cfg.verbose & 2) ---------------------------
   // slot k accumulates milliseconds of one kernel class; see fetidp_get_profile.
-  static constexpr int kProfSlots = 24;
+  static constexpr int kProfSlots = 16;
   double prof_ms[kProfSlots] = {};
   double prof_cnt[kProfSlots] = {};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
@@ -176,7 +176,9 @@ struct fetidp_solver {
     if (st) (void)hipStreamDestroy(st);
   }
 
-  bool fp32() const { return cfg.precision == FETIDP_FP32_REFINE; }
+  bool fell_back = false;  // FP32 requested but the factor fell back to FP64
+  std::string fallback_reason;
+  bool fp32() const { return cfg.precision == FETIDP_FP32_REFINE && !fell_back; }
 
   // ---------------------------------------------------------------- setup
   void setup(int64_t n_nodes, const double* h_xyz, int64_t n_tets, const int32_t* h_t2n,
@@ -239,12 +241,15 @@ struct fetidp_solver {
     coarse_ptr.upload(plans.coarse.ptr);
     coarse_srcpos.upload(plans.coarse.srcpos);
     local_items.upload(plans.local_sched.items);
+    local_items_ll.upload(plans.local_sched_ll.items);
     coarse_items.upload(plans.coarse_sched.items);
     {
       std::vector<int64_t> base(topo.n_sub);
       for (int s = 0; s < topo.n_sub; ++s) base[s] = topo.sub[s].pool_off;
       local_base.upload(base);
       coarse_base.upload(std::vector<int64_t>{0});
+      local_inv_base.upload(plans.inv_off);
+      coarse_inv_base.upload(std::vector<int64_t>{0});
     }
     devsub.upload(plans.devsub);
     vec_dof.upload(plans.vec_dof);
@@ -269,6 +274,8 @@ struct fetidp_solver {
     f_cur.alloc(nf);
     vec.alloc(topo.vec_elems);
     vout.alloc(topo.vec_elems);
+    yvec.alloc(topo.vec_elems);
+    tbuf.alloc(topo.vec_elems);
     U.alloc(topo.vec_elems);
     ybuf.alloc(topo.y_elems);
     dbuf.alloc(topo.dvec_elems);
@@ -283,7 +290,7 @@ struct fetidp_solver {
     rglob.alloc(nf);
     partial.alloc(3 * kDotBlocks);
     cpool.alloc(plans.coarse_pool_elems);
-    cinv.alloc(kTileElems);
+    cinv.alloc(plans.coarse_inv_elems);
     sinv.alloc(size_t(topo.n_c) * size_t(std::max<int64_t>(topo.n_c, 1)));
     HIPCHECK(hipStreamSynchronize(st));
     device_ready = true;
@@ -325,14 +332,14 @@ struct fetidp_solver {
   // ---------------------------------------------------------------- factor
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& base,
-                    T* pool, T* inv, int* failp, T* P) {
+                    const DBuf<int64_t>& inv_base, T* pool, T* inv, int* failp, T* P) {
     const int slot0 = P ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     for (const TileLaunch& L : S.launches) {
       if (L.mode < 0) {
         timed(5, [&] { launch_extract_P<T>(topo.n_sub, plans.max_ndp, devsub.p, pool, P, st); });
         continue;
       }
-      timed(slot0 + L.mode, [&] { launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, pool, inv, failp, st); });
+      timed(slot0 + L.mode, [&] { launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, inv_base.p, pool, inv, failp, st); });
     }
   }
 
@@ -350,7 +357,7 @@ struct fetidp_solver {
       launch_assemble_f64(plans.rhs_global.size(), rhs_target.p, rhs_ptr.p, rhs_src.p, Fe.p, f_glob.p, st);
     // subdomain pools
     pool.alloc(size_t(topo.pool_elems));
-    inv.alloc(size_t(topo.n_sub) * kTileElems);
+    inv.alloc(size_t(plans.inv_elems));
     W.alloc(size_t(topo.w_elems));
     P.alloc(size_t(topo.p_elems));
     pool.zero(st);
@@ -363,14 +370,17 @@ struct fetidp_solver {
     launch_scaling(topo.dvec_elems, topo.n_lam, rho_pos.p, pool.p, sizeof(T) == 4, rho.p, lam_pos.p,
                    d_sgn.p, d_sgnD.p, d_half.p, st);
     fail.zero(st);
-    run_schedule<T>(plans.local_sched, local_items, local_base, pool.p, inv.p, fail.p, P.p);
+    if constexpr (sizeof(T) == 8)
+      run_schedule<T>(plans.local_sched, local_items, local_base, local_inv_base, pool.p, inv.p, fail.p, P.p);
+    else
+      run_schedule<T>(plans.local_sched_ll, local_items_ll, local_base, local_inv_base, pool.p, inv.p, fail.p, P.p);
     timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, W.p, st); });
     // coarse problem in FP64
     cpool.zero(st);
     launch_gather_pool<T>(plans.coarse.size(), coarse_target.p, coarse_ptr.p, coarse_srcpos.p,
                           pool.p, cpool.p, st);
     cfail.zero(st);
-    run_schedule<double>(plans.coarse_sched, coarse_items, coarse_base, cpool.p, cinv.p, cfail.p, nullptr);
+    run_schedule<double>(plans.coarse_sched, coarse_items, coarse_base, coarse_inv_base, cpool.p, cinv.p, cfail.p, nullptr);
     launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, sinv.p, st);
     HIPCHECK(hipStreamSynchronize(st));
     prof_collect();
@@ -389,9 +399,18 @@ struct fetidp_solver {
     ensure_device();
     factored = false;
     const double t0 = now_ms();
+    fell_back = false;
     if (fp32()) {
       pool64.release(); inv64.release(); W64.release(); P64.release();
-      factor_impl<float>(pool32, inv32, W32, P32);
+      try {
+        factor_impl<float>(pool32, inv32, W32, P32);
+      } catch (const NumericError& e) {
+        // FP32 factor lost positive definiteness: refactor in FP64 (solve follows)
+        fell_back = true;
+        fallback_reason = e.what();
+        pool32.release(); inv32.release(); W32.release(); P32.release();
+        factor_impl<double>(pool64, inv64, W64, P64);
+      }
     } else {
       pool32.release(); inv32.release(); W32.release(); P32.release();
       factor_impl<double>(pool64, inv64, W64, P64);
@@ -434,25 +453,32 @@ struct fetidp_solver {
 
   // condense the global free-dof RHS `fsrc` into the dual RHS dv (and fc, a_s)
   template <typename T>
-  void condense(const DBuf<T>& pool, const DBuf<T>& W, const double* fsrc) {
+  void condense(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, const double* fsrc) {
     launch_split_rhs(topo.vec_elems, vec_dof.p, vec_w.p, fsrc, vec.p, st);
-    launch_forward<T>(topo.n_sub, devsub.p, pool.p, vec.p, st);
+    timed(12, [&] {
+      launch_forward<T>(topo.n_sub, plans.max_T, plans.max_Tr, devsub.p, pool.p, inv.p,
+                        local_inv_base.p, vec.p, yvec.p, st);
+    });
     launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
     launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, fc.p, uc.p, st);
-    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, 1.0, dbuf.p, st);
+    // vec_D' holds -a_s: out = a_s - Phi u
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, dv.p, st);
   }
 
   // recover the global free-dof solution from lambda into xout
   template <typename T>
-  void recover(const DBuf<T>& pool, const DBuf<T>& W, double* xout) {
+  void recover(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, double* xout) {
     launch_gemv_dual<T>(topo.n_sub, plans.max_wrows, devsub.p, W.p, 0, d_lam.p, d_sgn.p, lam.p, ybuf.p, st, plans.max_ndp);
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
     launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, gc.p, uc.p, st);
-    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, 1.0, dbuf.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
-    launch_interior<T>(topo.n_sub, devsub.p, pool.p, vec.p, U.p, vout.p, st);
+    timed(13, [&] {
+      launch_interior<T>(topo.n_sub, plans.max_Ti, devsub.p, pool.p, inv.p, local_inv_base.p, yvec.p,
+                         U.p, tbuf.p, vout.p, st);
+    });
     launch_gather_global(topo.n_free, free_kind.p, free_a.p, vout.p, ubar.p, uc.p, xout, st);
   }
 
@@ -522,15 +548,15 @@ struct fetidp_solver {
   double rz_ = 0.0, prev_ = 0.0;
 
   template <typename T>
-  int solve_impl(const DBuf<T>& pool, const DBuf<T>& W, const DBuf<T>& P, const double* h_lam0,
-                 fetidp_stats& S) {
+  int solve_impl(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, const DBuf<T>& P,
+                 const double* h_lam0, fetidp_stats& S) {
     const int64_t nl = topo.n_lam, nf = topo.n_free;
     hist.clear();
     ref_hist.clear();
     int its = 0, status = FETIDP_SUCCESS;
     double dn = 0.0, res = 0.0;
     if (!fp32()) {
-      condense<T>(pool, W, f_glob.p);
+      condense<T>(pool, inv, W, f_glob.p);
       if (h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
       else lam.zero(st);
       double tol = cfg.dual_tolerance;
@@ -538,7 +564,7 @@ struct fetidp_solver {
       for (int round = 0; round < 6; ++round) {
         status = pcpg<T>(W, P, tol, cfg.max_iterations - its, fresh, its, dn);
         fresh = false;
-        recover<T>(pool, W, x.p);
+        recover<T>(pool, inv, W, x.p);
         res = global_residual(x.p, f_glob.p);
         ref_hist.push_back(res);
         if (status != FETIDP_SUCCESS || res <= cfg.tolerance) break;
@@ -559,12 +585,12 @@ struct fetidp_solver {
       res = 1.0;
       double pq_prev = 0.0;
       for (k = 0; k < cfg.max_refinements; ++k) {
-        condense<T>(pool, W, f_cur.p);
+        condense<T>(pool, inv, W, f_cur.p);
         if (k == 0 && h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
         else lam.zero(st);
         status = pcpg<T>(W, P, cfg.inner_tolerance, cfg.max_iterations - its, true, its, dn);
         if (status == FETIDP_NUMERICAL_ERROR || status == FETIDP_DIVERGED) break;
-        recover<T>(pool, W, x.p);                      // z = M32^-1 r
+        recover<T>(pool, inv, W, x.p);                      // z = M32^-1 r
         if (k == 0) {
           HIPCHECK(hipMemcpyAsync(pf.p, x.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
         } else {
@@ -604,8 +630,8 @@ struct fetidp_solver {
   int solve(const double* h_lam0, double* h_x, fetidp_stats& S) {
     if (!factored) throw InputError("solve before factor");
     const double t0 = now_ms();
-    int status = fp32() ? solve_impl<float>(pool32, W32, P32, h_lam0, S)
-                        : solve_impl<double>(pool64, W64, P64, h_lam0, S);
+    int status = fp32() ? solve_impl<float>(pool32, inv32, W32, P32, h_lam0, S)
+                        : solve_impl<double>(pool64, inv64, W64, P64, h_lam0, S);
     if (h_x) HIPCHECK(hipMemcpyAsync(h_x, xtot.p, sizeof(double) * topo.n_free, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     prof_collect();
@@ -650,19 +676,19 @@ static int32_t tile_cholesky_test(int64_t m, int64_t pc, const double* A, double
       pool[pool_pos(0, i, j)] = T(A[i * m + j]);
       if (i / kTile == j / kTile) pool[tile_offset(i / kTile, i / kTile) + (i % kTile) * kTile + j % kTile] = T(A[i * m + j]);
     }
-  TileSchedule S = single_matrix_schedule(Tp, Ta);
+  TileSchedule S = single_matrix_schedule(Tp, Ta, sizeof(T) == 8);
   DBuf<T> dpool, dinv;
   DBuf<TileWork> items;
   DBuf<int64_t> base;
   DBuf<int> fail;
   dpool.upload(pool);
-  dinv.alloc(kTileElems);
+  dinv.alloc(size_t(std::max(Tp, 1)) * kTileElems);
   items.upload(S.items);
   base.upload(std::vector<int64_t>{0});
   fail.alloc(1);
   HIPCHECK(hipMemset(fail.p, 0, sizeof(int)));
   for (const TileLaunch& L : S.launches)
-    launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, dpool.p, dinv.p, fail.p, nullptr);
+    launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, base.p, dpool.p, dinv.p, fail.p, nullptr);
   HIPCHECK(hipDeviceSynchronize());
   HIPCHECK(hipMemcpy(pool.data(), dpool.p, sizeof(T) * elems, hipMemcpyDeviceToHost));
   int hf = 0;
@@ -751,6 +777,7 @@ int32_t fetidp_solve(fetidp_handle h, const double* lambda0, double* x_free, fet
   int32_t rc = guard(h, [&] { return int32_t(h->solve(lambda0, x_free, S)); });
   if (h) {
     S.status = rc;
+    S.fell_back = h->fell_back ? 1 : 0;
     S.factor_ms = h->last_factor_ms;
     S.symbolic_ms = h->symbolic_ms;
   }
@@ -834,8 +861,8 @@ int32_t fetidp_apply_preconditioner(fetidp_handle h, const double* r, double* z)
 int32_t fetidp_get_dual_rhs(fetidp_handle h, double* d) {
   return guard(h, [&] {
     if (!d) throw InputError("null vector");
-    h->with_ops([&] { h->condense<double>(h->pool64, h->W64, h->f_glob.p); },
-                [&] { h->condense<float>(h->pool32, h->W32, h->f_glob.p); });
+    h->with_ops([&] { h->condense<double>(h->pool64, h->inv64, h->W64, h->f_glob.p); },
+                [&] { h->condense<float>(h->pool32, h->inv32, h->W32, h->f_glob.p); });
     HIPCHECK(hipStreamSynchronize(h->st));
     HIPCHECK(hipMemcpy(d, h->dv.p, sizeof(double) * h->topo.n_lam, hipMemcpyDeviceToHost));
     return int32_t(FETIDP_SUCCESS);
@@ -907,7 +934,7 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
         if (L.mode == mode)
           for (int64_t i = 0; i < L.n; ++i) {
             const TileWork& w = sch[q]->items[L.off + i];
-            fl += 2.0 * kTile * kTile * kTile * double(w.k1 - w.k0) + (mode == 1 ? 2.0 * kTile * kTile * kTile : 0.0);
+            fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (mode == 1 ? 2.0 : 0.0));
           }
       if (k < cap) out[k++] = fl;
     }

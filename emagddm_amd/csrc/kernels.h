@@ -27,10 +27,11 @@ void launch_gather_pool(int64_t n, const int64_t* target, const int64_t* ptr, co
                         const T* pool, double* out, hipStream_t st);
 
 // mode 0: Schur update list; 1: column step below the diagonal (update + trsm with
-// the inverse diagonal tile); 2: column step diagonal (update + Cholesky + inverse).
+// the stored inverse diagonal tile + diagonal-tile share); 2: diagonal Cholesky + inverse.
+// Inverse diagonal tile of column J of matrix m: inv + inv_base[m] + J * 4096.
 template <typename T>
-void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base, T* pool,
-                      T* inv, int* fail, hipStream_t st);
+void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base,
+                      const int64_t* inv_base, T* pool, T* inv, int* fail, hipStream_t st);
 
 // MFMA lane-layout probe (one wave, 512 doubles of output).
 void launch_mfma_probe(int is_f32, double* out);
@@ -42,13 +43,16 @@ void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, con
                       hipStream_t st);
 void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, double* sinv, hipStream_t st);
 
-// Forward solve y = L_rr^-1 v_r in place; then v_c <- v_c - L_cr y, v_a <- L_a r y.
+// Forward solve, one launch per block column: yv_r = L_rr^-1 vec_r; afterwards
+// vec_c = f_c - L_cr y and vec_D' = -L_D'r y (= -E K_rr^-1 f_r).
 template <typename T>
-void launch_forward(int ns, const DevSub* subs, const T* pool, double* vec, hipStream_t st);
-// Interior solve: out_i = L_ii^-T (y_i - L_{D,c; i}^T U) with y = vec (forward result).
+void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool, const T* inv,
+                    const int64_t* inv_base, double* vec, double* yv, hipStream_t st);
+// Interior solve: xout_i = L_ii^-T (yv_i - L_{D,c; i}^T U); tbuf is scratch (vec space).
 template <typename T>
-void launch_interior(int ns, const DevSub* subs, const T* pool, const double* vec, const double* U,
-                     double* out, hipStream_t st);
+void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const T* inv,
+                     const int64_t* inv_base, const double* yv, const double* U, double* tbuf,
+                     double* xout, hipStream_t st);
 // rho[slot] = pool diagonal at each D slot; stiffness scaling weights and 0.5 means.
 void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void* pool, int is_f32,
                     double* rho, const int64_t* lam_pos, const double* sgn, double* sgnD,

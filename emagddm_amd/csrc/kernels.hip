@@ -267,12 +267,17 @@ __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[2][2], c
   }
 }
 
-// MODE 0: C_IJ -= sum_K L_IK L_JK^T                     (Schur updates)
-// MODE 1: same, then X = C L_JJ^-T with the inverse tile (column step, I > J)
-// MODE 2: same, then Cholesky of the diagonal tile and its inverse (I == J)
+// MODE 0: C_IJ -= sum_K L_IK L_JK^T                                (Schur updates)
+// MODE 1: same, then X = C L_JJ^-T with the stored inverse diagonal tile; with
+//         flags & 1 the diagonal tile of row I also takes its share C_II -= X X^T
+//         (column step, I > J; every WG of a launch owns a distinct row I)
+// MODE 2: same (empty K range when the diagonal was updated incrementally), then
+//         Cholesky of the diagonal tile and its inverse
+// The inverse diagonal tile of column J of matrix m lives at inv + inv_base[m] + J*4096.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict__ work,
                                                      const int64_t* __restrict__ base,
+                                                     const int64_t* __restrict__ inv_base,
                                                      T* __restrict__ pool, T* __restrict__ inv,
                                                      int* __restrict__ fail) {
   using acc_t = typename Mfma<T>::acc_t;
@@ -315,13 +320,14 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
     __syncthreads();
     if constexpr (MODE == 1) {
       // X[i][j] = sum_k C'[i][k] Linv[j][k]:  D2[p=j][q=i], A op = Linv (p,k), B op = C' (q,k)
-      const T* Li = inv + (int64_t)w.mat * kTileElems;
+      const T* Li = inv + inv_base[w.mat] + (int64_t)w.J * kTileElems;
       acc_t a2[2][2];
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) a2[x][y] = acc_t{0, 0, 0, 0};
       tile_mma<T, 64, kLd>(a2, Li, s, pb0, qb0, lr, lk);
+      __syncthreads();  // every wave is done reading C' from s
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -330,6 +336,26 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
           for (int r = 0; r < 4; ++r) {
             const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
             C[j * 64 + i] = a2[x][y][r];
+            s[j * kLd + i] = a2[x][y][r];
+          }
+      if (!(w.flags & 1)) return;  // uniform per workgroup
+      __syncthreads();
+      // C_II -= X X^T
+      acc_t a3[2][2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) a3[x][y] = acc_t{0, 0, 0, 0};
+      tile_mma<T, kLd, kLd>(a3, s, s, pb0, qb0, lr, lk);
+      T* Cd = pool + b + tile_off(w.I, w.I);
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
+            Cd[j * 64 + i] -= a3[x][y][r];
           }
     } else {
       // Cholesky of the 64x64 tile in LDS (lower), right-looking, 256 threads
@@ -368,7 +394,7 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
         }
       }
       __syncthreads();
-      T* Li = inv + (int64_t)w.mat * kTileElems;
+      T* Li = inv + inv_base[w.mat] + (int64_t)w.J * kTileElems;
       for (int e = tid; e < kTileElems; e += 256) {
         const int i = e & 63, j = e >> 6;
         C[e] = (i >= j) ? s[j * kLd + i] : T(0);
@@ -379,20 +405,20 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
 }
 
 template <typename T>
-void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base, T* pool,
-                      T* inv, int* fail, hipStream_t st) {
+void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base,
+                      const int64_t* inv_base, T* pool, T* inv, int* fail, hipStream_t st) {
   if (n <= 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, mat_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, mat_base, inv_base, pool, inv, fail);
   else if (mode == 1)
-    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, mat_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, mat_base, inv_base, pool, inv, fail);
   else
-    hipLaunchKernelGGL((k_tile_update<T, 2>), dim3(n), dim3(256), 0, st, w, mat_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update<T, 2>), dim3(n), dim3(256), 0, st, w, mat_base, inv_base, pool, inv, fail);
 }
-template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, double*,
-                                       double*, int*, hipStream_t);
-template void launch_tile_mode<float>(int, const TileWork*, int64_t, const int64_t*, float*, float*,
-                                      int*, hipStream_t);
+template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
+                                       double*, double*, int*, hipStream_t);
+template void launch_tile_mode<float>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
+                                      float*, float*, int*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Operator extraction after the partial factorization
@@ -466,134 +492,140 @@ void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, double* s
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves with the stored partial factor (one workgroup per subdomain)
+// Triangular solves with the stored partial factor: one launch per block column,
+// one workgroup per (row tile, subdomain); diagonal blocks use the stored inverse.
 // ---------------------------------------------------------------------------
+// y = Linv * b for a 64-vector b (Linv column-major); result in ys (LDS), all threads
 template <typename T>
-__global__ __launch_bounds__(256) void k_forward(const DevSub* __restrict__ subs,
-                                                 const T* __restrict__ pool, double* __restrict__ vec) {
-  const DevSub S = subs[blockIdx.x];
-  const T* P = pool + S.pool_off;
-  double* v = vec + S.vec_off;
-  __shared__ double part[4][64];
-  __shared__ T dg[64 * kLd];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  for (int J = 0; J < S.Tr; ++J) {
-    double acc = 0.0;
-    for (int K = wv; K < J; K += 4) {
-      const T* L = P + tile_off(J, K);
-      const double* y = v + K * 64;
-#pragma unroll 8
-      for (int c = 0; c < 64; ++c) acc += (double)L[c * 64 + lane] * y[c];
-    }
-    part[wv][lane] = acc;
-    const T* D = P + tile_off(J, J);
-    for (int e = tid; e < kTileElems; e += 256) dg[(e >> 6) * kLd + (e & 63)] = D[e];
-    __syncthreads();
-    if (wv == 0) {
-      double t = v[J * 64 + lane] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-      double y = 0.0;
-      for (int c = 0; c < 64; ++c) {
-        const double yc = __shfl(t, c) / (double)dg[c * kLd + c];
-        if (lane == c) y = yc;
-        if (lane > c) t -= (double)dg[c * kLd + lane] * yc;
-      }
-      v[J * 64 + lane] = y;
-    }
-    __syncthreads();
-  }
-  // trailing rows: c rows get f_c - L_cr y, D' rows get L_D'r y
-  for (int I = S.Tr + wv; I < S.T; I += 4) {
-    const int k0 = I >= S.Tc ? S.Ti : 0;
-    double acc = 0.0;
-    for (int K = k0; K < S.Tr; ++K) {
-      const T* L = P + tile_off(I, K);
-      const double* y = v + K * 64;
-#pragma unroll 8
-      for (int c = 0; c < 64; ++c) acc += (double)L[c * 64 + lane] * y[c];
-    }
-    if (I < S.Tc)
-      v[I * 64 + lane] -= acc;
-    else
-      v[I * 64 + lane] = acc;
-  }
+__device__ __forceinline__ void tile_matvec(const T* L, const double* b, double* ys,
+                                            double (*part)[64], int tid) {
+  const int i = tid & 63, q = tid >> 6;
+  double acc = 0.0;
+#pragma unroll 4
+  for (int k = q * 16; k < q * 16 + 16; ++k) acc += (double)L[k * 64 + i] * b[k];
+  part[q][i] = acc;
+  __syncthreads();
+  if (tid < 64) ys[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+  __syncthreads();
 }
-
-template <typename T>
-void launch_forward(int ns, const DevSub* subs, const T* pool, double* vec, hipStream_t st) {
-  if (ns <= 0) return;
-  hipLaunchKernelGGL(k_forward<T>, dim3(ns), dim3(256), 0, st, subs, pool, vec);
-}
-template void launch_forward<double>(int, const DevSub*, const double*, double*, hipStream_t);
-template void launch_forward<float>(int, const DevSub*, const float*, double*, hipStream_t);
 
 // sum over column c of tile L (64x64 col-major) times u (64): thread t -> column t>>2,
-// rows (t&3)*16 .. +16; reduced across the 4 lanes sharing a column.
+// rows (t&3)*16 .. +16; the four partial sums of a column are combined with shuffles.
 template <typename T>
 __device__ __forceinline__ double colsum_part(const T* L, const double* u, int tid) {
   const int c = tid >> 2, r0 = (tid & 3) * 16;
   double s = 0.0;
 #pragma unroll
   for (int r = 0; r < 16; ++r) s += (double)L[c * 64 + r0 + r] * u[r0 + r];
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
   return s;
 }
 
+// Forward step J: y_J = L_JJ^-1 b_J (written to yv by the row-J workgroup);
+// rows I > J: b_I -= L_IJ y_J.  D' rows are structurally zero while J < Ti.
 template <typename T>
-__global__ __launch_bounds__(256) void k_interior(const DevSub* __restrict__ subs,
+__global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
-                                                  const double* __restrict__ vec,
-                                                  const double* __restrict__ U,
-                                                  double* __restrict__ out) {
-  const DevSub S = subs[blockIdx.x];
-  const T* P = pool + S.pool_off;
-  const double* yv = vec + S.vec_off;
-  double* v = out + S.vec_off;
-  const double* u = U + S.vec_off;
-  __shared__ T dg[64 * kLd];
-  __shared__ double tv[64];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  // t_K = y_K - sum_{I in [Ti, Tc)} L_IK^T U_I
-  for (int K = 0; K < S.Ti; ++K) {
-    double s = 0.0;
-    for (int I = S.Ti; I < S.Tc; ++I) s += colsum_part(P + tile_off(I, K), u + I * 64, tid);
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    if ((tid & 3) == 0) v[K * 64 + (tid >> 2)] = yv[K * 64 + (tid >> 2)] - s;
+                                                  const T* __restrict__ inv,
+                                                  const int64_t* __restrict__ inv_base, int J,
+                                                  double* __restrict__ vec, double* __restrict__ yv) {
+  const DevSub S = subs[blockIdx.y];
+  if (J >= S.Tr) return;
+  const int I = J + blockIdx.x;
+  if (I >= S.T || (J < S.Ti && I >= S.Tc)) return;
+  __shared__ double part[4][64];
+  __shared__ double ys[64];
+  const int tid = threadIdx.x;
+  double* v = vec + S.vec_off;
+  tile_matvec(inv + inv_base[blockIdx.y] + (int64_t)J * kTileElems, v + J * 64, ys, part, tid);
+  if (I == J) {
+    if (tid < 64) yv[S.vec_off + J * 64 + tid] = ys[tid];
+    return;
   }
+  const T* L = pool + S.pool_off + tile_off(I, J);
+  const int i = tid & 63, q = tid >> 6;
+  double acc = 0.0;
+#pragma unroll 4
+  for (int c = q * 16; c < q * 16 + 16; ++c) acc += (double)L[c * 64 + i] * ys[c];
+  part[q][i] = acc;
   __syncthreads();
-  // backward substitution with L_ii^T
-  for (int J = S.Ti - 1; J >= 0; --J) {
-    double s = 0.0;
-    for (int K = J + 1; K < S.Ti; ++K) s += colsum_part(P + tile_off(K, J), v + K * 64, tid);
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    if ((tid & 3) == 0) tv[tid >> 2] = s;
-    const T* D = P + tile_off(J, J);
-    for (int e = tid; e < kTileElems; e += 256) dg[(e >> 6) * kLd + (e & 63)] = D[e];
-    __syncthreads();
-    if (wv == 0) {
-      double t = v[J * 64 + lane] - tv[lane];
-      double x = 0.0;
-      for (int c = 63; c >= 0; --c) {
-        const double xc = __shfl(t, c) / (double)dg[c * kLd + c];
-        if (lane == c) x = xc;
-        if (lane < c) t -= (double)dg[lane * kLd + c] * xc;  // L[c][lane]
-      }
-      v[J * 64 + lane] = x;
-    }
-    __syncthreads();
-  }
+  if (tid < 64) v[I * 64 + tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }
 
 template <typename T>
-void launch_interior(int ns, const DevSub* subs, const T* pool, const double* vec, const double* U,
-                     double* out, hipStream_t st) {
+void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool, const T* inv,
+                    const int64_t* inv_base, double* vec, double* yv, hipStream_t st) {
   if (ns <= 0) return;
-  hipLaunchKernelGGL(k_interior<T>, dim3(ns), dim3(256), 0, st, subs, pool, vec, U, out);
+  for (int J = 0; J < max_Tr; ++J)
+    hipLaunchKernelGGL(k_fwd_step<T>, dim3(max_T - J, ns), dim3(256), 0, st, subs, pool, inv,
+                       inv_base, J, vec, yv);
 }
-template void launch_interior<double>(int, const DevSub*, const double*, const double*,
-                                      const double*, double*, hipStream_t);
-template void launch_interior<float>(int, const DevSub*, const float*, const double*, const double*,
-                                     double*, hipStream_t);
+template void launch_forward<double>(int, int, int, const DevSub*, const double*, const double*,
+                                     const int64_t*, double*, double*, hipStream_t);
+template void launch_forward<float>(int, int, int, const DevSub*, const float*, const float*,
+                                    const int64_t*, double*, double*, hipStream_t);
+
+// t_K = y_K - sum_{I in [Ti, Tc)} L_IK^T U_I for every interior tile column K
+template <typename T>
+__global__ __launch_bounds__(256) void k_int_prep(const DevSub* __restrict__ subs,
+                                                  const T* __restrict__ pool,
+                                                  const double* __restrict__ yv,
+                                                  const double* __restrict__ U,
+                                                  double* __restrict__ tbuf) {
+  const DevSub S = subs[blockIdx.y];
+  const int K = blockIdx.x;
+  if (K >= S.Ti) return;
+  const int tid = threadIdx.x;
+  const double* u = U + S.vec_off;
+  double s = 0.0;
+  for (int I = S.Ti; I < S.Tc; ++I) s += colsum_part(pool + S.pool_off + tile_off(I, K), u + I * 64, tid);
+  if ((tid & 3) == 0) tbuf[S.vec_off + K * 64 + (tid >> 2)] = yv[S.vec_off + K * 64 + (tid >> 2)] - s;
+}
+
+// Backward step J: x_J = L_JJ^-T t_J (row-J workgroup writes x);
+// rows I < J: t_I -= L_JI^T x_J.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ subs,
+                                                  const T* __restrict__ pool,
+                                                  const T* __restrict__ inv,
+                                                  const int64_t* __restrict__ inv_base, int J,
+                                                  double* __restrict__ tbuf,
+                                                  double* __restrict__ xout) {
+  const DevSub S = subs[blockIdx.y];
+  if (J >= S.Ti) return;
+  const int I = blockIdx.x;
+  if (I > J) return;
+  __shared__ double xs[64];
+  const int tid = threadIdx.x;
+  double* t = tbuf + S.vec_off;
+  const double x = colsum_part(inv + inv_base[blockIdx.y] + (int64_t)J * kTileElems, t + J * 64, tid);
+  if ((tid & 3) == 0) xs[tid >> 2] = x;
+  __syncthreads();
+  if (I == J) {
+    if (tid < 64) xout[S.vec_off + J * 64 + tid] = xs[tid];
+    return;
+  }
+  const double s = colsum_part(pool + S.pool_off + tile_off(J, I), xs, tid);
+  if ((tid & 3) == 0) t[I * 64 + (tid >> 2)] -= s;
+}
+
+template <typename T>
+void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const T* inv,
+                     const int64_t* inv_base, const double* yv, const double* U, double* tbuf,
+                     double* xout, hipStream_t st) {
+  if (ns <= 0 || max_Ti <= 0) return;
+  hipLaunchKernelGGL(k_int_prep<T>, dim3(max_Ti, ns), dim3(256), 0, st, subs, pool, yv, U, tbuf);
+  for (int J = max_Ti - 1; J >= 0; --J)
+    hipLaunchKernelGGL(k_bwd_step<T>, dim3(J + 1, ns), dim3(256), 0, st, subs, pool, inv, inv_base,
+                       J, tbuf, xout);
+}
+template void launch_interior<double>(int, int, const DevSub*, const double*, const double*,
+                                      const int64_t*, const double*, const double*, double*,
+                                      double*, hipStream_t);
+template void launch_interior<float>(int, int, const DevSub*, const float*, const float*,
+                                     const int64_t*, const double*, const double*, double*, double*,
+                                     hipStream_t);
 
 __global__ __launch_bounds__(256) void k_rho(int64_t n, const int64_t* __restrict__ pos,
                                              const void* __restrict__ pool, int is_f32,

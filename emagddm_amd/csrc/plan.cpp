@@ -45,22 +45,31 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
   TileLaunch L{mode, int64_t(S.items.size()), int64_t(list.size())};
   S.items.insert(S.items.end(), list.begin(), list.end());
   S.launches.push_back(L);
-  for (const TileWork& w : list) S.flops += 2.0 * kTile * kTile * kTile * double(w.k1 - w.k0);
+  // mode 1 adds the TRSM product (and the diagonal-tile share when flagged)
+  for (const TileWork& w : list) {
+    const double extra = mode == 1 ? 1.0 + (w.flags & 1) : 0.0;
+    S.flops += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + extra);
+  }
 }
 
 }  // namespace
 
-TileSchedule single_matrix_schedule(int Tp, int Ta) {
+// Incremental diagonal (inc = true): the column-step workgroups (mode 1, flags & 1)
+// also apply their share to the diagonal tile of their row, so diagonal items
+// carry an empty K range and Schur lists skip the diagonal -- short critical path.
+// Left-looking diagonal (inc = false): one rounding per diagonal entry and Schur
+// step, used for FP32 factors where repeated rounding of the diagonal matters.
+TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc) {
   TileSchedule S;
   for (int j = 0; j < Tp; ++j) {
-    add_tiles(S, 2, {TileWork{0, j, j, 0, j, 0}});
+    add_tiles(S, 2, {TileWork{0, j, j, 0, inc ? 0 : j, 0}});
     std::vector<TileWork> below;
-    for (int I = j + 1; I < Ta; ++I) below.push_back(TileWork{0, I, j, 0, j, 0});
+    for (int I = j + 1; I < Ta; ++I) below.push_back(TileWork{0, I, j, 0, j, inc ? 1 : 0});
     add_tiles(S, 1, below);
   }
   std::vector<TileWork> schur;
   for (int J = Tp; J < Ta; ++J)
-    for (int I = J; I < Ta; ++I) schur.push_back(TileWork{0, I, J, 0, Tp, 0});
+    for (int I = inc ? J + 1 : J; I < Ta; ++I) schur.push_back(TileWork{0, I, J, 0, Tp, 0});
   add_tiles(S, 0, schur);
   return S;
 }
@@ -217,7 +226,7 @@ void build_plans(const Topology& T, Plans& P) {
     P.coarse.ptr.push_back(int64_t(tgt.size()));
 
     // coarse schedule (single matrix, eliminate the first ncp/64 tile columns)
-    P.coarse_sched = single_matrix_schedule(int(ncp / kTile), int(Tc));
+    P.coarse_sched = single_matrix_schedule(int(ncp / kTile), int(Tc), true);
   }
 
   // ---- coarse gather positions ----------------------------------------------
@@ -236,9 +245,11 @@ void build_plans(const Topology& T, Plans& P) {
     }
   }
 
-  // ---- local factorization schedule -------------------------------------------
-  {
-    TileSchedule& S = P.local_sched;
+  // ---- local factorization schedules (incremental diagonal / left-looking) ------
+  for (int variant = 0; variant < 2; ++variant) {
+    const bool inc = variant == 0;
+    TileSchedule& S = inc ? P.local_sched : P.local_sched_ll;
+    const int fl = inc ? 1 : 0;
     int maxTi = 0, maxTd = 0;
     for (const SubLayout& L : T.sub) {
       maxTi = std::max(maxTi, L.Ti);
@@ -250,8 +261,8 @@ void build_plans(const Topology& T, Plans& P) {
       for (int64_t s = 0; s < ns; ++s) {
         const SubLayout& L = T.sub[s];
         if (j >= L.Ti) continue;
-        diag.push_back(TileWork{int32_t(s), j, j, 0, j, 0});
-        for (int I = j + 1; I < L.Tc; ++I) below.push_back(TileWork{int32_t(s), I, j, 0, j, 0});
+        diag.push_back(TileWork{int32_t(s), j, j, 0, inc ? 0 : j, 0});
+        for (int I = j + 1; I < L.Tc; ++I) below.push_back(TileWork{int32_t(s), I, j, 0, j, fl});
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 1, below);
@@ -261,7 +272,7 @@ void build_plans(const Topology& T, Plans& P) {
       const SubLayout& L = T.sub[s];
       if (L.Ti == 0) continue;
       for (int J = L.Ti; J < L.Tc; ++J)
-        for (int I = J; I < L.Tc; ++I) schur1.push_back(TileWork{int32_t(s), I, J, 0, L.Ti, 0});
+        for (int I = inc ? J + 1 : J; I < L.Tc; ++I) schur1.push_back(TileWork{int32_t(s), I, J, 0, L.Ti, 0});
     }
     add_tiles(S, 0, schur1);
     S.launches.push_back(TileLaunch{-1, 0, 0});  // snapshot S_DD for the preconditioner
@@ -272,8 +283,8 @@ void build_plans(const Topology& T, Plans& P) {
         const SubLayout& L = T.sub[s];
         const int col = L.Ti + j;
         if (col >= L.Tr) continue;
-        diag.push_back(TileWork{int32_t(s), col, col, L.Ti, col, 0});
-        for (int I = col + 1; I < L.T; ++I) below.push_back(TileWork{int32_t(s), I, col, L.Ti, col, 0});
+        diag.push_back(TileWork{int32_t(s), col, col, inc ? 0 : L.Ti, inc ? 0 : col, 0});
+        for (int I = col + 1; I < L.T; ++I) below.push_back(TileWork{int32_t(s), I, col, L.Ti, col, fl});
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 1, below);
@@ -283,7 +294,7 @@ void build_plans(const Topology& T, Plans& P) {
       const SubLayout& L = T.sub[s];
       if (L.Tr == L.Ti) continue;
       for (int J = L.Tr; J < L.T; ++J)
-        for (int I = J; I < L.T; ++I) schur2.push_back(TileWork{int32_t(s), I, J, L.Ti, L.Tr, 0});
+        for (int I = inc ? J + 1 : J; I < L.T; ++I) schur2.push_back(TileWork{int32_t(s), I, J, L.Ti, L.Tr, 0});
     }
     add_tiles(S, 0, schur2);
   }
@@ -295,11 +306,12 @@ void build_plans(const Topology& T, Plans& P) {
   for (int64_t t : P.pool.target) check(t >= 0 && t < T.pool_elems, "pool target");
   for (int64_t t : P.coarse.target) check(t >= 0 && t < P.coarse_pool_elems, "coarse target");
   for (int64_t sp : P.coarse.srcpos) check(sp < T.pool_elems, "coarse source");
-  for (const TileWork& w : P.local_sched.items) {
-    const SubLayout& L = T.sub[w.mat];
-    check(w.mat >= 0 && w.mat < ns && w.J <= w.I && w.I < L.T && w.k0 >= 0 && w.k1 <= w.J,
-          "local tile");
-  }
+  for (const TileSchedule* S : {&P.local_sched, &P.local_sched_ll})
+    for (const TileWork& w : S->items) {
+      const SubLayout& L = T.sub[w.mat];
+      check(w.mat >= 0 && w.mat < ns && w.J <= w.I && w.I < L.T && w.k0 >= 0 && w.k1 <= w.J,
+            "local tile");
+    }
   {
     const int Ta = int(2 * P.coarse_ncp / kTile);
     for (const TileWork& w : P.coarse_sched.items)
@@ -307,6 +319,22 @@ void build_plans(const Topology& T, Plans& P) {
   }
   for (int64_t y : P.c_ypos) check(y >= 0 && y < T.y_elems, "coarse y position");
   for (int64_t v : P.c_vpos) check(v >= 0 && v < T.vec_elems, "coarse v position");
+
+  // ---- inverse diagonal tiles: one per factored tile column ---------------------
+  P.inv_off.resize(ns);
+  {
+    int64_t off = 0;
+    for (int64_t s = 0; s < ns; ++s) {
+      P.inv_off[s] = off;
+      off += int64_t(T.sub[s].Tr) * kTileElems;
+      P.max_T = std::max(P.max_T, T.sub[s].T);
+      P.max_Tr = std::max(P.max_Tr, T.sub[s].Tr);
+      P.max_Ti = std::max(P.max_Ti, T.sub[s].Ti);
+    }
+    P.inv_elems = std::max<int64_t>(off, kTileElems);
+    P.coarse_inv_elems = P.coarse_ncp;  // (ncp / 64) tiles of 4096 = ncp * 64 elements
+    P.coarse_inv_elems *= kTile;
+  }
 
   // ---- device layout table ----------------------------------------------------
   P.devsub.resize(ns);

@@ -54,9 +54,12 @@ struct Plans {
   std::vector<int64_t> rho_pos;  // per D slot: pool position of its diagonal (-1 padding)
   int64_t coarse_ncp = 0;        // padded coarse order
   int64_t coarse_pool_elems = 0;
-  TileSchedule local_sched, coarse_sched;
+  TileSchedule local_sched, local_sched_ll, coarse_sched;
   std::vector<DevSub> devsub;
   int max_ndp = 0, max_ncp = 0, max_m = 0, max_wrows = 0;
+  int max_T = 0, max_Tr = 0, max_Ti = 0;
+  std::vector<int64_t> inv_off;  // per subdomain: offset of its inverse diagonal tiles
+  int64_t inv_elems = 0, coarse_inv_elems = 0;
 };
 
 // Packed index of (a, b), a <= b, in the 21-entry upper triangle of a 6x6.
@@ -66,7 +69,7 @@ void build_plans(const Topology& T, Plans& P);
 
 // Left-looking schedule of one matrix of Ta tile rows: eliminate the first Tp
 // tile columns, then update the trailing (Ta-Tp)^2 block with the Schur term.
-TileSchedule single_matrix_schedule(int Tp, int Ta);
+TileSchedule single_matrix_schedule(int Tp, int Ta, bool incremental_diagonal);
 
 inline int64_t tile_offset(int64_t I, int64_t J) { return (I * (I + 1) / 2 + J) * kTileElems; }
 inline int64_t pool_pos(int64_t base, int64_t r, int64_t c) {  // r >= c

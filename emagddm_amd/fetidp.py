@@ -38,6 +38,7 @@ class SolverResult:
     initial_dual_residual: float = 0.0
     factor_time_ms: float = 0.0
     symbolic_time_ms: float = 0.0
+    fell_back_to_fp64: bool = False
 
 
 def _dp(a):
@@ -153,7 +154,8 @@ class FETIDPSolver:
                             error_msg="" if rc == 0 else self.last_error(),
                             refinements=st.refinements, dual_residual=st.dual_residual,
                             initial_dual_residual=st.initial_dual_residual,
-                            factor_time_ms=st.factor_ms, symbolic_time_ms=st.symbolic_ms)
+                            factor_time_ms=st.factor_ms, symbolic_time_ms=st.symbolic_ms,
+                            fell_back_to_fp64=bool(st.fell_back))
 
     # --------------------------------------------------------------- introspection
     def topology(self):
@@ -208,7 +210,7 @@ class FETIDPSolver:
 
     PROFILE_SLOTS = ["integrate", "pool_assembly", "tile_schur", "tile_column", "tile_diag",
                      "extract", "coarse_schur", "coarse_column", "coarse_diag", "dual_gemv",
-                     "dual_rest", "precond_gemv"]
+                     "dual_rest", "precond_gemv", "forward_trsv", "interior_trsv"]
 
     def profile(self):
         """{kernel class: (ms, launches)} plus tile flops, from event timing."""
@@ -216,7 +218,7 @@ class FETIDPSolver:
         n = self._lib.fetidp_get_profile(self._h, _dp(buf), 64)
         out = {name: (float(buf[2 * i]), int(buf[2 * i + 1]))
                for i, name in enumerate(self.PROFILE_SLOTS)}
-        fl = buf[48:54] if n >= 54 else np.zeros(6)
+        fl = buf[32:38] if n >= 38 else np.zeros(6)
         out["flops"] = dict(zip(["tile_schur", "tile_column", "tile_diag", "coarse_schur",
                                  "coarse_column", "coarse_diag"], (float(v) for v in fl)))
         return out

@@ -51,7 +51,7 @@ typedef struct fetidp_stats {
   int32_t status;
   int32_t iterations;       /* total PCPG iterations (all refinement steps) */
   int32_t refinements;      /* outer refinement steps taken */
-  int32_t reserved;
+  int32_t fell_back;        /* 1: FP32 factor was not positive definite, refactored in FP64 */
   double residual_norm;     /* final ||f - K u||_2 / ||f||_2 (FP64, assembled system) */
   double dual_residual;     /* final PCPG ||r|| / ||d|| */
   double initial_dual_residual; /* ||d - F lambda0|| / ||d|| */
@@ -137,7 +137,8 @@ int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S);
 /* Event profile when cfg.verbose & 2: out[2k], out[2k+1] = milliseconds and
  * launch count of kernel class k (0 integrate, 1 pool assembly, 2..4 local tile
  * modes 0..2, 5 operator extraction, 6..8 coarse tile modes 0..2, 9 dual GEMV
- * of F, 10 rest of F, 11 preconditioner GEMV), then the algorithmic flops of
+ * of F, 10 rest of F, 11 preconditioner GEMV, 12 forward solve, 13 interior
+ * solve, 14-15 spare), then the algorithmic flops of
  * local modes 0..2 and coarse modes 0..2.  Returns the number of values written. */
 int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap);
 void fetidp_reset_profile(fetidp_handle h);
