@@ -104,6 +104,7 @@ struct fetidp_solver {
   DBuf<int64_t> coarse_target, coarse_ptr, coarse_srcpos;
   DBuf<TileWork> local_items, local_items_ll, coarse_items;
   DBuf<int64_t> local_base, coarse_base, local_inv_base, coarse_inv_base;
+  DBuf<int32_t> ftile;
   DBuf<DevSub> devsub;
   DBuf<int32_t> vec_dof;
   DBuf<double> vec_w;
@@ -114,8 +115,9 @@ struct fetidp_solver {
   DBuf<int64_t> c_ptr, c_ypos, c_vpos;
   DBuf<int8_t> free_kind;
   DBuf<int64_t> free_a;
-  DBuf<double> pool64, inv64, W64, P64;
-  DBuf<float> pool32, inv32, W32, P32;
+  DBuf<double> pool64, inv64, W64;
+  DBuf<float> pool32, inv32, W32;
+  DBuf<float> Pf;  // Dirichlet preconditioner blocks, FP32 for both precisions
   DBuf<int> fail;
   DBuf<double> cpool, cinv, sinv;
   DBuf<int> cfail;
@@ -249,6 +251,7 @@ struct fetidp_solver {
       local_base.upload(base);
       coarse_base.upload(std::vector<int64_t>{0});
       local_inv_base.upload(plans.inv_off);
+      ftile.upload(plans.ftile);
       coarse_inv_base.upload(std::vector<int64_t>{0});
     }
     devsub.upload(plans.devsub);
@@ -332,7 +335,7 @@ struct fetidp_solver {
   // ---------------------------------------------------------------- factor
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& base,
-                    const DBuf<int64_t>& inv_base, T* pool, T* inv, int* failp, T* P) {
+                    const DBuf<int64_t>& inv_base, T* pool, T* inv, int* failp, float* P) {
     const int slot0 = P ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     for (const TileLaunch& L : S.launches) {
       if (L.mode < 0) {
@@ -344,7 +347,7 @@ struct fetidp_solver {
   }
 
   template <typename T>
-  void factor_impl(DBuf<T>& pool, DBuf<T>& inv, DBuf<T>& W, DBuf<T>& P) {
+  void factor_impl(DBuf<T>& pool, DBuf<T>& inv, DBuf<T>& W, DBuf<float>& P) {
     const int64_t nt = topo.n_tets;
     // element matrices A_e = K_e + M_e (and F_e when the RHS is given by J)
     timed(0, [&] {
@@ -401,19 +404,19 @@ struct fetidp_solver {
     const double t0 = now_ms();
     fell_back = false;
     if (fp32()) {
-      pool64.release(); inv64.release(); W64.release(); P64.release();
+      pool64.release(); inv64.release(); W64.release();
       try {
-        factor_impl<float>(pool32, inv32, W32, P32);
+        factor_impl<float>(pool32, inv32, W32, Pf);
       } catch (const NumericError& e) {
         // FP32 factor lost positive definiteness: refactor in FP64 (solve follows)
         fell_back = true;
         fallback_reason = e.what();
-        pool32.release(); inv32.release(); W32.release(); P32.release();
-        factor_impl<double>(pool64, inv64, W64, P64);
+        pool32.release(); inv32.release(); W32.release();
+        factor_impl<double>(pool64, inv64, W64, Pf);
       }
     } else {
-      pool32.release(); inv32.release(); W32.release(); P32.release();
-      factor_impl<double>(pool64, inv64, W64, P64);
+      pool32.release(); inv32.release(); W32.release();
+      factor_impl<double>(pool64, inv64, W64, Pf);
     }
     factored = true;
     return now_ms() - t0;
@@ -445,9 +448,8 @@ struct fetidp_solver {
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
-  template <typename T>
-  void apply_M(const DBuf<T>& P, const double* rin, double* zout) {
-    timed(11, [&] { launch_gemv_dual<T>(topo.n_sub, plans.max_ndp, devsub.p, P.p, 1, d_lam.p, d_sgnD.p, rin, dbuf.p, st, plans.max_ndp); });
+  void apply_M(const double* rin, double* zout) {
+    timed(11, [&] { launch_gemv_dual<float>(topo.n_sub, plans.max_ndp, devsub.p, Pf.p, 1, d_lam.p, d_sgnD.p, rin, dbuf.p, st, plans.max_ndp); });
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgnD.p, dbuf.p, zout, st);
   }
 
@@ -457,7 +459,7 @@ struct fetidp_solver {
     launch_split_rhs(topo.vec_elems, vec_dof.p, vec_w.p, fsrc, vec.p, st);
     timed(12, [&] {
       launch_forward<T>(topo.n_sub, plans.max_T, plans.max_Tr, devsub.p, pool.p, inv.p,
-                        local_inv_base.p, vec.p, yvec.p, st);
+                        local_inv_base.p, ftile.p, vec.p, yvec.p, st);
     });
     launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
     launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, fc.p, uc.p, st);
@@ -476,8 +478,8 @@ struct fetidp_solver {
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
     timed(13, [&] {
-      launch_interior<T>(topo.n_sub, plans.max_Ti, devsub.p, pool.p, inv.p, local_inv_base.p, yvec.p,
-                         U.p, tbuf.p, vout.p, st);
+      launch_interior<T>(topo.n_sub, plans.max_Ti, devsub.p, pool.p, inv.p, local_inv_base.p, ftile.p,
+                         yvec.p, U.p, tbuf.p, vout.p, st);
     });
     launch_gather_global(topo.n_free, free_kind.p, free_a.p, vout.p, ubar.p, uc.p, xout, st);
   }
@@ -494,7 +496,7 @@ struct fetidp_solver {
   // PCPG on F lambda = d from the current lambda until ||r||/||d|| < tol.
   // `fresh` recomputes r, z, p from lambda; otherwise continues the recurrence.
   template <typename T>
-  int pcpg(const DBuf<T>& W, const DBuf<T>& P, double tol, int max_it, bool fresh, int& its,
+  int pcpg(const DBuf<T>& W, double tol, int max_it, bool fresh, int& its,
            double& dn) {
     const int64_t nl = topo.n_lam;
     double o[3];
@@ -508,7 +510,7 @@ struct fetidp_solver {
       apply_F<T>(W, lam.p, q.p);
       HIPCHECK(hipMemcpyAsync(r.p, dv.p, sizeof(double) * nl, hipMemcpyDeviceToDevice, st));
       launch_axpy(nl, -1.0, q.p, r.p, st);
-      apply_M<T>(P, r.p, z.p);
+      apply_M(r.p, z.p);
       const double* a[2] = {r.p, r.p};
       const double* b[2] = {r.p, z.p};
       dots(2, a, b, nl, o);
@@ -527,7 +529,7 @@ struct fetidp_solver {
       if (!(std::fabs(pq) >= 1e-300)) return FETIDP_NUMERICAL_ERROR;
       const double alpha = rz_ / pq;
       launch_axpy2(nl, alpha, p.p, q.p, lam.p, r.p, st);
-      apply_M<T>(P, r.p, z.p);
+      apply_M(r.p, z.p);
       const double* a[2] = {r.p, r.p};
       const double* b[2] = {r.p, z.p};
       dots(2, a, b, nl, o);
@@ -548,7 +550,7 @@ struct fetidp_solver {
   double rz_ = 0.0, prev_ = 0.0;
 
   template <typename T>
-  int solve_impl(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, const DBuf<T>& P,
+  int solve_impl(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W,
                  const double* h_lam0, fetidp_stats& S) {
     const int64_t nl = topo.n_lam, nf = topo.n_free;
     hist.clear();
@@ -562,7 +564,7 @@ struct fetidp_solver {
       double tol = cfg.dual_tolerance;
       bool fresh = true;
       for (int round = 0; round < 6; ++round) {
-        status = pcpg<T>(W, P, tol, cfg.max_iterations - its, fresh, its, dn);
+        status = pcpg<T>(W, tol, cfg.max_iterations - its, fresh, its, dn);
         fresh = false;
         recover<T>(pool, inv, W, x.p);
         res = global_residual(x.p, f_glob.p);
@@ -588,7 +590,7 @@ struct fetidp_solver {
         condense<T>(pool, inv, W, f_cur.p);
         if (k == 0 && h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
         else lam.zero(st);
-        status = pcpg<T>(W, P, cfg.inner_tolerance, cfg.max_iterations - its, true, its, dn);
+        status = pcpg<T>(W, cfg.inner_tolerance, cfg.max_iterations - its, true, its, dn);
         if (status == FETIDP_NUMERICAL_ERROR || status == FETIDP_DIVERGED) break;
         recover<T>(pool, inv, W, x.p);                      // z = M32^-1 r
         if (k == 0) {
@@ -630,8 +632,8 @@ struct fetidp_solver {
   int solve(const double* h_lam0, double* h_x, fetidp_stats& S) {
     if (!factored) throw InputError("solve before factor");
     const double t0 = now_ms();
-    int status = fp32() ? solve_impl<float>(pool32, inv32, W32, P32, h_lam0, S)
-                        : solve_impl<double>(pool64, inv64, W64, P64, h_lam0, S);
+    int status = fp32() ? solve_impl<float>(pool32, inv32, W32, h_lam0, S)
+                        : solve_impl<double>(pool64, inv64, W64, h_lam0, S);
     if (h_x) HIPCHECK(hipMemcpyAsync(h_x, xtot.p, sizeof(double) * topo.n_free, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     prof_collect();
@@ -707,7 +709,7 @@ void fetidp_config_default(fetidp_config* cfg) {
   cfg->precision = FETIDP_FP64;
   cfg->max_iterations = 1000;
   cfg->tolerance = 1e-8;
-  cfg->dual_tolerance = 1e-10;
+  cfg->dual_tolerance = 1e-11;
   cfg->max_refinements = 12;
   cfg->verbose = 0;
   cfg->inner_tolerance = 1e-2;
@@ -724,7 +726,7 @@ fetidp_handle fetidp_create(const fetidp_config* cfg) {
     }
     if (h->cfg.max_iterations <= 0) h->cfg.max_iterations = 1000;
     if (!(h->cfg.tolerance > 0)) h->cfg.tolerance = 1e-8;
-    if (!(h->cfg.dual_tolerance > 0)) h->cfg.dual_tolerance = 1e-10;
+    if (!(h->cfg.dual_tolerance > 0)) h->cfg.dual_tolerance = 1e-11;
     if (h->cfg.max_refinements <= 0) h->cfg.max_refinements = 12;
     if (!(h->cfg.inner_tolerance > 0)) h->cfg.inner_tolerance = 1e-2;
     return h;
@@ -842,12 +844,12 @@ static int32_t dual_io(fetidp_handle h, const double* in, double* out, int which
         [&] {
           HIPCHECK(hipMemcpy(h->p.p, in, sizeof(double) * nl, hipMemcpyHostToDevice));
           if (which == 0) h->apply_F<double>(h->W64, h->p.p, h->q.p);
-          else h->apply_M<double>(h->P64, h->p.p, h->q.p);
+          else h->apply_M(h->p.p, h->q.p);
         },
         [&] {
           HIPCHECK(hipMemcpy(h->p.p, in, sizeof(double) * nl, hipMemcpyHostToDevice));
           if (which == 0) h->apply_F<float>(h->W32, h->p.p, h->q.p);
-          else h->apply_M<float>(h->P32, h->p.p, h->q.p);
+          else h->apply_M(h->p.p, h->q.p);
         });
     HIPCHECK(hipStreamSynchronize(h->st));
     HIPCHECK(hipMemcpy(out, h->q.p, sizeof(double) * nl, hipMemcpyDeviceToHost));

@@ -37,7 +37,8 @@ void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat
 void launch_mfma_probe(int is_f32, double* out);
 
 template <typename T>
-void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, T* P, hipStream_t st);
+void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, float* P,
+                      hipStream_t st);
 template <typename T>
 void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool, T* W,
                       hipStream_t st);
@@ -47,12 +48,13 @@ void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, double* s
 // vec_c = f_c - L_cr y and vec_D' = -L_D'r y (= -E K_rr^-1 f_r).
 template <typename T>
 void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool, const T* inv,
-                    const int64_t* inv_base, double* vec, double* yv, hipStream_t st);
+                    const int64_t* inv_base, const int32_t* ftile, double* vec, double* yv,
+                    hipStream_t st);
 // Interior solve: xout_i = L_ii^-T (yv_i - L_{D,c; i}^T U); tbuf is scratch (vec space).
 template <typename T>
 void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const T* inv,
-                     const int64_t* inv_base, const double* yv, const double* U, double* tbuf,
-                     double* xout, hipStream_t st);
+                     const int64_t* inv_base, const int32_t* ftile, const double* yv,
+                     const double* U, double* tbuf, double* xout, hipStream_t st);
 // rho[slot] = pool diagonal at each D slot; stiffness scaling weights and 0.5 means.
 void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void* pool, int is_f32,
                     double* rho, const int64_t* lam_pos, const double* sgn, double* sgnD,

@@ -222,6 +222,23 @@ struct Mfma<float> {
 
 constexpr int kLd = 65;  // padded LDS leading dimension for a 64x64 tile
 
+// sum_{j<n} a[j*as] * b[j*bs] with four independent accumulators so that the LDS
+// loads of consecutive terms overlap instead of serializing on load latency.
+template <typename T>
+__device__ __forceinline__ T dot_strided(const T* a, int as, const T* b, int bs, int n) {
+  T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+  int j = 0;
+#pragma unroll 2
+  for (; j + 4 <= n; j += 4) {
+    s0 += a[j * as] * b[j * bs];
+    s1 += a[(j + 1) * as] * b[(j + 1) * bs];
+    s2 += a[(j + 2) * as] * b[(j + 2) * bs];
+    s3 += a[(j + 3) * as] * b[(j + 3) * bs];
+  }
+  for (; j < n; ++j) s0 += a[j * as] * b[j * bs];
+  return (s0 + s1) + (s2 + s3);
+}
+
 // Lane-layout probe of the 16x16x4 MFMA builtins: with A[p][0] = p (A[p][k>0] = 0)
 // and B[0][q] = 1 the product holds D[p][q] = p, with A[p][0] = 1 and B[0][q] = q
 // it holds q; each lane writes its 4 accumulator values for both products.
@@ -267,6 +284,58 @@ __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[2][2], c
   }
 }
 
+// acc += A_panel * B_panel^T over ktot (multiple of 16) columns.  A row panel of
+// tiles (J, k0 .. k1) is contiguous: element (p, k) at k*64 + p, so the K loop is
+// one panel product.  64x16 chunks of both panels are staged through LDS (two
+// stages, next chunk prefetched into registers), so each element crosses L2 once
+// per workgroup; the four waves then read their MFMA operands from LDS.
+template <typename T>
+__device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
+                                          const T* __restrict__ Ap, const T* __restrict__ Bq,
+                                          int ktot, T* sA, T* sB, int tid, int pb0, int qb0,
+                                          int lr, int lk) {
+  constexpr int KC = 16, CH = 64 * KC;
+  const int nst = ktot / KC;
+  if (nst == 0) return;
+  T ra[4], rb[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    ra[u] = Ap[tid + 256 * u];
+    rb[u] = Bq[tid + 256 * u];
+  }
+  for (int c = 0; c < nst; ++c) {
+    T* a = sA + (c & 1) * CH;
+    T* bb = sB + (c & 1) * CH;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[tid + 256 * u] = ra[u];
+      bb[tid + 256 * u] = rb[u];
+    }
+    __syncthreads();
+    if (c + 1 < nst) {
+      const int64_t o = (int64_t)(c + 1) * CH;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ra[u] = Ap[o + tid + 256 * u];
+        rb[u] = Bq[o + tid + 256 * u];
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
+      const int kc = kk + lk;
+      const T a0 = a[kc * 64 + pb0 + lr];
+      const T a1 = a[kc * 64 + pb0 + 16 + lr];
+      const T b0 = bb[kc * 64 + qb0 + lr];
+      const T b1 = bb[kc * 64 + qb0 + 16 + lr];
+      acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+    }
+  }
+  __syncthreads();
+}
+
 // MODE 0: C_IJ -= sum_K L_IK L_JK^T                                (Schur updates)
 // MODE 1: same, then X = C L_JJ^-T with the stored inverse diagonal tile; with
 //         flags & 1 the diagonal tile of row I also takes its share C_II -= X X^T
@@ -293,8 +362,11 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) acc[x][y] = acc_t{0, 0, 0, 0};
-  for (int K = w.k0; K < w.k1; ++K)
-    tile_mma<T, 64, 64>(acc, pool + b + tile_off(w.J, K), pool + b + tile_off(w.I, K), pb0, qb0, lr, lk);
+  {
+    __shared__ T sA[2 * 64 * 16], sB[2 * 64 * 16];
+    panel_mma<T>(acc, pool + b + tile_off(w.J, w.k0), pool + b + tile_off(w.I, w.k0),
+                 (w.k1 - w.k0) * 64, sA, sB, tid, pb0, qb0, lr, lk);
+  }
 
   if constexpr (MODE == 0) {
 #pragma unroll
@@ -358,39 +430,44 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
             Cd[j * 64 + i] -= a3[x][y][r];
           }
     } else {
-      // Cholesky of the 64x64 tile in LDS (lower), right-looking, 256 threads
+      // Cholesky of the 64x64 tile and its inverse by wave 0 alone (lane = row for
+      // the factor, lane = column for the inverse): no workgroup barriers, LDS
+      // reads are either broadcasts or lane-contiguous.
       __shared__ T si[64 * kLd];
-      int bad = 0;
-      for (int k = 0; k < 64; ++k) {
-        __syncthreads();
-        T piv = s[k * kLd + k];
-        T d;
-        if (!(piv > T(0)) || !(piv < T(3.0e38))) {
-          bad = 1;
-          d = T(1);
-        } else {
-          d = sqrt(piv);
-        }
-        __syncthreads();
-        if (tid == 0) s[k * kLd + k] = d;
-        for (int i = k + 1 + tid; i < 64; i += 256) s[k * kLd + i] /= d;
-        __syncthreads();
+      __shared__ T rdg[64];
+      if (wave == 0) {
         const int i = lane;
-        for (int j = k + 1 + wave; j < 64; j += 4)
-          if (i >= j) s[j * kLd + i] -= s[k * kLd + i] * s[k * kLd + j];
-      }
-      __syncthreads();
-      if (bad && tid == 0) atomicOr(fail + w.mat, 1);
-      // inverse of the lower factor, one column per thread of wave 0
-      if (tid < 64) {
-        const int cidx = tid;
-        for (int i = 0; i < cidx; ++i) si[cidx * kLd + i] = T(0);
-        T y = T(1) / s[cidx * kLd + cidx];
-        si[cidx * kLd + cidx] = y;
-        for (int i = cidx + 1; i < 64; ++i) {
-          T acc1 = T(0);
-          for (int k = cidx; k < i; ++k) acc1 += s[k * kLd + i] * si[cidx * kLd + k];
-          si[cidx * kLd + i] = -acc1 / s[i * kLd + i];
+        int bad = 0;
+        for (int k = 0; k < 64; ++k) {
+          const T v = dot_strided(s + i, kLd, s + k, kLd, k);
+          const T piv = s[k * kLd + k] - __shfl(v, k);
+          T d;
+          if (!(piv > T(0)) || !(piv < T(3.0e38))) {
+            bad = 1;
+            d = T(1);
+          } else {
+            d = sqrt(piv);
+          }
+          const T rd = T(1) / d;
+          if (i > k) s[k * kLd + i] = (s[k * kLd + i] - v) * rd;
+          if (i == k) {
+            s[k * kLd + k] = d;
+            rdg[k] = rd;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (bad && lane == 0) atomicOr(fail + w.mat, 1);
+        // inverse: column c = lane, y_i = -(sum_{k<i} L_ik y_k) / L_ii, y_c = 1 / L_cc
+        const int c = lane;
+        for (int ii = 0; ii < 64; ++ii) {
+          const T acc1 = dot_strided(s + ii, kLd, si + c * kLd, 1, ii);
+          const T y = ii < c ? T(0) : (ii == c ? rdg[ii] : -acc1 * rdg[ii]);
+          si[c * kLd + ii] = y;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
       }
       __syncthreads();
@@ -431,21 +508,25 @@ __device__ __forceinline__ T pool_sym(const T* P, int a, int b) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_extract_P(const DevSub* __restrict__ subs,
-                                                   const T* __restrict__ pool, T* __restrict__ P) {
+                                                   const T* __restrict__ pool, float* __restrict__ P) {
   const DevSub S = subs[blockIdx.y];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)S.ndp * S.ndp) return;
   const int a = int(e / S.ndp), b = int(e % S.ndp);
-  P[S.p_off + e] = pool_sym(pool + S.pool_off, S.o_d + a, S.o_d + b);
+  P[S.p_off + e] = (float)pool_sym(pool + S.pool_off, S.o_d + a, S.o_d + b);
 }
 
+// The Dirichlet preconditioner is stored in FP32 for both factor precisions: it
+// only steers PCPG, the dual operator F and the residual checks stay in the
+// factor precision / FP64.
 template <typename T>
-void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, T* P, hipStream_t st) {
+void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, float* P,
+                      hipStream_t st) {
   const int64_t n = (int64_t)max_ndp * max_ndp;
   if (n <= 0 || ns <= 0) return;
   hipLaunchKernelGGL(k_extract_P<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, P);
 }
-template void launch_extract_P<double>(int, int, const DevSub*, const double*, double*, hipStream_t);
+template void launch_extract_P<double>(int, int, const DevSub*, const double*, float*, hipStream_t);
 template void launch_extract_P<float>(int, int, const DevSub*, const float*, float*, hipStream_t);
 
 // W = [Z; Phi^T] from the trailing block: Z = -A(D',D'), Phi = -A(D',c)
@@ -528,12 +609,13 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
                                                   const T* __restrict__ inv,
-                                                  const int64_t* __restrict__ inv_base, int J,
+                                                  const int64_t* __restrict__ inv_base,
+                                                  const int32_t* __restrict__ ftile, int J,
                                                   double* __restrict__ vec, double* __restrict__ yv) {
   const DevSub S = subs[blockIdx.y];
   if (J >= S.Tr) return;
   const int I = J + blockIdx.x;
-  if (I >= S.T || (J < S.Ti && I >= S.Tc)) return;
+  if (I >= S.T || ftile[S.ft_off + I] > J) return;  // tile (I, J) outside the envelope
   __shared__ double part[4][64];
   __shared__ double ys[64];
   const int tid = threadIdx.x;
@@ -555,21 +637,23 @@ __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ sub
 
 template <typename T>
 void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool, const T* inv,
-                    const int64_t* inv_base, double* vec, double* yv, hipStream_t st) {
+                    const int64_t* inv_base, const int32_t* ftile, double* vec, double* yv,
+                    hipStream_t st) {
   if (ns <= 0) return;
   for (int J = 0; J < max_Tr; ++J)
     hipLaunchKernelGGL(k_fwd_step<T>, dim3(max_T - J, ns), dim3(256), 0, st, subs, pool, inv,
-                       inv_base, J, vec, yv);
+                       inv_base, ftile, J, vec, yv);
 }
 template void launch_forward<double>(int, int, int, const DevSub*, const double*, const double*,
-                                     const int64_t*, double*, double*, hipStream_t);
+                                     const int64_t*, const int32_t*, double*, double*, hipStream_t);
 template void launch_forward<float>(int, int, int, const DevSub*, const float*, const float*,
-                                    const int64_t*, double*, double*, hipStream_t);
+                                    const int64_t*, const int32_t*, double*, double*, hipStream_t);
 
 // t_K = y_K - sum_{I in [Ti, Tc)} L_IK^T U_I for every interior tile column K
 template <typename T>
 __global__ __launch_bounds__(256) void k_int_prep(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
+                                                  const int32_t* __restrict__ ftile,
                                                   const double* __restrict__ yv,
                                                   const double* __restrict__ U,
                                                   double* __restrict__ tbuf) {
@@ -579,7 +663,8 @@ __global__ __launch_bounds__(256) void k_int_prep(const DevSub* __restrict__ sub
   const int tid = threadIdx.x;
   const double* u = U + S.vec_off;
   double s = 0.0;
-  for (int I = S.Ti; I < S.Tc; ++I) s += colsum_part(pool + S.pool_off + tile_off(I, K), u + I * 64, tid);
+  for (int I = S.Ti; I < S.Tc; ++I)
+    if (ftile[S.ft_off + I] <= K) s += colsum_part(pool + S.pool_off + tile_off(I, K), u + I * 64, tid);
   if ((tid & 3) == 0) tbuf[S.vec_off + K * 64 + (tid >> 2)] = yv[S.vec_off + K * 64 + (tid >> 2)] - s;
 }
 
@@ -589,13 +674,14 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
                                                   const T* __restrict__ inv,
-                                                  const int64_t* __restrict__ inv_base, int J,
+                                                  const int64_t* __restrict__ inv_base,
+                                                  const int32_t* __restrict__ ftile, int J,
                                                   double* __restrict__ tbuf,
                                                   double* __restrict__ xout) {
   const DevSub S = subs[blockIdx.y];
   if (J >= S.Ti) return;
   const int I = blockIdx.x;
-  if (I > J) return;
+  if (I > J || I < ftile[S.ft_off + J]) return;  // tile (J, I) outside the envelope
   __shared__ double xs[64];
   const int tid = threadIdx.x;
   double* t = tbuf + S.vec_off;
@@ -612,20 +698,20 @@ __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ sub
 
 template <typename T>
 void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const T* inv,
-                     const int64_t* inv_base, const double* yv, const double* U, double* tbuf,
-                     double* xout, hipStream_t st) {
+                     const int64_t* inv_base, const int32_t* ftile, const double* yv,
+                     const double* U, double* tbuf, double* xout, hipStream_t st) {
   if (ns <= 0 || max_Ti <= 0) return;
-  hipLaunchKernelGGL(k_int_prep<T>, dim3(max_Ti, ns), dim3(256), 0, st, subs, pool, yv, U, tbuf);
+  hipLaunchKernelGGL(k_int_prep<T>, dim3(max_Ti, ns), dim3(256), 0, st, subs, pool, ftile, yv, U, tbuf);
   for (int J = max_Ti - 1; J >= 0; --J)
     hipLaunchKernelGGL(k_bwd_step<T>, dim3(J + 1, ns), dim3(256), 0, st, subs, pool, inv, inv_base,
-                       J, tbuf, xout);
+                       ftile, J, tbuf, xout);
 }
 template void launch_interior<double>(int, int, const DevSub*, const double*, const double*,
-                                      const int64_t*, const double*, const double*, double*,
-                                      double*, hipStream_t);
+                                      const int64_t*, const int32_t*, const double*, const double*,
+                                      double*, double*, hipStream_t);
 template void launch_interior<float>(int, int, const DevSub*, const float*, const float*,
-                                     const int64_t*, const double*, const double*, double*, double*,
-                                     hipStream_t);
+                                     const int64_t*, const int32_t*, const double*, const double*,
+                                     double*, double*, hipStream_t);
 
 __global__ __launch_bounds__(256) void k_rho(int64_t n, const int64_t* __restrict__ pos,
                                              const void* __restrict__ pool, int is_f32,

@@ -59,17 +59,22 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
 // carry an empty K range and Schur lists skip the diagonal -- short critical path.
 // Left-looking diagonal (inc = false): one rounding per diagonal entry and Schur
 // step, used for FP32 factors where repeated rounding of the diagonal matters.
-TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc) {
+TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc, const int32_t* ft) {
   TileSchedule S;
+  auto f = [&](int I) { return ft ? ft[I] : 0; };
   for (int j = 0; j < Tp; ++j) {
-    add_tiles(S, 2, {TileWork{0, j, j, 0, inc ? 0 : j, 0}});
+    add_tiles(S, 2, {TileWork{0, j, j, inc ? 0 : f(j), inc ? 0 : j, 0}});
     std::vector<TileWork> below;
-    for (int I = j + 1; I < Ta; ++I) below.push_back(TileWork{0, I, j, 0, j, inc ? 1 : 0});
+    for (int I = j + 1; I < Ta; ++I)
+      if (f(I) <= j) below.push_back(TileWork{0, I, j, std::max(f(I), f(j)), j, inc ? 1 : 0});
     add_tiles(S, 1, below);
   }
   std::vector<TileWork> schur;
   for (int J = Tp; J < Ta; ++J)
-    for (int I = inc ? J + 1 : J; I < Ta; ++I) schur.push_back(TileWork{0, I, J, 0, Tp, 0});
+    for (int I = inc ? J + 1 : J; I < Ta; ++I) {
+      const int k0 = std::max(f(I), f(J));
+      if (k0 < Tp) schur.push_back(TileWork{0, I, J, k0, Tp, 0});
+    }
   add_tiles(S, 0, schur);
   return S;
 }
@@ -91,9 +96,16 @@ void build_plans(const Topology& T, Plans& P) {
     P.vec_dof.assign(T.vec_elems, -1);
     P.vec_w.assign(T.vec_elems, 0.0);
     P.rho_pos.assign(T.dvec_elems, -1);
+    P.ftile_off.resize(ns + 1, 0);
+    for (int64_t s = 0; s < ns; ++s) P.ftile_off[s + 1] = P.ftile_off[s] + T.sub[s].T;
+    P.ftile.assign(P.ftile_off[ns], 0);
     for (int64_t s = 0; s < ns; ++s) {
       const SubLayout& L = T.sub[s];
       const int32_t* dofs = &T.sub_dof[T.sub_dof_ptr[s]];
+      // first nonzero column of every local row (lower triangle); the Cholesky
+      // factor keeps this envelope, so tiles left of it stay zero
+      std::vector<int32_t> frow(L.m);
+      for (int r = 0; r < L.m; ++r) frow[r] = r;
       const int n = L.ni + L.nd + L.nc;
       auto padded = [&](int k) { return k < L.ni ? k : (k < L.ni + L.nd ? L.o_d + (k - L.ni) : L.o_c + (k - L.ni - L.nd)); };
       for (int k = 0; k < n; ++k) {
@@ -116,6 +128,7 @@ void build_plans(const Topology& T, Plans& P) {
             const int32_t c = enc(int64_t(packed_index(a, b)) * nt + t, T.tet_sign[6 * t + a] * T.tet_sign[6 * t + b]);
             tgt.push_back(uint64_t(pool_pos(L.pool_off, la, lb)));
             code.push_back(c);
+            frow[la] = std::min(frow[la], lb);
             if (la != lb && la / kTile == lb / kTile) {
               tgt.push_back(uint64_t(L.pool_off + tile_offset(la / kTile, la / kTile) + (la % kTile) * kTile + lb % kTile));
               code.push_back(c);
@@ -133,8 +146,15 @@ void build_plans(const Topology& T, Plans& P) {
       for (int k = 0; k < L.nd; ++k) {
         tgt.push_back(uint64_t(pool_pos(L.pool_off, L.o_a + k, L.o_d + k)));
         code.push_back(int32_t(one_code));
+        frow[L.o_a + k] = std::min(frow[L.o_a + k], L.o_d + k);
       }
       for (int k = 0; k < n; ++k) loc[dofs[k]] = -1;
+      int32_t* ft = &P.ftile[P.ftile_off[s]];
+      for (int I = 0; I < L.T; ++I) {
+        int f = I;
+        for (int r = I * kTile; r < (I + 1) * kTile; ++r) f = std::min(f, frow[r] / kTile);
+        ft[I] = f;
+      }
     }
     group_emissions(tgt, code, uint64_t(T.pool_elems), P.pool.target, P.pool.ptr, P.pool.src);
   }
@@ -188,6 +208,9 @@ void build_plans(const Topology& T, Plans& P) {
     P.coarse_pool_elems = Tc * (Tc + 1) / 2 * kTileElems;
     std::vector<uint64_t> tgt;
     std::vector<int64_t> srcpos;
+    std::vector<int64_t> cfrow(2 * ncp);
+    for (int64_t r = 0; r < 2 * ncp; ++r) cfrow[r] = r;
+    for (int64_t k = 0; k < nc; ++k) cfrow[ncp + k] = k;
     for (int64_t s = 0; s < ns; ++s) {
       const SubLayout& L = T.sub[s];
       for (int ka = 0; ka < L.nc; ++ka) {
@@ -195,6 +218,7 @@ void build_plans(const Topology& T, Plans& P) {
         for (int kb = 0; kb <= ka; ++kb) {
           const int64_t Cb = T.c_loc[L.cvec_off + kb];
           if (Ca < Cb) throw std::runtime_error("primal order not monotone");
+          cfrow[Ca] = std::min(cfrow[Ca], Cb);
           const int64_t sp = pool_pos(L.pool_off, L.o_c + ka, L.o_c + kb);
           tgt.push_back(uint64_t(pool_pos(0, Ca, Cb)));
           srcpos.push_back(sp);
@@ -226,7 +250,13 @@ void build_plans(const Topology& T, Plans& P) {
     P.coarse.ptr.push_back(int64_t(tgt.size()));
 
     // coarse schedule (single matrix, eliminate the first ncp/64 tile columns)
-    P.coarse_sched = single_matrix_schedule(int(ncp / kTile), int(Tc), true);
+    P.coarse_ftile.assign(Tc, 0);
+    for (int64_t I = 0; I < Tc; ++I) {
+      int64_t f = I;
+      for (int64_t r = I * kTile; r < (I + 1) * kTile; ++r) f = std::min(f, cfrow[r] / kTile);
+      P.coarse_ftile[I] = int32_t(f);
+    }
+    P.coarse_sched = single_matrix_schedule(int(ncp / kTile), int(Tc), true, P.coarse_ftile.data());
   }
 
   // ---- coarse gather positions ----------------------------------------------
@@ -246,6 +276,8 @@ void build_plans(const Topology& T, Plans& P) {
   }
 
   // ---- local factorization schedules (incremental diagonal / left-looking) ------
+  // Only tiles inside the envelope (J >= f_I) are touched; the update of tile
+  // (I, J) runs over K in [max(phase start, f_I, f_J), J).
   for (int variant = 0; variant < 2; ++variant) {
     const bool inc = variant == 0;
     TileSchedule& S = inc ? P.local_sched : P.local_sched_ll;
@@ -261,8 +293,10 @@ void build_plans(const Topology& T, Plans& P) {
       for (int64_t s = 0; s < ns; ++s) {
         const SubLayout& L = T.sub[s];
         if (j >= L.Ti) continue;
-        diag.push_back(TileWork{int32_t(s), j, j, 0, inc ? 0 : j, 0});
-        for (int I = j + 1; I < L.Tc; ++I) below.push_back(TileWork{int32_t(s), I, j, 0, j, fl});
+        const int32_t* ft = &P.ftile[P.ftile_off[s]];
+        diag.push_back(TileWork{int32_t(s), j, j, inc ? 0 : ft[j], inc ? 0 : j, 0});
+        for (int I = j + 1; I < L.Tc; ++I)
+          if (ft[I] <= j) below.push_back(TileWork{int32_t(s), I, j, std::max(ft[I], ft[j]), j, fl});
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 1, below);
@@ -271,8 +305,12 @@ void build_plans(const Topology& T, Plans& P) {
     for (int64_t s = 0; s < ns; ++s) {
       const SubLayout& L = T.sub[s];
       if (L.Ti == 0) continue;
+      const int32_t* ft = &P.ftile[P.ftile_off[s]];
       for (int J = L.Ti; J < L.Tc; ++J)
-        for (int I = inc ? J + 1 : J; I < L.Tc; ++I) schur1.push_back(TileWork{int32_t(s), I, J, 0, L.Ti, 0});
+        for (int I = inc ? J + 1 : J; I < L.Tc; ++I) {
+          const int k0 = std::max(ft[I], ft[J]);
+          if (k0 < L.Ti) schur1.push_back(TileWork{int32_t(s), I, J, k0, L.Ti, 0});
+        }
     }
     add_tiles(S, 0, schur1);
     S.launches.push_back(TileLaunch{-1, 0, 0});  // snapshot S_DD for the preconditioner
@@ -283,8 +321,11 @@ void build_plans(const Topology& T, Plans& P) {
         const SubLayout& L = T.sub[s];
         const int col = L.Ti + j;
         if (col >= L.Tr) continue;
-        diag.push_back(TileWork{int32_t(s), col, col, inc ? 0 : L.Ti, inc ? 0 : col, 0});
-        for (int I = col + 1; I < L.T; ++I) below.push_back(TileWork{int32_t(s), I, col, L.Ti, col, fl});
+        const int32_t* ft = &P.ftile[P.ftile_off[s]];
+        diag.push_back(TileWork{int32_t(s), col, col, inc ? 0 : std::max(L.Ti, ft[col]), inc ? 0 : col, 0});
+        for (int I = col + 1; I < L.T; ++I)
+          if (ft[I] <= col)
+            below.push_back(TileWork{int32_t(s), I, col, std::max(L.Ti, std::max(ft[I], ft[col])), col, fl});
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 1, below);
@@ -293,8 +334,12 @@ void build_plans(const Topology& T, Plans& P) {
     for (int64_t s = 0; s < ns; ++s) {
       const SubLayout& L = T.sub[s];
       if (L.Tr == L.Ti) continue;
+      const int32_t* ft = &P.ftile[P.ftile_off[s]];
       for (int J = L.Tr; J < L.T; ++J)
-        for (int I = inc ? J + 1 : J; I < L.T; ++I) schur2.push_back(TileWork{int32_t(s), I, J, L.Ti, L.Tr, 0});
+        for (int I = inc ? J + 1 : J; I < L.T; ++I) {
+          const int k0 = std::max(L.Ti, std::max(ft[I], ft[J]));
+          if (k0 < L.Tr) schur2.push_back(TileWork{int32_t(s), I, J, k0, L.Tr, 0});
+        }
     }
     add_tiles(S, 0, schur2);
   }
@@ -346,6 +391,7 @@ void build_plans(const Topology& T, Plans& P) {
     D.T = L.T; D.Ti = L.Ti; D.Tr = L.Tr; D.Tc = L.Tc;
     D.pool_off = L.pool_off; D.vec_off = L.vec_off; D.dvec_off = L.dvec_off;
     D.w_off = L.w_off; D.p_off = L.p_off; D.y_off = L.y_off; D.cvec_off = L.cvec_off;
+    D.ft_off = P.ftile_off[s];
     P.max_ndp = std::max(P.max_ndp, L.ndp);
     P.max_ncp = std::max(P.max_ncp, L.ncp);
     P.max_m = std::max(P.max_m, L.m);

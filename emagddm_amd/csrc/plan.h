@@ -59,6 +59,9 @@ struct Plans {
   int max_ndp = 0, max_ncp = 0, max_m = 0, max_wrows = 0;
   int max_T = 0, max_Tr = 0, max_Ti = 0;
   std::vector<int64_t> inv_off;  // per subdomain: offset of its inverse diagonal tiles
+  std::vector<int32_t> ftile;    // per subdomain and tile row: first nonzero tile column
+  std::vector<int64_t> ftile_off;
+  std::vector<int32_t> coarse_ftile;
   int64_t inv_elems = 0, coarse_inv_elems = 0;
 };
 
@@ -69,7 +72,9 @@ void build_plans(const Topology& T, Plans& P);
 
 // Left-looking schedule of one matrix of Ta tile rows: eliminate the first Tp
 // tile columns, then update the trailing (Ta-Tp)^2 block with the Schur term.
-TileSchedule single_matrix_schedule(int Tp, int Ta, bool incremental_diagonal);
+// ft (optional): first nonzero tile column of every tile row (envelope).
+TileSchedule single_matrix_schedule(int Tp, int Ta, bool incremental_diagonal,
+                                    const int32_t* ft = nullptr);
 
 inline int64_t tile_offset(int64_t I, int64_t J) { return (I * (I + 1) / 2 + J) * kTileElems; }
 inline int64_t pool_pos(int64_t base, int64_t r, int64_t c) {  // r >= c

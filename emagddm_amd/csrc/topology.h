@@ -27,6 +27,7 @@ struct TileWork {
 struct DevSub {
   int32_t nd, nc, ndp, ncp, o_d, o_c, o_a, m, T, Ti, Tr, Tc;
   int64_t pool_off, vec_off, dvec_off, w_off, p_off, y_off, cvec_off;
+  int64_t ft_off;  // offset of this subdomain's envelope (first nonzero tile per tile row)
 };
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }

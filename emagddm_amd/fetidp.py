@@ -60,7 +60,7 @@ def _i32(a):
 class FETIDPSolver:
     """MI355X FETI-DP solver for curl(nu curl A) + kappa A = J on Nedelec tets."""
 
-    def __init__(self, precision="fp64", tolerance=1e-8, dual_tolerance=1e-10,
+    def __init__(self, precision="fp64", tolerance=1e-8, dual_tolerance=1e-11,
                  max_iterations=1000, max_refinements=12, profile=False,
                  inner_tolerance=1e-2):
         self._lib = _lib.load()

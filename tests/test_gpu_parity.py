@@ -67,7 +67,9 @@ def test_dual_operators_match_oracle(gpu_available, kind, precision):
                 d=np.linalg.norm(d - o.d) / np.linalg.norm(o.d),
                 S=np.linalg.norm(Sc - o.Sc) / np.linalg.norm(o.Sc))
     print("  rel errors", errs, flush=True)
-    assert all(e <= tol for e in errs.values()), errs
+    # the Dirichlet preconditioner blocks are stored in FP32 in both precisions
+    assert errs["M"] <= max(tol, 1e-6), errs
+    assert all(errs[k] <= tol for k in ("F", "d", "S")), errs
 
 
 @pytest.mark.parametrize("kind,precision", [("micro", "fp64"), ("cube16", "fp64"),
