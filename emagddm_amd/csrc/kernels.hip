@@ -284,6 +284,62 @@ __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[2][2], c
   }
 }
 
+__device__ __forceinline__ double readlane(double v, int lane) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float readlane(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Cholesky of the 64x64 tile held in LDS (column-major, ld kLd) by one wave and
+// the inverse of the factor: lane i keeps row i in registers, L[k][j] reaches
+// every lane through s_readlane (fully unrolled, so k and j are constants);
+// then lane c builds column c of L^-1 from broadcast LDS reads of L.
+// On return s holds L (strict upper part zeroed), si holds L^-1, rdg 1/diag.
+template <typename T>
+__device__ __forceinline__ int wave_potrf_inv(T* s, T* si, T* rdg, int lane) {
+  T r[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) r[j] = s[j * kLd + lane];
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < 64; ++k) {
+    T v = T(0);
+#pragma unroll
+    for (int j = 0; j < k; ++j) v += r[j] * readlane(r[j], k);
+    const T piv = readlane(r[k] - v, k);
+    T d;
+    if (!(piv > T(0)) || !(piv < T(3.0e38))) {
+      bad = 1;
+      d = T(1);
+    } else {
+      d = sqrt(piv);
+    }
+    const T rd = T(1) / d;
+    r[k] = lane == k ? d : (lane > k ? (r[k] - v) * rd : T(0));
+    if (lane == k) rdg[k] = rd;
+  }
+#pragma unroll
+  for (int j = 0; j < 64; ++j) s[j * kLd + lane] = r[j];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  T y[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) {
+    T acc = T(0);
+#pragma unroll
+    for (int k = 0; k < i; ++k) acc += s[k * kLd + i] * y[k];
+    y[i] = i < lane ? T(0) : (i == lane ? rdg[i] : -acc * rdg[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 64; ++i) si[lane * kLd + i] = y[i];
+  return bad;
+}
+
 // acc += A_panel * B_panel^T over ktot (multiple of 16) columns.  A row panel of
 // tiles (J, k0 .. k1) is contiguous: element (p, k) at k*64 + p, so the K loop is
 // one panel product.  64x16 chunks of both panels are staged through LDS (two
@@ -430,45 +486,12 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
             Cd[j * 64 + i] -= a3[x][y][r];
           }
     } else {
-      // Cholesky of the 64x64 tile and its inverse by wave 0 alone (lane = row for
-      // the factor, lane = column for the inverse): no workgroup barriers, LDS
-      // reads are either broadcasts or lane-contiguous.
+      // Cholesky of the 64x64 tile and its inverse by wave 0 (registers + readlane)
       __shared__ T si[64 * kLd];
       __shared__ T rdg[64];
       if (wave == 0) {
-        const int i = lane;
-        int bad = 0;
-        for (int k = 0; k < 64; ++k) {
-          const T v = dot_strided(s + i, kLd, s + k, kLd, k);
-          const T piv = s[k * kLd + k] - __shfl(v, k);
-          T d;
-          if (!(piv > T(0)) || !(piv < T(3.0e38))) {
-            bad = 1;
-            d = T(1);
-          } else {
-            d = sqrt(piv);
-          }
-          const T rd = T(1) / d;
-          if (i > k) s[k * kLd + i] = (s[k * kLd + i] - v) * rd;
-          if (i == k) {
-            s[k * kLd + k] = d;
-            rdg[k] = rd;
-          }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
+        const int bad = wave_potrf_inv<T>(s, si, rdg, lane);
         if (bad && lane == 0) atomicOr(fail + w.mat, 1);
-        // inverse: column c = lane, y_i = -(sum_{k<i} L_ik y_k) / L_ii, y_c = 1 / L_cc
-        const int c = lane;
-        for (int ii = 0; ii < 64; ++ii) {
-          const T acc1 = dot_strided(s + ii, kLd, si + c * kLd, 1, ii);
-          const T y = ii < c ? T(0) : (ii == c ? rdg[ii] : -acc1 * rdg[ii]);
-          si[c * kLd + ii] = y;
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
       }
       __syncthreads();
       T* Li = inv + inv_base[w.mat] + (int64_t)w.J * kTileElems;
