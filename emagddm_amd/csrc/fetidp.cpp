@@ -103,8 +103,9 @@ struct fetidp_solver {
   DBuf<double> csr_val;
   DBuf<int64_t> coarse_target, coarse_ptr, coarse_srcpos;
   DBuf<TileWork> local_items, local_items_ll, coarse_items;
-  DBuf<int64_t> local_base, coarse_base, local_inv_base, coarse_inv_base;
-  DBuf<int32_t> ftile;
+  DBuf<int64_t> local_rowbase, coarse_rowbase, local_inv_base, coarse_inv_base;
+  DBuf<int64_t> rowabs, coarse_rowabs;
+  DBuf<int32_t> ftile, coarse_ftile;
   DBuf<DevSub> devsub;
   DBuf<int32_t> vec_dof;
   DBuf<double> vec_w;
@@ -246,12 +247,14 @@ struct fetidp_solver {
     local_items_ll.upload(plans.local_sched_ll.items);
     coarse_items.upload(plans.coarse_sched.items);
     {
-      std::vector<int64_t> base(topo.n_sub);
-      for (int s = 0; s < topo.n_sub; ++s) base[s] = topo.sub[s].pool_off;
-      local_base.upload(base);
-      coarse_base.upload(std::vector<int64_t>{0});
+      std::vector<int64_t> rb(plans.ftile_off.begin(), plans.ftile_off.end() - 1);
+      local_rowbase.upload(rb);
+      coarse_rowbase.upload(std::vector<int64_t>{0});
+      rowabs.upload(plans.rowabs);
+      coarse_rowabs.upload(plans.coarse_rowabs);
       local_inv_base.upload(plans.inv_off);
       ftile.upload(plans.ftile);
+      coarse_ftile.upload(plans.coarse_ftile);
       coarse_inv_base.upload(std::vector<int64_t>{0});
     }
     devsub.upload(plans.devsub);
@@ -334,15 +337,18 @@ struct fetidp_solver {
 
   // ---------------------------------------------------------------- factor
   template <typename T>
-  void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& base,
-                    const DBuf<int64_t>& inv_base, T* pool, T* inv, int* failp, float* P) {
+  void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& rowbase,
+                    const DBuf<int64_t>& rabs, const DBuf<int64_t>& inv_base, T* pool, T* inv,
+                    int* failp, float* P) {
     const int slot0 = P ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     for (const TileLaunch& L : S.launches) {
       if (L.mode < 0) {
-        timed(5, [&] { launch_extract_P<T>(topo.n_sub, plans.max_ndp, devsub.p, pool, P, st); });
+        timed(5, [&] { launch_extract_P<T>(topo.n_sub, plans.max_ndp, devsub.p, pool, rowabs.p, ftile.p, P, st); });
         continue;
       }
-      timed(slot0 + L.mode, [&] { launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, inv_base.p, pool, inv, failp, st); });
+      timed(slot0 + L.mode, [&] {
+        launch_tile_mode<T>(L.mode, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st);
+      });
     }
   }
 
@@ -359,7 +365,7 @@ struct fetidp_solver {
     if (rhs_from_J)
       launch_assemble_f64(plans.rhs_global.size(), rhs_target.p, rhs_ptr.p, rhs_src.p, Fe.p, f_glob.p, st);
     // subdomain pools
-    pool.alloc(size_t(topo.pool_elems));
+    pool.alloc(size_t(plans.pool_elems));
     inv.alloc(size_t(plans.inv_elems));
     W.alloc(size_t(topo.w_elems));
     P.alloc(size_t(topo.p_elems));
@@ -374,17 +380,17 @@ struct fetidp_solver {
                    d_sgn.p, d_sgnD.p, d_half.p, st);
     fail.zero(st);
     if constexpr (sizeof(T) == 8)
-      run_schedule<T>(plans.local_sched, local_items, local_base, local_inv_base, pool.p, inv.p, fail.p, P.p);
+      run_schedule<T>(plans.local_sched, local_items, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, P.p);
     else
-      run_schedule<T>(plans.local_sched_ll, local_items_ll, local_base, local_inv_base, pool.p, inv.p, fail.p, P.p);
-    timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, W.p, st); });
+      run_schedule<T>(plans.local_sched_ll, local_items_ll, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, P.p);
+    timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st); });
     // coarse problem in FP64
     cpool.zero(st);
     launch_gather_pool<T>(plans.coarse.size(), coarse_target.p, coarse_ptr.p, coarse_srcpos.p,
                           pool.p, cpool.p, st);
     cfail.zero(st);
-    run_schedule<double>(plans.coarse_sched, coarse_items, coarse_base, coarse_inv_base, cpool.p, cinv.p, cfail.p, nullptr);
-    launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, sinv.p, st);
+    run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, nullptr);
+    launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, coarse_rowabs.p, coarse_ftile.p, sinv.p, st);
     HIPCHECK(hipStreamSynchronize(st));
     prof_collect();
     std::vector<int> hf(topo.n_sub);
@@ -458,7 +464,7 @@ struct fetidp_solver {
   void condense(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, const double* fsrc) {
     launch_split_rhs(topo.vec_elems, vec_dof.p, vec_w.p, fsrc, vec.p, st);
     timed(12, [&] {
-      launch_forward<T>(topo.n_sub, plans.max_T, plans.max_Tr, devsub.p, pool.p, inv.p,
+      launch_forward<T>(topo.n_sub, plans.max_T, plans.max_Tr, devsub.p, pool.p, rowabs.p, inv.p,
                         local_inv_base.p, ftile.p, vec.p, yvec.p, st);
     });
     launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
@@ -478,8 +484,8 @@ struct fetidp_solver {
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
     timed(13, [&] {
-      launch_interior<T>(topo.n_sub, plans.max_Ti, devsub.p, pool.p, inv.p, local_inv_base.p, ftile.p,
-                         yvec.p, U.p, tbuf.p, vout.p, st);
+      launch_interior<T>(topo.n_sub, plans.max_Ti, devsub.p, pool.p, rowabs.p, inv.p, local_inv_base.p,
+                         ftile.p, yvec.p, U.p, tbuf.p, vout.p, st);
     });
     launch_gather_global(topo.n_free, free_kind.p, free_a.p, vout.p, ubar.p, uc.p, xout, st);
   }
@@ -681,16 +687,19 @@ static int32_t tile_cholesky_test(int64_t m, int64_t pc, const double* A, double
   TileSchedule S = single_matrix_schedule(Tp, Ta, sizeof(T) == 8);
   DBuf<T> dpool, dinv;
   DBuf<TileWork> items;
-  DBuf<int64_t> base;
+  DBuf<int64_t> base, rabs;
   DBuf<int> fail;
   dpool.upload(pool);
   dinv.alloc(size_t(std::max(Tp, 1)) * kTileElems);
   items.upload(S.items);
   base.upload(std::vector<int64_t>{0});
+  std::vector<int64_t> ra(Ta);
+  for (int I = 0; I < Ta; ++I) ra[I] = tile_offset(I, 0);  // dense layout: no envelope
+  rabs.upload(ra);
   fail.alloc(1);
   HIPCHECK(hipMemset(fail.p, 0, sizeof(int)));
   for (const TileLaunch& L : S.launches)
-    launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, base.p, dpool.p, dinv.p, fail.p, nullptr);
+    launch_tile_mode<T>(L.mode, items.p + L.off, L.n, base.p, rabs.p, base.p, dpool.p, dinv.p, fail.p, nullptr);
   HIPCHECK(hipDeviceSynchronize());
   HIPCHECK(hipMemcpy(pool.data(), dpool.p, sizeof(T) * elems, hipMemcpyDeviceToHost));
   int hf = 0;
@@ -797,7 +806,7 @@ int32_t fetidp_get_sizes(fetidp_handle h, int64_t* out) {
     out[3] = T.n_c;
     out[4] = T.n_sub;
     out[5] = T.n_tets;
-    out[6] = T.pool_elems;
+    out[6] = h->plans.pool_elems;
     out[7] = T.w_elems + T.p_elems;
     out[8] = h->plans.coarse_ncp;
     out[9] = h->plans.max_m;
@@ -910,10 +919,12 @@ int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S) {
     HIPCHECK(hipStreamSynchronize(h->st));
     std::vector<double> pool(h->plans.coarse_pool_elems);
     HIPCHECK(hipMemcpy(pool.data(), tmp.p, sizeof(double) * pool.size(), hipMemcpyDeviceToHost));
+    const std::vector<int64_t>& cra = h->plans.coarse_rowabs;
+    const std::vector<int32_t>& cft = h->plans.coarse_ftile;
     for (int64_t a = 0; a < nc; ++a)
       for (int64_t b = 0; b < nc; ++b) {
         const int64_t hi = std::max(a, b), lo = std::min(a, b);
-        S[a * nc + b] = pool[pool_pos(0, hi, lo)];
+        S[a * nc + b] = lo / kTile < cft[hi / kTile] ? 0.0 : pool[env_pos(cra.data(), hi, lo)];
       }
     (void)ncp;
     return int32_t(FETIDP_SUCCESS);

@@ -29,31 +29,34 @@ void launch_gather_pool(int64_t n, const int64_t* target, const int64_t* ptr, co
 // mode 0: Schur update list; 1: column step below the diagonal (update + trsm with
 // the stored inverse diagonal tile + diagonal-tile share); 2: diagonal Cholesky + inverse.
 // Inverse diagonal tile of column J of matrix m: inv + inv_base[m] + J * 4096.
+// Matrix m's tile (I, J) lives at pool + rowabs[rowbase[m] + I] + J * 4096.
 template <typename T>
-void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base,
-                      const int64_t* inv_base, T* pool, T* inv, int* fail, hipStream_t st);
+void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
+                      const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
+                      hipStream_t st);
 
 // MFMA lane-layout probe (one wave, 512 doubles of output).
 void launch_mfma_probe(int is_f32, double* out);
 
 template <typename T>
-void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, float* P,
-                      hipStream_t st);
+void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, const int64_t* rowabs,
+                      const int32_t* ftile, float* P, hipStream_t st);
 template <typename T>
-void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool, T* W,
-                      hipStream_t st);
-void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, double* sinv, hipStream_t st);
+void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool,
+                      const int64_t* rowabs, const int32_t* ftile, T* W, hipStream_t st);
+void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, const int64_t* rowabs,
+                         const int32_t* ftile, double* sinv, hipStream_t st);
 
 // Forward solve, one launch per block column: yv_r = L_rr^-1 vec_r; afterwards
 // vec_c = f_c - L_cr y and vec_D' = -L_D'r y (= -E K_rr^-1 f_r).
 template <typename T>
-void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool, const T* inv,
-                    const int64_t* inv_base, const int32_t* ftile, double* vec, double* yv,
-                    hipStream_t st);
+void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool,
+                    const int64_t* rowabs, const T* inv, const int64_t* inv_base,
+                    const int32_t* ftile, double* vec, double* yv, hipStream_t st);
 // Interior solve: xout_i = L_ii^-T (yv_i - L_{D,c; i}^T U); tbuf is scratch (vec space).
 template <typename T>
-void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const T* inv,
-                     const int64_t* inv_base, const int32_t* ftile, const double* yv,
+void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const int64_t* rowabs,
+                     const T* inv, const int64_t* inv_base, const int32_t* ftile, const double* yv,
                      const double* U, double* tbuf, double* xout, hipStream_t st);
 // rho[slot] = pool diagonal at each D slot; stiffness scaling weights and 0.5 means.
 void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void* pool, int is_f32,

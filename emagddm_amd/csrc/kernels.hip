@@ -3,8 +3,9 @@
 // Data layout in HBM
 //  * element data: structure-of-arrays, entry k of tet t at [k * nt + t]
 //    (coalesced per-thread-per-tet writes); one trailing 1.0 constant.
-//  * factor pool: per matrix, the lower triangle of 64x64 tiles, tile (I,J)
-//    (I >= J) at base + (I(I+1)/2 + J) * 4096, each tile column-major.
+//  * factor pool: per matrix, the tiles of the lower triangle that lie inside the
+//    envelope (J >= f_I), packed by tile row: tile (I, J) at rowabs[I] + J * 4096,
+//    each 64x64 tile column-major; a tile row is one contiguous panel.
 //  * iteration operators W_s = [Z; Phi^T] and P_s = S_DD: row-major dense.
 //  * vectors: FP64 always.
 //
@@ -19,9 +20,6 @@
 
 namespace emag {
 
-__device__ __forceinline__ int64_t tile_off(int I, int J) {
-  return ((int64_t)I * (I + 1) / 2 + J) * (int64_t)kTileElems;
-}
 
 // ---------------------------------------------------------------------------
 // Element integration (closed form, FP64)
@@ -401,14 +399,16 @@ __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
 // The inverse diagonal tile of column J of matrix m lives at inv + inv_base[m] + J*4096.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict__ work,
-                                                     const int64_t* __restrict__ base,
+                                                     const int64_t* __restrict__ rowbase,
+                                                     const int64_t* __restrict__ rowabs,
                                                      const int64_t* __restrict__ inv_base,
                                                      T* __restrict__ pool, T* __restrict__ inv,
                                                      int* __restrict__ fail) {
   using acc_t = typename Mfma<T>::acc_t;
   const TileWork w = work[blockIdx.x];
-  const int64_t b = base[w.mat];
-  T* C = pool + b + tile_off(w.I, w.J);
+  // envelope-packed storage: tile (I, J) at ra[I] + J * 4096
+  const int64_t* ra = rowabs + rowbase[w.mat];
+  T* C = pool + ra[w.I] + (int64_t)w.J * kTileElems;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   // D = C^T: rows p of D are columns j of C, columns q of D are rows i of C.
@@ -420,8 +420,9 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
     for (int y = 0; y < 2; ++y) acc[x][y] = acc_t{0, 0, 0, 0};
   {
     __shared__ T sA[2 * 64 * 16], sB[2 * 64 * 16];
-    panel_mma<T>(acc, pool + b + tile_off(w.J, w.k0), pool + b + tile_off(w.I, w.k0),
-                 (w.k1 - w.k0) * 64, sA, sB, tid, pb0, qb0, lr, lk);
+    panel_mma<T>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
+                 pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, sA, sB, tid,
+                 pb0, qb0, lr, lk);
   }
 
   if constexpr (MODE == 0) {
@@ -475,7 +476,7 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
 #pragma unroll
         for (int y = 0; y < 2; ++y) a3[x][y] = acc_t{0, 0, 0, 0};
       tile_mma<T, kLd, kLd>(a3, s, s, pb0, qb0, lr, lk);
-      T* Cd = pool + b + tile_off(w.I, w.I);
+      T* Cd = pool + ra[w.I] + (int64_t)w.I * kTileElems;
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -505,94 +506,115 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
 }
 
 template <typename T>
-void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* mat_base,
-                      const int64_t* inv_base, T* pool, T* inv, int* fail, hipStream_t st) {
+void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
+                      const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
+                      hipStream_t st) {
   if (n <= 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, mat_base, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
   else if (mode == 1)
-    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, mat_base, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
   else
-    hipLaunchKernelGGL((k_tile_update<T, 2>), dim3(n), dim3(256), 0, st, w, mat_base, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update<T, 2>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
 }
 template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                       double*, double*, int*, hipStream_t);
+                                       const int64_t*, double*, double*, int*, hipStream_t);
 template void launch_tile_mode<float>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                      float*, float*, int*, hipStream_t);
+                                      const int64_t*, float*, float*, int*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Operator extraction after the partial factorization
 // ---------------------------------------------------------------------------
+// Symmetric read of element (a, b) from an envelope-packed matrix (ra: tile-row
+// offsets, ft: first stored tile per row); entries outside the envelope are 0.
 template <typename T>
-__device__ __forceinline__ T pool_sym(const T* P, int a, int b) {
+__device__ __forceinline__ T pool_sym(const T* P, const int64_t* ra, const int32_t* ft, int a,
+                                      int b) {
   const int hi = a > b ? a : b, lo = a > b ? b : a;
-  return P[tile_off(hi >> 6, lo >> 6) + (lo & 63) * 64 + (hi & 63)];
+  if ((lo >> 6) < ft[hi >> 6]) return T(0);
+  return P[ra[hi >> 6] + (int64_t)(lo >> 6) * kTileElems + (lo & 63) * 64 + (hi & 63)];
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_extract_P(const DevSub* __restrict__ subs,
-                                                   const T* __restrict__ pool, float* __restrict__ P) {
+                                                   const T* __restrict__ pool,
+                                                   const int64_t* __restrict__ rowabs,
+                                                   const int32_t* __restrict__ ftile,
+                                                   float* __restrict__ P) {
   const DevSub S = subs[blockIdx.y];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)S.ndp * S.ndp) return;
   const int a = int(e / S.ndp), b = int(e % S.ndp);
-  P[S.p_off + e] = (float)pool_sym(pool + S.pool_off, S.o_d + a, S.o_d + b);
+  P[S.p_off + e] = (float)pool_sym(pool, rowabs + S.ft_off, ftile + S.ft_off, S.o_d + a, S.o_d + b);
 }
 
 // The Dirichlet preconditioner is stored in FP32 for both factor precisions: it
 // only steers PCPG, the dual operator F and the residual checks stay in the
 // factor precision / FP64.
 template <typename T>
-void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, float* P,
-                      hipStream_t st) {
+void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, const int64_t* rowabs,
+                      const int32_t* ftile, float* P, hipStream_t st) {
   const int64_t n = (int64_t)max_ndp * max_ndp;
   if (n <= 0 || ns <= 0) return;
-  hipLaunchKernelGGL(k_extract_P<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, P);
+  hipLaunchKernelGGL(k_extract_P<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, rowabs,
+                     ftile, P);
 }
-template void launch_extract_P<double>(int, int, const DevSub*, const double*, float*, hipStream_t);
-template void launch_extract_P<float>(int, int, const DevSub*, const float*, float*, hipStream_t);
+template void launch_extract_P<double>(int, int, const DevSub*, const double*, const int64_t*,
+                                       const int32_t*, float*, hipStream_t);
+template void launch_extract_P<float>(int, int, const DevSub*, const float*, const int64_t*,
+                                      const int32_t*, float*, hipStream_t);
 
 // W = [Z; Phi^T] from the trailing block: Z = -A(D',D'), Phi = -A(D',c)
 template <typename T>
 __global__ __launch_bounds__(256) void k_extract_W(const DevSub* __restrict__ subs,
-                                                   const T* __restrict__ pool, T* __restrict__ W) {
+                                                   const T* __restrict__ pool,
+                                                   const int64_t* __restrict__ rowabs,
+                                                   const int32_t* __restrict__ ftile,
+                                                   T* __restrict__ W) {
   const DevSub S = subs[blockIdx.y];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int rows = S.ndp + S.ncp;
   if (e >= (int64_t)rows * S.ndp) return;
   const int row = int(e / S.ndp), k = int(e % S.ndp);
-  const T* P = pool + S.pool_off;
+  const int64_t* ra = rowabs + S.ft_off;
+  const int32_t* ft = ftile + S.ft_off;
   T v;
   if (row < S.ndp)
-    v = -pool_sym(P, S.o_a + row, S.o_a + k);
+    v = -pool_sym(pool, ra, ft, S.o_a + row, S.o_a + k);
   else
-    v = -pool_sym(P, S.o_a + k, S.o_c + (row - S.ndp));
+    v = -pool_sym(pool, ra, ft, S.o_a + k, S.o_c + (row - S.ndp));
   W[S.w_off + e] = v;
 }
 
 template <typename T>
-void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool, T* W,
-                      hipStream_t st) {
+void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool,
+                      const int64_t* rowabs, const int32_t* ftile, T* W, hipStream_t st) {
   const int64_t n = (int64_t)max_rows * max_ndp;
   if (n <= 0 || ns <= 0) return;
-  hipLaunchKernelGGL(k_extract_W<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, W);
+  hipLaunchKernelGGL(k_extract_W<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, rowabs,
+                     ftile, W);
 }
-template void launch_extract_W<double>(int, int, int, const DevSub*, const double*, double*,
-                                       hipStream_t);
-template void launch_extract_W<float>(int, int, int, const DevSub*, const float*, float*, hipStream_t);
+template void launch_extract_W<double>(int, int, int, const DevSub*, const double*, const int64_t*,
+                                       const int32_t*, double*, hipStream_t);
+template void launch_extract_W<float>(int, int, int, const DevSub*, const float*, const int64_t*,
+                                      const int32_t*, float*, hipStream_t);
 
 __global__ __launch_bounds__(256) void k_extract_sinv(int64_t nc, int64_t ncp,
                                                       const double* __restrict__ cpool,
+                                                      const int64_t* __restrict__ rowabs,
+                                                      const int32_t* __restrict__ ftile,
                                                       double* __restrict__ sinv) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nc * nc) return;
   const int a = int(e / nc), b = int(e % nc);
-  sinv[e] = -pool_sym(cpool, int(ncp) + a, int(ncp) + b);
+  sinv[e] = -pool_sym(cpool, rowabs, ftile, int(ncp) + a, int(ncp) + b);
 }
 
-void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, double* sinv, hipStream_t st) {
+void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, const int64_t* rowabs,
+                         const int32_t* ftile, double* sinv, hipStream_t st) {
   if (nc <= 0) return;
-  hipLaunchKernelGGL(k_extract_sinv, dim3((nc * nc + 255) / 256), dim3(256), 0, st, nc, ncp, cpool, sinv);
+  hipLaunchKernelGGL(k_extract_sinv, dim3((nc * nc + 255) / 256), dim3(256), 0, st, nc, ncp, cpool,
+                     rowabs, ftile, sinv);
 }
 
 // ---------------------------------------------------------------------------
@@ -631,6 +653,7 @@ __device__ __forceinline__ double colsum_part(const T* L, const double* u, int t
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
+                                                  const int64_t* __restrict__ rowabs,
                                                   const T* __restrict__ inv,
                                                   const int64_t* __restrict__ inv_base,
                                                   const int32_t* __restrict__ ftile, int J,
@@ -648,7 +671,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ sub
     if (tid < 64) yv[S.vec_off + J * 64 + tid] = ys[tid];
     return;
   }
-  const T* L = pool + S.pool_off + tile_off(I, J);
+  const T* L = pool + rowabs[S.ft_off + I] + (int64_t)J * kTileElems;
   const int i = tid & 63, q = tid >> 6;
   double acc = 0.0;
 #pragma unroll 4
@@ -659,23 +682,26 @@ __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ sub
 }
 
 template <typename T>
-void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool, const T* inv,
-                    const int64_t* inv_base, const int32_t* ftile, double* vec, double* yv,
-                    hipStream_t st) {
+void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool,
+                    const int64_t* rowabs, const T* inv, const int64_t* inv_base,
+                    const int32_t* ftile, double* vec, double* yv, hipStream_t st) {
   if (ns <= 0) return;
   for (int J = 0; J < max_Tr; ++J)
-    hipLaunchKernelGGL(k_fwd_step<T>, dim3(max_T - J, ns), dim3(256), 0, st, subs, pool, inv,
+    hipLaunchKernelGGL(k_fwd_step<T>, dim3(max_T - J, ns), dim3(256), 0, st, subs, pool, rowabs, inv,
                        inv_base, ftile, J, vec, yv);
 }
-template void launch_forward<double>(int, int, int, const DevSub*, const double*, const double*,
-                                     const int64_t*, const int32_t*, double*, double*, hipStream_t);
-template void launch_forward<float>(int, int, int, const DevSub*, const float*, const float*,
-                                    const int64_t*, const int32_t*, double*, double*, hipStream_t);
+template void launch_forward<double>(int, int, int, const DevSub*, const double*, const int64_t*,
+                                     const double*, const int64_t*, const int32_t*, double*,
+                                     double*, hipStream_t);
+template void launch_forward<float>(int, int, int, const DevSub*, const float*, const int64_t*,
+                                    const float*, const int64_t*, const int32_t*, double*, double*,
+                                    hipStream_t);
 
 // t_K = y_K - sum_{I in [Ti, Tc)} L_IK^T U_I for every interior tile column K
 template <typename T>
 __global__ __launch_bounds__(256) void k_int_prep(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
+                                                  const int64_t* __restrict__ rowabs,
                                                   const int32_t* __restrict__ ftile,
                                                   const double* __restrict__ yv,
                                                   const double* __restrict__ U,
@@ -687,7 +713,8 @@ __global__ __launch_bounds__(256) void k_int_prep(const DevSub* __restrict__ sub
   const double* u = U + S.vec_off;
   double s = 0.0;
   for (int I = S.Ti; I < S.Tc; ++I)
-    if (ftile[S.ft_off + I] <= K) s += colsum_part(pool + S.pool_off + tile_off(I, K), u + I * 64, tid);
+    if (ftile[S.ft_off + I] <= K)
+      s += colsum_part(pool + rowabs[S.ft_off + I] + (int64_t)K * kTileElems, u + I * 64, tid);
   if ((tid & 3) == 0) tbuf[S.vec_off + K * 64 + (tid >> 2)] = yv[S.vec_off + K * 64 + (tid >> 2)] - s;
 }
 
@@ -696,6 +723,7 @@ __global__ __launch_bounds__(256) void k_int_prep(const DevSub* __restrict__ sub
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
+                                                  const int64_t* __restrict__ rowabs,
                                                   const T* __restrict__ inv,
                                                   const int64_t* __restrict__ inv_base,
                                                   const int32_t* __restrict__ ftile, int J,
@@ -715,26 +743,27 @@ __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ sub
     if (tid < 64) xout[S.vec_off + J * 64 + tid] = xs[tid];
     return;
   }
-  const double s = colsum_part(pool + S.pool_off + tile_off(J, I), xs, tid);
+  const double s = colsum_part(pool + rowabs[S.ft_off + J] + (int64_t)I * kTileElems, xs, tid);
   if ((tid & 3) == 0) t[I * 64 + (tid >> 2)] -= s;
 }
 
 template <typename T>
-void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const T* inv,
-                     const int64_t* inv_base, const int32_t* ftile, const double* yv,
+void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const int64_t* rowabs,
+                     const T* inv, const int64_t* inv_base, const int32_t* ftile, const double* yv,
                      const double* U, double* tbuf, double* xout, hipStream_t st) {
   if (ns <= 0 || max_Ti <= 0) return;
-  hipLaunchKernelGGL(k_int_prep<T>, dim3(max_Ti, ns), dim3(256), 0, st, subs, pool, ftile, yv, U, tbuf);
+  hipLaunchKernelGGL(k_int_prep<T>, dim3(max_Ti, ns), dim3(256), 0, st, subs, pool, rowabs, ftile, yv,
+                     U, tbuf);
   for (int J = max_Ti - 1; J >= 0; --J)
-    hipLaunchKernelGGL(k_bwd_step<T>, dim3(J + 1, ns), dim3(256), 0, st, subs, pool, inv, inv_base,
-                       ftile, J, tbuf, xout);
+    hipLaunchKernelGGL(k_bwd_step<T>, dim3(J + 1, ns), dim3(256), 0, st, subs, pool, rowabs, inv,
+                       inv_base, ftile, J, tbuf, xout);
 }
-template void launch_interior<double>(int, int, const DevSub*, const double*, const double*,
-                                      const int64_t*, const int32_t*, const double*, const double*,
-                                      double*, double*, hipStream_t);
-template void launch_interior<float>(int, int, const DevSub*, const float*, const float*,
-                                     const int64_t*, const int32_t*, const double*, const double*,
-                                     double*, double*, hipStream_t);
+template void launch_interior<double>(int, int, const DevSub*, const double*, const int64_t*,
+                                      const double*, const int64_t*, const int32_t*, const double*,
+                                      const double*, double*, double*, hipStream_t);
+template void launch_interior<float>(int, int, const DevSub*, const float*, const int64_t*,
+                                     const float*, const int64_t*, const int32_t*, const double*,
+                                     const double*, double*, double*, hipStream_t);
 
 __global__ __launch_bounds__(256) void k_rho(int64_t n, const int64_t* __restrict__ pos,
                                              const void* __restrict__ pool, int is_f32,

@@ -88,32 +88,73 @@ void build_plans(const Topology& T, Plans& P) {
   std::vector<int32_t> loc(nf, -1);
 
   // ---- subdomain pools ------------------------------------------------------
+  // Pass 1: envelope.  The first nonzero column of every local row (lower
+  // triangle) bounds the Cholesky fill, so only tiles (I, J >= f_I) are stored:
+  // tile row I is one contiguous run of (I - f_I + 1) tiles.
+  P.ftile_off.resize(ns + 1, 0);
+  for (int64_t s = 0; s < ns; ++s) P.ftile_off[s + 1] = P.ftile_off[s] + T.sub[s].T;
+  P.ftile.assign(P.ftile_off[ns], 0);
+  P.rowabs.assign(P.ftile_off[ns], 0);
+  P.vec_dof.assign(T.vec_elems, -1);
+  P.vec_w.assign(T.vec_elems, 0.0);
+  P.rho_pos.assign(T.dvec_elems, -1);
+  auto padded_of = [](const SubLayout& L, int k) {
+    return k < L.ni ? k : (k < L.ni + L.nd ? L.o_d + (k - L.ni) : L.o_c + (k - L.ni - L.nd));
+  };
+  {
+    int64_t off = 0;
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& L = T.sub[s];
+      const int32_t* dofs = &T.sub_dof[T.sub_dof_ptr[s]];
+      const int n = L.ni + L.nd + L.nc;
+      std::vector<int32_t> frow(L.m);
+      for (int r = 0; r < L.m; ++r) frow[r] = r;
+      for (int k = 0; k < n; ++k) loc[dofs[k]] = padded_of(L, k);
+      for (int64_t j = T.sub_tet_ptr[s]; j < T.sub_tet_ptr[s + 1]; ++j) {
+        const int64_t t = T.sub_tet[j];
+        for (int a = 0; a < 6; ++a) {
+          const int32_t da = T.tet_free[6 * t + a];
+          if (da < 0) continue;
+          for (int b = 0; b < 6; ++b) {
+            const int32_t db = T.tet_free[6 * t + b];
+            if (db < 0) continue;
+            const int la = loc[da], lb = loc[db];
+            if (la >= lb) frow[la] = std::min(frow[la], lb);
+          }
+        }
+      }
+      for (int k = 0; k < L.nd; ++k) frow[L.o_a + k] = std::min(frow[L.o_a + k], L.o_d + k);
+      for (int k = 0; k < n; ++k) loc[dofs[k]] = -1;
+      int32_t* ft = &P.ftile[P.ftile_off[s]];
+      int64_t* ra = &P.rowabs[P.ftile_off[s]];
+      P.pool_base.push_back(off);
+      for (int I = 0; I < L.T; ++I) {
+        int f = I;
+        for (int r = I * kTile; r < (I + 1) * kTile; ++r) f = std::min(f, frow[r] / kTile);
+        ft[I] = f;
+        ra[I] = off - int64_t(f) * kTileElems;
+        off += int64_t(I - f + 1) * kTileElems;
+      }
+    }
+    P.pool_elems = off;
+  }
+  // Pass 2: contribution lists of every stored entry.
   {
     std::vector<uint64_t> tgt;
     std::vector<int32_t> code;
     tgt.reserve(size_t(nt) * 26);
     code.reserve(size_t(nt) * 26);
-    P.vec_dof.assign(T.vec_elems, -1);
-    P.vec_w.assign(T.vec_elems, 0.0);
-    P.rho_pos.assign(T.dvec_elems, -1);
-    P.ftile_off.resize(ns + 1, 0);
-    for (int64_t s = 0; s < ns; ++s) P.ftile_off[s + 1] = P.ftile_off[s] + T.sub[s].T;
-    P.ftile.assign(P.ftile_off[ns], 0);
     for (int64_t s = 0; s < ns; ++s) {
       const SubLayout& L = T.sub[s];
       const int32_t* dofs = &T.sub_dof[T.sub_dof_ptr[s]];
-      // first nonzero column of every local row (lower triangle); the Cholesky
-      // factor keeps this envelope, so tiles left of it stay zero
-      std::vector<int32_t> frow(L.m);
-      for (int r = 0; r < L.m; ++r) frow[r] = r;
+      const int64_t* ra = &P.rowabs[P.ftile_off[s]];
       const int n = L.ni + L.nd + L.nc;
-      auto padded = [&](int k) { return k < L.ni ? k : (k < L.ni + L.nd ? L.o_d + (k - L.ni) : L.o_c + (k - L.ni - L.nd)); };
       for (int k = 0; k < n; ++k) {
-        loc[dofs[k]] = padded(k);
-        P.vec_dof[L.vec_off + padded(k)] = dofs[k];
-        P.vec_w[L.vec_off + padded(k)] = 1.0 / T.free_mult[dofs[k]];
+        loc[dofs[k]] = padded_of(L, k);
+        P.vec_dof[L.vec_off + padded_of(L, k)] = dofs[k];
+        P.vec_w[L.vec_off + padded_of(L, k)] = 1.0 / T.free_mult[dofs[k]];
       }
-      for (int k = 0; k < L.nd; ++k) P.rho_pos[L.dvec_off + k] = pool_pos(L.pool_off, L.o_d + k, L.o_d + k);
+      for (int k = 0; k < L.nd; ++k) P.rho_pos[L.dvec_off + k] = env_pos(ra, L.o_d + k, L.o_d + k);
       for (int64_t j = T.sub_tet_ptr[s]; j < T.sub_tet_ptr[s + 1]; ++j) {
         const int64_t t = T.sub_tet[j];
         for (int a = 0; a < 6; ++a) {
@@ -126,37 +167,30 @@ void build_plans(const Topology& T, Plans& P) {
             const int lb = loc[db];
             if (la < lb) continue;
             const int32_t c = enc(int64_t(packed_index(a, b)) * nt + t, T.tet_sign[6 * t + a] * T.tet_sign[6 * t + b]);
-            tgt.push_back(uint64_t(pool_pos(L.pool_off, la, lb)));
+            tgt.push_back(uint64_t(env_pos(ra, la, lb)));
             code.push_back(c);
-            frow[la] = std::min(frow[la], lb);
             if (la != lb && la / kTile == lb / kTile) {
-              tgt.push_back(uint64_t(L.pool_off + tile_offset(la / kTile, la / kTile) + (la % kTile) * kTile + lb % kTile));
+              // upper half of a diagonal tile: element (lb, la) of tile (I, I)
+              tgt.push_back(uint64_t(ra[la / kTile] + int64_t(la / kTile) * kTileElems + (la % kTile) * kTile + lb % kTile));
               code.push_back(c);
             }
           }
         }
       }
       auto diag_one = [&](int r) {
-        tgt.push_back(uint64_t(pool_pos(L.pool_off, r, r)));
+        tgt.push_back(uint64_t(env_pos(ra, r, r)));
         code.push_back(int32_t(one_code));
       };
       for (int r = L.ni; r < L.nip; ++r) diag_one(r);
       for (int r = L.o_d + L.nd; r < L.o_c; ++r) diag_one(r);
       for (int r = L.o_c + L.nc; r < L.o_a; ++r) diag_one(r);
       for (int k = 0; k < L.nd; ++k) {
-        tgt.push_back(uint64_t(pool_pos(L.pool_off, L.o_a + k, L.o_d + k)));
+        tgt.push_back(uint64_t(env_pos(ra, L.o_a + k, L.o_d + k)));
         code.push_back(int32_t(one_code));
-        frow[L.o_a + k] = std::min(frow[L.o_a + k], L.o_d + k);
       }
       for (int k = 0; k < n; ++k) loc[dofs[k]] = -1;
-      int32_t* ft = &P.ftile[P.ftile_off[s]];
-      for (int I = 0; I < L.T; ++I) {
-        int f = I;
-        for (int r = I * kTile; r < (I + 1) * kTile; ++r) f = std::min(f, frow[r] / kTile);
-        ft[I] = f;
-      }
     }
-    group_emissions(tgt, code, uint64_t(T.pool_elems), P.pool.target, P.pool.ptr, P.pool.src);
+    group_emissions(tgt, code, uint64_t(P.pool_elems), P.pool.target, P.pool.ptr, P.pool.src);
   }
 
   // ---- global CSR and RHS ---------------------------------------------------
@@ -199,15 +233,12 @@ void build_plans(const Topology& T, Plans& P) {
     group_emissions(tgt, code, uint64_t(nf), P.rhs_global.target, P.rhs_global.ptr, P.rhs_global.src);
   }
 
-  // ---- coarse pool: [[S~, I], [I, 0]], padded ---------------------------------
+  // ---- coarse pool: [[S~, I], [I, 0]], padded, envelope-packed ------------------
   {
     const int64_t nc = T.n_c;
     const int64_t ncp = round_up(std::max<int64_t>(nc, 1), kTile);
     P.coarse_ncp = ncp;
     const int64_t Tc = 2 * ncp / kTile;
-    P.coarse_pool_elems = Tc * (Tc + 1) / 2 * kTileElems;
-    std::vector<uint64_t> tgt;
-    std::vector<int64_t> srcpos;
     std::vector<int64_t> cfrow(2 * ncp);
     for (int64_t r = 0; r < 2 * ncp; ++r) cfrow[r] = r;
     for (int64_t k = 0; k < nc; ++k) cfrow[ncp + k] = k;
@@ -219,22 +250,46 @@ void build_plans(const Topology& T, Plans& P) {
           const int64_t Cb = T.c_loc[L.cvec_off + kb];
           if (Ca < Cb) throw std::runtime_error("primal order not monotone");
           cfrow[Ca] = std::min(cfrow[Ca], Cb);
-          const int64_t sp = pool_pos(L.pool_off, L.o_c + ka, L.o_c + kb);
-          tgt.push_back(uint64_t(pool_pos(0, Ca, Cb)));
+        }
+      }
+    }
+    P.coarse_ftile.assign(Tc, 0);
+    P.coarse_rowabs.assign(Tc, 0);
+    int64_t off = 0;
+    for (int64_t I = 0; I < Tc; ++I) {
+      int64_t f = I;
+      for (int64_t r = I * kTile; r < (I + 1) * kTile; ++r) f = std::min(f, cfrow[r] / kTile);
+      P.coarse_ftile[I] = int32_t(f);
+      P.coarse_rowabs[I] = off - f * kTileElems;
+      off += (I - f + 1) * kTileElems;
+    }
+    P.coarse_pool_elems = off;
+    const int64_t* cra = P.coarse_rowabs.data();
+    std::vector<uint64_t> tgt;
+    std::vector<int64_t> srcpos;
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& L = T.sub[s];
+      const int64_t* ra = &P.rowabs[P.ftile_off[s]];
+      for (int ka = 0; ka < L.nc; ++ka) {
+        const int64_t Ca = T.c_loc[L.cvec_off + ka];
+        for (int kb = 0; kb <= ka; ++kb) {
+          const int64_t Cb = T.c_loc[L.cvec_off + kb];
+          const int64_t sp = env_pos(ra, L.o_c + ka, L.o_c + kb);
+          tgt.push_back(uint64_t(env_pos(cra, Ca, Cb)));
           srcpos.push_back(sp);
           if (Ca != Cb && Ca / kTile == Cb / kTile) {
-            tgt.push_back(uint64_t(tile_offset(Ca / kTile, Ca / kTile) + (Ca % kTile) * kTile + Cb % kTile));
+            tgt.push_back(uint64_t(cra[Ca / kTile] + (Ca / kTile) * kTileElems + (Ca % kTile) * kTile + Cb % kTile));
             srcpos.push_back(sp);
           }
         }
       }
     }
     for (int64_t r = nc; r < ncp; ++r) {
-      tgt.push_back(uint64_t(pool_pos(0, r, r)));
+      tgt.push_back(uint64_t(env_pos(cra, r, r)));
       srcpos.push_back(-1);
     }
     for (int64_t k = 0; k < nc; ++k) {
-      tgt.push_back(uint64_t(pool_pos(0, ncp + k, k)));
+      tgt.push_back(uint64_t(env_pos(cra, ncp + k, k)));
       srcpos.push_back(-1);
     }
     std::vector<uint32_t> idx(tgt.size());
@@ -248,14 +303,7 @@ void build_plans(const Topology& T, Plans& P) {
       P.coarse.srcpos.push_back(srcpos[idx[i]]);
     }
     P.coarse.ptr.push_back(int64_t(tgt.size()));
-
     // coarse schedule (single matrix, eliminate the first ncp/64 tile columns)
-    P.coarse_ftile.assign(Tc, 0);
-    for (int64_t I = 0; I < Tc; ++I) {
-      int64_t f = I;
-      for (int64_t r = I * kTile; r < (I + 1) * kTile; ++r) f = std::min(f, cfrow[r] / kTile);
-      P.coarse_ftile[I] = int32_t(f);
-    }
     P.coarse_sched = single_matrix_schedule(int(ncp / kTile), int(Tc), true, P.coarse_ftile.data());
   }
 
@@ -348,19 +396,26 @@ void build_plans(const Topology& T, Plans& P) {
   auto check = [](bool ok, const char* what) {
     if (!ok) throw std::runtime_error(std::string("plan out of range: ") + what);
   };
-  for (int64_t t : P.pool.target) check(t >= 0 && t < T.pool_elems, "pool target");
+  for (int64_t t : P.pool.target) check(t >= 0 && t < P.pool_elems, "pool target");
   for (int64_t t : P.coarse.target) check(t >= 0 && t < P.coarse_pool_elems, "coarse target");
-  for (int64_t sp : P.coarse.srcpos) check(sp < T.pool_elems, "coarse source");
+  for (int64_t sp : P.coarse.srcpos) check(sp < P.pool_elems, "coarse source");
+  // every tile a work item touches (C, panels from k0, diagonal) lies in the envelope
   for (const TileSchedule* S : {&P.local_sched, &P.local_sched_ll})
     for (const TileWork& w : S->items) {
+      check(w.mat >= 0 && w.mat < ns, "local tile matrix");
       const SubLayout& L = T.sub[w.mat];
-      check(w.mat >= 0 && w.mat < ns && w.J <= w.I && w.I < L.T && w.k0 >= 0 && w.k1 <= w.J,
-            "local tile");
+      const int32_t* ft = &P.ftile[P.ftile_off[w.mat]];
+      check(w.J <= w.I && w.I < L.T && w.k1 <= w.J && w.J >= ft[w.I], "local tile");
+      check(w.k0 >= w.k1 || (w.k0 >= ft[w.I] && w.k0 >= ft[w.J]), "local tile panel");
     }
   {
     const int Ta = int(2 * P.coarse_ncp / kTile);
-    for (const TileWork& w : P.coarse_sched.items)
-      check(w.mat == 0 && w.J <= w.I && w.I < Ta && w.k0 >= 0 && w.k1 <= w.J, "coarse tile");
+    for (const TileWork& w : P.coarse_sched.items) {
+      check(w.mat == 0 && w.J <= w.I && w.I < Ta && w.k1 <= w.J && w.J >= P.coarse_ftile[w.I],
+            "coarse tile");
+      check(w.k0 >= w.k1 || (w.k0 >= P.coarse_ftile[w.I] && w.k0 >= P.coarse_ftile[w.J]),
+            "coarse tile panel");
+    }
   }
   for (int64_t y : P.c_ypos) check(y >= 0 && y < T.y_elems, "coarse y position");
   for (int64_t v : P.c_vpos) check(v >= 0 && v < T.vec_elems, "coarse v position");
@@ -392,6 +447,7 @@ void build_plans(const Topology& T, Plans& P) {
     D.pool_off = L.pool_off; D.vec_off = L.vec_off; D.dvec_off = L.dvec_off;
     D.w_off = L.w_off; D.p_off = L.p_off; D.y_off = L.y_off; D.cvec_off = L.cvec_off;
     D.ft_off = P.ftile_off[s];
+    D.pool_off = P.pool_base[s];
     P.max_ndp = std::max(P.max_ndp, L.ndp);
     P.max_ncp = std::max(P.max_ncp, L.ncp);
     P.max_m = std::max(P.max_m, L.m);

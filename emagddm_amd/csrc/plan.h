@@ -60,8 +60,13 @@ struct Plans {
   int max_T = 0, max_Tr = 0, max_Ti = 0;
   std::vector<int64_t> inv_off;  // per subdomain: offset of its inverse diagonal tiles
   std::vector<int32_t> ftile;    // per subdomain and tile row: first nonzero tile column
-  std::vector<int64_t> ftile_off;
+  std::vector<int64_t> ftile_off;     // per subdomain: offset of its tile rows in ftile/rowabs
+  std::vector<int64_t> rowabs;        // per (subdomain, tile row I): element offset of tile (I, 0)
+                                      // in the packed pool; tile (I, J >= f_I) at rowabs + J*4096
+  std::vector<int64_t> pool_base;     // per subdomain: first element of its packed tiles
+  int64_t pool_elems = 0;
   std::vector<int32_t> coarse_ftile;
+  std::vector<int64_t> coarse_rowabs;
   int64_t inv_elems = 0, coarse_inv_elems = 0;
 };
 
@@ -76,9 +81,15 @@ void build_plans(const Topology& T, Plans& P);
 TileSchedule single_matrix_schedule(int Tp, int Ta, bool incremental_diagonal,
                                     const int32_t* ft = nullptr);
 
+// Dense lower-triangular tile layout (used for matrices without an envelope).
 inline int64_t tile_offset(int64_t I, int64_t J) { return (I * (I + 1) / 2 + J) * kTileElems; }
 inline int64_t pool_pos(int64_t base, int64_t r, int64_t c) {  // r >= c
   return base + tile_offset(r / kTile, c / kTile) + (c % kTile) * kTile + (r % kTile);
+}
+// Envelope-packed layout: element (r, c), r >= c inside the envelope, of a
+// matrix whose tile-row offsets are rowabs.
+inline int64_t env_pos(const int64_t* rowabs, int64_t r, int64_t c) {
+  return rowabs[r / kTile] + (c / kTile) * kTileElems + (c % kTile) * kTile + (r % kTile);
 }
 
 }  // namespace emag
