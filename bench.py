@@ -21,6 +21,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes over the same workload, each scaled by what the counter reads for a 512 MiB
+# device-to-device copy (FETCH x2.000, WRITE x1.000); the counters cannot be read live.
+MEASURED_TRAFFIC = {("cube48j", "k_tile_update<double,1>"): 1.057e9}
 WORKLOAD_TEXT = {
     "cube48j": "48^3-hex Kuhn cube (663,552 tets, 753,552 free edges), 6x6x6 jittered "
                "subdomains 6-10 hexes wide, nu=1, kappa=1e-2, 2 Gaussian current loops, PEC",
@@ -39,6 +43,9 @@ def parse():
     ap.add_argument("--variants", default="fp64,fp32")
     ap.add_argument("--cpu-sample-iters", type=int, default=600)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="copy this many MiB device-to-device once before the variants, so a "
+                         "counter profile sees a copy of known size")
     return ap.parse_args()
 
 
@@ -70,7 +77,7 @@ def run_variant(prob, precision, steps, warmup):
                 residual=r.residual_norm, profile=prof, steps=steps)
 
 
-def roofline(v, precision):
+def roofline(v, precision, workload):
     """Dominant kernel: the column-step tile kernel (update + TRSM on MFMA)."""
     prof = v["profile"]
     candidates = ["tile_column", "tile_schur", "tile_diag"]
@@ -83,10 +90,11 @@ def roofline(v, precision):
     avg_ms = ms / launches
     achieved = per_launch_flops / (avg_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[precision]
+    kernel = (f"k_tile_update<{'double' if precision == 'fp64' else 'float'},"
+              f"{ {'tile_schur': 0, 'tile_column': 1, 'tile_diag': 2}[name]}>")
     return dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
-                frac=round(achieved / peak, 4), traffic=None,
-                kernel=f"k_tile_update<{'double' if precision == 'fp64' else 'float'},"
-                       f"{ {'tile_schur': 0, 'tile_column': 1, 'tile_diag': 2}[name]}>",
+                frac=round(achieved / peak, 4), traffic=MEASURED_TRAFFIC.get((workload, kernel)),
+                kernel=kernel,
                 launches_per_step=launches // v["steps"],
                 flops_per_launch=per_launch_flops, avg_launch_ms=avg_ms), name
 
@@ -137,6 +145,13 @@ def main():
     from emagddm_amd import workloads
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     prob = workloads.make_workload(a.workload)
+    if a.calibration_copy_mib > 0:
+        n = a.calibration_copy_mib << 17  # float64 elements
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     variants = {}
     for prec in a.variants.split(","):
         try:
@@ -162,7 +177,7 @@ def main():
     agree = None
     if "fp64" in xs and "fp32" in xs:
         agree = float(np.linalg.norm(xs["fp32"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
-    roof, _ = roofline(vb, best)
+    roof, _ = roofline(vb, best, a.workload)
     out = {
         "metric": "time-to-1e-8 resid: learned+FP32-refined vs plain FP64 FETI-DP; PCPG iters",
         "value": round(vb["s_per_solve"], 5),

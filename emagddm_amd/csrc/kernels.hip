@@ -805,15 +805,36 @@ void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void*
 // ---------------------------------------------------------------------------
 // PCPG operators
 // ---------------------------------------------------------------------------
-constexpr int kGemvRows = 64;
+constexpr int kGemvRows = 32;
 
-// which = 0: M = W_s ((ndp+ncp) x ndp), out at y_off; which = 1: M = P_s (ndp x ndp), out at dvec_off
+// 16-byte row segment: 2 doubles or 4 floats per lane and load.
+template <typename T> struct Seg;
+template <> struct Seg<double> {
+  static constexpr int V = 2;
+  __device__ static double dot(const double* a, const double* x) {
+    const double2 u = *reinterpret_cast<const double2*>(a);
+    return u.x * x[0] + u.y * x[1];
+  }
+};
+template <> struct Seg<float> {
+  static constexpr int V = 4;
+  __device__ static double dot(const float* a, const double* x) {
+    const float4 u = *reinterpret_cast<const float4*>(a);
+    return (double)u.x * x[0] + (double)u.y * x[1] + (double)u.z * x[2] + (double)u.w * x[3];
+  }
+};
+
+// which = 0: M = W_s ((ndp+ncp) x ndp), out at y_off; which = 1: M = P_s (ndp x ndp), out at dvec_off.
+// Rows are ndp-strided with ndp a multiple of 64 and row bases 256-byte aligned, so every lane
+// reads whole 16-byte segments; columns in [nd, ndp) meet zeros in the gathered vector.
+// Each wave walks two rows at once so four segment loads are in flight per lane.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemv_dual(const DevSub* __restrict__ subs,
                                                    const T* __restrict__ M, int which,
                                                    const int32_t* __restrict__ d_lam,
                                                    const double* __restrict__ d_scale,
                                                    const double* __restrict__ v, double* __restrict__ y) {
+  constexpr int V = Seg<T>::V;
   extern __shared__ double xs[];
   const DevSub S = subs[blockIdx.y];
   const int rows = which == 0 ? S.ndp + S.ncp : S.ndp;
@@ -827,16 +848,37 @@ __global__ __launch_bounds__(256) void k_gemv_dual(const DevSub* __restrict__ su
   }
   __syncthreads();
   const T* base = M + (which == 0 ? S.w_off : S.p_off);
-  const int ncol = S.nd;
-  for (int rr = wv; rr < kGemvRows; rr += 4) {
-    const int row = r0 + rr;
-    if (row >= rows) break;
-    const T* a = base + (int64_t)row * S.ndp;
-    double s = 0.0;
-    for (int k = lane; k < ncol; k += 64) s += (double)a[k] * xs[k];
+  const int kend = (S.nd + V - 1) / V * V;  // <= ndp
+  double* out = y + (which == 0 ? S.y_off : S.dvec_off);
+  for (int rr = 2 * wv; rr < kGemvRows; rr += 8) {
+    const int row0 = r0 + rr;
+    if (row0 >= rows) break;
+    const bool two = row0 + 1 < rows;
+    const T* a0 = base + (int64_t)row0 * S.ndp;
+    const T* a1 = two ? a0 + S.ndp : a0;
+    double s0 = 0.0, s1 = 0.0, t0 = 0.0, t1 = 0.0;
+    int k = lane * V;
+    for (; k + 64 * V < kend; k += 128 * V) {
+      s0 += Seg<T>::dot(a0 + k, xs + k);
+      s1 += Seg<T>::dot(a1 + k, xs + k);
+      t0 += Seg<T>::dot(a0 + k + 64 * V, xs + k + 64 * V);
+      t1 += Seg<T>::dot(a1 + k + 64 * V, xs + k + 64 * V);
+    }
+    if (k < kend) {
+      s0 += Seg<T>::dot(a0 + k, xs + k);
+      s1 += Seg<T>::dot(a1 + k, xs + k);
+    }
+    s0 += t0;
+    s1 += t1;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) y[(which == 0 ? S.y_off : S.dvec_off) + row] = s;
+    for (int o = 32; o >= 1; o >>= 1) {
+      s0 += __shfl_xor(s0, o);
+      s1 += __shfl_xor(s1, o);
+    }
+    if (lane == 0) {
+      out[row0] = s0;
+      if (two) out[row0 + 1] = s1;
+    }
   }
 }
 
