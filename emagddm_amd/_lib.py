@@ -21,14 +21,15 @@ class Config(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_iterations", C.c_int32),
                 ("tolerance", C.c_double), ("dual_tolerance", C.c_double),
                 ("max_refinements", C.c_int32), ("verbose", C.c_int32),
-                ("inner_tolerance", C.c_double)]
+                ("inner_tolerance", C.c_double), ("graphs", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("refinements", C.c_int32),
                 ("fell_back", C.c_int32), ("residual_norm", C.c_double),
                 ("dual_residual", C.c_double), ("initial_dual_residual", C.c_double),
-                ("factor_ms", C.c_double), ("solve_ms", C.c_double), ("symbolic_ms", C.c_double)]
+                ("factor_ms", C.c_double), ("solve_ms", C.c_double), ("symbolic_ms", C.c_double),
+                ("guess_ms", C.c_double), ("learned_guess", C.c_int32), ("reserved", C.c_int32)]
 
 
 EXPORTS = [
@@ -40,8 +41,13 @@ EXPORTS = [
     "fetidp_get_refinement_history",
     "fetidp_get_coarse_matrix", "fetidp_get_profile", "fetidp_reset_profile",
     "fetidp_last_error", "fetidp_destroy", "emag_nedelec_integrate", "emag_tile_cholesky_test",
-    "emag_mfma_layout_probe",
+    "emag_mfma_layout_probe", "fetidp_set_lambda0_model", "fetidp_get_dual_graph",
+    "fetidp_get_lambda_features",
+    "emag_mpnn_create", "emag_mpnn_destroy", "emag_mpnn_last_error", "emag_mpnn_num_params",
+    "emag_mpnn_get_params", "emag_mpnn_set_params", "emag_mpnn_set_graph", "emag_mpnn_forward",
+    "emag_mpnn_gradient", "emag_mpnn_train_step", "emag_mpnn_mfma_probe",
 ]
+HEADERS = ["emagddm_fetidp.h", "emagddm_mpnn.h"]
 
 _lib = None
 
@@ -93,9 +99,29 @@ def load():
     lib.emag_nedelec_integrate.argtypes = [C.c_int64, P, P, P, P, P, P, P, P, P, P]
     lib.emag_tile_cholesky_test.argtypes = [C.c_int32, C.c_int64, C.c_int64, dp, dp]
     lib.emag_mfma_layout_probe.argtypes = [C.c_int32, dp]
+    fp = C.POINTER(C.c_float)
+    lib.fetidp_set_lambda0_model.argtypes = [P, P]
+    lib.fetidp_get_dual_graph.argtypes = [P, ip, ip, fp]
+    lib.fetidp_get_dual_graph.restype = C.c_int64
+    lib.fetidp_get_lambda_features.argtypes = [P, fp, dp]
+    lib.emag_mpnn_create.argtypes = [C.c_uint64]
+    lib.emag_mpnn_create.restype = P
+    lib.emag_mpnn_destroy.argtypes = [P]
+    lib.emag_mpnn_destroy.restype = None
+    lib.emag_mpnn_last_error.argtypes = [P]
+    lib.emag_mpnn_last_error.restype = C.c_char_p
+    lib.emag_mpnn_num_params.argtypes = []
+    lib.emag_mpnn_num_params.restype = C.c_int64
+    lib.emag_mpnn_get_params.argtypes = [P, fp]
+    lib.emag_mpnn_set_params.argtypes = [P, fp]
+    lib.emag_mpnn_set_graph.argtypes = [P, C.c_int64, ip, ip, fp]
+    lib.emag_mpnn_forward.argtypes = [P, C.c_int64, P, P, P]
+    lib.emag_mpnn_gradient.argtypes = [P, C.c_int64, P, P, P, fp, dp, P]
+    lib.emag_mpnn_train_step.argtypes = [P, C.c_int64, P, P, P, C.c_float, dp, P]
+    lib.emag_mpnn_mfma_probe.argtypes = [fp, fp, fp]
     for name in EXPORTS:
         fn = getattr(lib, name)
-        if fn.restype is C.c_int:  # default
+        if fn.restype is C.c_int:  # default int return -> status
             fn.restype = C.c_int32
     _lib = lib
     return lib

@@ -7,6 +7,7 @@
 // statuses instead of exceptions, factor once / solve many.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -18,7 +19,9 @@
 #include <vector>
 
 #include "emagddm_fetidp.h"
+#include "emagddm_mpnn.h"
 #include "kernels.h"
+#include "mpnn.h"
 #include "plan.h"
 #include "topology.h"
 
@@ -126,6 +129,12 @@ struct fetidp_solver {
   DBuf<double> lam, r, z, p, q, dv, ubar, x, xtot, rglob, partial, pf, qf;
   std::vector<double> hist, ref_hist;
   double last_dual = 0.0, init_dual = 0.0;
+  DBuf<double> dsc;                  // PCPG device scalars (kSc* slots)
+  double* hsc = nullptr;             // pinned, mapped host copy written by the kernels
+  double* hsc_dev = nullptr;
+  hipGraphExec_t pcpg_exec = nullptr;
+  const void* pcpg_key[3] = {nullptr, nullptr, nullptr};
+  int pcpg_key_t = 0;
 
   // ---- optional event profiling (cfg.verbose & 2) ---------------------------
   // slot k accumulates milliseconds of one kernel class; see fetidp_get_profile.
@@ -176,7 +185,117 @@ struct fetidp_solver {
       (void)hipEventDestroy(pe.second.second);
     }
     for (hipEvent_t e : prof_pool) (void)hipEventDestroy(e);
+    mp_graph_free(lam_graph);
+    if (pcpg_exec) (void)hipGraphExecDestroy(pcpg_exec);
+    if (hsc) (void)hipHostFree(hsc);
     if (st) (void)hipStreamDestroy(st);
+  }
+
+  // ---- learned initial multiplier (optional model, not owned) ---------------
+  emag_mpnn* lam_model = nullptr;
+  bool graph_built = false;
+  std::vector<int32_t> g_rp, g_col;
+  std::vector<float> g_coef, g_dir;  // coupling t_i . t_j; unit tangent per multiplier
+  MpGraph lam_graph;
+  DBuf<float> lam_X, lam_y, lam_dir;
+  double last_guess_ms = 0.0;
+  bool last_learned = false;
+
+  // Interface graph: multipliers whose mesh edges share a node, coupled by the
+  // cosine of their tangents (pairs with orthogonal tangents are left out).
+  void build_dual_graph() {
+    if (graph_built) return;
+    const int64_t nl = topo.n_lam;
+    std::vector<int32_t> edge_of_free(size_t(topo.n_free), -1);
+    for (int64_t e = 0; e < topo.n_edges; ++e)
+      if (topo.edge_free[e] >= 0) edge_of_free[topo.edge_free[e]] = int32_t(e);
+    std::vector<int32_t> na(nl), nb(nl);
+    g_dir.assign(size_t(3 * nl), 0.f);
+    for (int64_t l = 0; l < nl; ++l) {
+      const int32_t e = edge_of_free[topo.free_of_lam[l]];
+      na[l] = topo.edge_nodes[2 * e];
+      nb[l] = topo.edge_nodes[2 * e + 1];
+      double t[3], n2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        t[c] = h_xyz_[3 * size_t(nb[l]) + c] - h_xyz_[3 * size_t(na[l]) + c];
+        n2 += t[c] * t[c];
+      }
+      const double inv = n2 > 0 ? 1.0 / std::sqrt(n2) : 0.0;
+      for (int c = 0; c < 3; ++c) g_dir[3 * l + c] = float(t[c] * inv);
+    }
+    // node -> multipliers incidence (CSR)
+    std::vector<int64_t> nptr(size_t(topo.n_nodes) + 1, 0);
+    for (int64_t l = 0; l < nl; ++l) { ++nptr[na[l] + 1]; ++nptr[nb[l] + 1]; }
+    for (int64_t v = 0; v < topo.n_nodes; ++v) nptr[v + 1] += nptr[v];
+    std::vector<int32_t> nlam(static_cast<size_t>(nptr[topo.n_nodes]));
+    std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
+    for (int64_t l = 0; l < nl; ++l) { nlam[fill[na[l]]++] = int32_t(l); nlam[fill[nb[l]]++] = int32_t(l); }
+    g_rp.assign(size_t(nl) + 1, 0);
+    g_col.clear();
+    g_coef.clear();
+    std::vector<int32_t> nbr;
+    for (int64_t l = 0; l < nl; ++l) {
+      nbr.clear();
+      for (int32_t v : {na[l], nb[l]})
+        for (int64_t k = nptr[v]; k < nptr[v + 1]; ++k)
+          if (nlam[k] != l) nbr.push_back(nlam[k]);
+      std::sort(nbr.begin(), nbr.end());
+      nbr.erase(std::unique(nbr.begin(), nbr.end()), nbr.end());
+      for (int32_t j : nbr) {
+        const double c = double(g_dir[3 * l]) * g_dir[3 * j] + double(g_dir[3 * l + 1]) * g_dir[3 * j + 1] +
+                         double(g_dir[3 * l + 2]) * g_dir[3 * j + 2];
+        if (std::fabs(c) < 1e-6) continue;
+        g_col.push_back(j);
+        g_coef.push_back(float(c));
+      }
+      g_rp[l + 1] = int32_t(g_col.size());
+    }
+    graph_built = true;
+  }
+
+  void ensure_lam_device() {
+    build_dual_graph();
+    if (lam_graph.n != topo.n_lam) {
+      mp_graph_upload(lam_graph, topo.n_lam, g_rp.data(), g_col.data(), g_coef.data());
+      lam_dir.upload(g_dir);
+      lam_X.alloc(size_t(topo.n_lam) * kMpF);
+      lam_y.alloc(size_t(topo.n_lam));
+    }
+  }
+
+  // network features from the condensed dual RHS dv; returns the output scale
+  // s_z = rms(M^-1 d) (0 when d vanishes)
+  double lambda_features(double* scales) {
+    ensure_lam_device();
+    const int64_t nl = topo.n_lam;
+    apply_M(dv.p, z.p);
+    const double* a[2] = {dv.p, z.p};
+    double o[2];
+    dots(2, a, a, nl, o);
+    const double sd = std::sqrt(o[0] / double(nl)), sz = std::sqrt(o[1] / double(nl));
+    if (scales) { scales[0] = sd; scales[1] = sz; }
+    if (!(sd > 0.0) || !(sz > 0.0)) {
+      HIPCHECK(hipMemsetAsync(lam_X.p, 0, sizeof(float) * nl * kMpF, st));
+      return 0.0;
+    }
+    launch_lambda_features(nl, dv.p, z.p, 1.0 / sd, 1.0 / sz, rho.p, lam_pos.p, lam_dir.p, lam_X.p, st);
+    return sz;
+  }
+
+  // lam = s_z * network(features)
+  void learned_guess() {
+    HIPCHECK(hipStreamSynchronize(st));  // time the guess alone, not queued condensation
+    const double t0 = now_ms();
+    const double sz = lambda_features(nullptr);
+    if (sz > 0.0) {
+      mpnn_forward(lam_model, lam_graph, 1, lam_X.p, lam_y.p, st);
+      launch_lambda_from_y(topo.n_lam, lam_y.p, sz, lam.p, st);
+    } else {
+      lam.zero(st);
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    last_guess_ms = now_ms() - t0;
+    last_learned = true;
   }
 
   bool fell_back = false;  // FP32 requested but the factor fell back to FP64
@@ -197,6 +316,10 @@ struct fetidp_solver {
       if (!(h_nu[m] > 0.0) || !(h_kappa[m] >= 0.0)) throw InputError("setup: need nu > 0, kappa >= 0");
     const double t0 = now_ms();
     setup_done = device_ready = factored = rhs_set = false;
+    graph_built = false;
+    mp_graph_free(lam_graph);
+    if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));  // sizes change with the plans
+    pcpg_exec = nullptr;
     MeshInput in{n_nodes, n_tets, h_t2n, h_sub, n_pec, h_pec};
     std::string e = build_topology(in, topo);
     if (!e.empty()) throw InputError("setup: " + e);
@@ -295,6 +418,11 @@ struct fetidp_solver {
     qf.alloc(nf);
     rglob.alloc(nf);
     partial.alloc(3 * kDotBlocks);
+    dsc.alloc(kScCount);
+    if (!hsc) {
+      HIPCHECK(hipHostMalloc((void**)&hsc, sizeof(double) * kScCount, hipHostMallocMapped));
+      HIPCHECK(hipHostGetDevicePointer((void**)&hsc_dev, hsc, 0));
+    }
     cpool.alloc(plans.coarse_pool_elems);
     cinv.alloc(plans.coarse_inv_elems);
     sinv.alloc(size_t(topo.n_c) * size_t(std::max<int64_t>(topo.n_c, 1)));
@@ -324,6 +452,13 @@ struct fetidp_solver {
     jq.upload(h_jq, size_t(12 * topo.n_tets));
     rhs_from_J = true;
     rhs_set = true;
+    if (factored) {
+      // same matrix, new source: re-integrate (A_e is rewritten with identical
+      // values) and re-assemble f, so no refactorization is needed
+      launch_integrate(topo.n_tets, xyz.p, t2n.p, mat.p, nu.p, kappa.p, jq.p, 1, Ae.p, nullptr, Fe.p, st);
+      launch_assemble_f64(plans.rhs_global.size(), rhs_target.p, rhs_ptr.p, rhs_src.p, Fe.p, f_glob.p, st);
+      HIPCHECK(hipStreamSynchronize(st));
+    }
   }
 
   void set_rhs_free(const double* h_f) {
@@ -499,12 +634,57 @@ struct fetidp_solver {
     return o[1] > 0 ? std::sqrt(o[0] / o[1]) : 0.0;
   }
 
+  // One PCPG iteration as a fixed kernel chain: the scalars alpha and beta stay
+  // on the device (k_pcpg_alpha / k_pcpg_beta), so the chain can be captured.
+  template <typename T>
+  void pcpg_iteration_kernels(const DBuf<T>& W) {
+    const int64_t nl = topo.n_lam;
+    apply_F<T>(W, p.p, q.p);
+    const double* a1[1] = {p.p};
+    const double* b1[1] = {q.p};
+    launch_dots(nl, 1, a1, b1, partial.p, st);
+    launch_pcpg_alpha(partial.p, dsc.p, hsc_dev, st);
+    launch_axpy2_dev(nl, dsc.p, p.p, q.p, lam.p, r.p, st);
+    apply_M(r.p, z.p);
+    const double* a2[2] = {r.p, r.p};
+    const double* b2[2] = {r.p, z.p};
+    launch_dots(nl, 2, a2, b2, partial.p, st);
+    launch_pcpg_beta(partial.p, dsc.p, hsc_dev, 0, st);
+    launch_xpby_dev(nl, z.p, dsc.p, p.p, st);
+  }
+
+  // Launch one iteration: replay of the captured graph, or the same kernels eagerly.
+  template <typename T>
+  void pcpg_iteration(const DBuf<T>& W) {
+    if (!cfg.graphs || profiling()) {
+      pcpg_iteration_kernels<T>(W);
+      return;
+    }
+    const void* key[3] = {W.p, Pf.p, sinv.p};
+    if (!pcpg_exec || pcpg_key_t != int(sizeof(T)) || std::memcmp(key, pcpg_key, sizeof(key))) {
+      if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));
+      pcpg_exec = nullptr;
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      pcpg_iteration_kernels<T>(W);
+      HIPCHECK(hipStreamEndCapture(st, &g));
+      const hipError_t e = hipGraphInstantiate(&pcpg_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHECK(e);
+      std::memcpy(pcpg_key, key, sizeof(key));
+      pcpg_key_t = int(sizeof(T));
+    }
+    HIPCHECK(hipGraphLaunch(pcpg_exec, st));
+  }
+
   // PCPG on F lambda = d from the current lambda until ||r||/||d|| < tol.
-  // `fresh` recomputes r, z, p from lambda; otherwise continues the recurrence.
+  // `fresh` recomputes r, z, p from lambda; otherwise continues the recurrence
+  // (r, z, p and the device scalar r.z persist between calls).
   template <typename T>
   int pcpg(const DBuf<T>& W, double tol, int max_it, bool fresh, int& its,
            double& dn) {
     const int64_t nl = topo.n_lam;
+    volatile const double* hs = hsc;
     double o[3];
     if (fresh) {
       const double* a1[1] = {dv.p};
@@ -514,46 +694,44 @@ struct fetidp_solver {
         return FETIDP_SUCCESS;
       }
       apply_F<T>(W, lam.p, q.p);
+      if (last_learned) {
+        // scale the learned start by its F-norm optimum alpha = (lam.d)/(lam.F lam),
+        // so ||lam0 - lam*||_F <= ||lam*||_F whatever the network predicted
+        const double* a2[2] = {lam.p, lam.p};
+        const double* b2[2] = {dv.p, q.p};
+        dots(2, a2, b2, nl, o);
+        const double alpha = o[1] > 0.0 ? o[0] / o[1] : 0.0;
+        launch_scale2(nl, alpha, lam.p, q.p, st);
+      }
       HIPCHECK(hipMemcpyAsync(r.p, dv.p, sizeof(double) * nl, hipMemcpyDeviceToDevice, st));
       launch_axpy(nl, -1.0, q.p, r.p, st);
       apply_M(r.p, z.p);
       const double* a[2] = {r.p, r.p};
       const double* b[2] = {r.p, z.p};
-      dots(2, a, b, nl, o);
-      last_dual = init_dual = std::sqrt(o[0]) / dn;
-      hist.push_back(last_dual);
-      rz_ = o[1];
+      launch_dots(nl, 2, a, b, partial.p, st);
+      launch_pcpg_beta(partial.p, dsc.p, hsc_dev, 1, st);
       HIPCHECK(hipMemcpyAsync(p.p, z.p, sizeof(double) * nl, hipMemcpyDeviceToDevice, st));
+      HIPCHECK(hipStreamSynchronize(st));
+      last_dual = init_dual = std::sqrt(hs[kScRR]) / dn;
+      hist.push_back(last_dual);
       prev_ = last_dual;
     }
     if (last_dual < tol) return FETIDP_SUCCESS;
     for (int k = 0; k < max_it; ++k) {
-      apply_F<T>(W, p.p, q.p);
-      const double* a1[1] = {p.p};
-      const double* b1[1] = {q.p};
-      const double pq = dots(1, a1, b1, nl, o);
-      if (!(std::fabs(pq) >= 1e-300)) return FETIDP_NUMERICAL_ERROR;
-      const double alpha = rz_ / pq;
-      launch_axpy2(nl, alpha, p.p, q.p, lam.p, r.p, st);
-      apply_M(r.p, z.p);
-      const double* a[2] = {r.p, r.p};
-      const double* b[2] = {r.p, z.p};
-      dots(2, a, b, nl, o);
+      pcpg_iteration<T>(W);
+      HIPCHECK(hipStreamSynchronize(st));
+      if (!(std::fabs(hs[kScPQ]) >= 1e-300)) return FETIDP_NUMERICAL_ERROR;
       ++its;
-      last_dual = std::sqrt(o[0]) / dn;
+      last_dual = std::sqrt(hs[kScRR]) / dn;
       hist.push_back(last_dual);
       if (!std::isfinite(last_dual)) return FETIDP_NUMERICAL_ERROR;
       if (its > 10 && last_dual > prev_ * 1000.0) return FETIDP_DIVERGED;
-      // finish the recurrence before any return so a later call can resume it
-      const double beta = o[1] / rz_;
-      rz_ = o[1];
       prev_ = last_dual;
-      launch_xpby(nl, z.p, beta, p.p, st);
       if (last_dual < tol) return FETIDP_SUCCESS;
     }
     return FETIDP_MAX_ITER_REACHED;
   }
-  double rz_ = 0.0, prev_ = 0.0;
+  double prev_ = 0.0;
 
   template <typename T>
   int solve_impl(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W,
@@ -566,6 +744,7 @@ struct fetidp_solver {
     if (!fp32()) {
       condense<T>(pool, inv, W, f_glob.p);
       if (h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
+      else if (lam_model) learned_guess();
       else lam.zero(st);
       double tol = cfg.dual_tolerance;
       bool fresh = true;
@@ -595,6 +774,7 @@ struct fetidp_solver {
       for (k = 0; k < cfg.max_refinements; ++k) {
         condense<T>(pool, inv, W, f_cur.p);
         if (k == 0 && h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
+        else if (k == 0 && lam_model) learned_guess();
         else lam.zero(st);
         status = pcpg<T>(W, cfg.inner_tolerance, cfg.max_iterations - its, true, its, dn);
         if (status == FETIDP_NUMERICAL_ERROR || status == FETIDP_DIVERGED) break;
@@ -638,6 +818,8 @@ struct fetidp_solver {
   int solve(const double* h_lam0, double* h_x, fetidp_stats& S) {
     if (!factored) throw InputError("solve before factor");
     const double t0 = now_ms();
+    last_guess_ms = 0.0;
+    last_learned = false;
     int status = fp32() ? solve_impl<float>(pool32, inv32, W32, h_lam0, S)
                         : solve_impl<double>(pool64, inv64, W64, h_lam0, S);
     if (h_x) HIPCHECK(hipMemcpyAsync(h_x, xtot.p, sizeof(double) * topo.n_free, hipMemcpyDeviceToHost, st));
@@ -645,6 +827,8 @@ struct fetidp_solver {
     prof_collect();
     S.solve_ms = now_ms() - t0;
     S.status = status;
+    S.guess_ms = last_guess_ms;
+    S.learned_guess = last_learned ? 1 : 0;
     return status;
   }
 
@@ -722,6 +906,7 @@ void fetidp_config_default(fetidp_config* cfg) {
   cfg->max_refinements = 12;
   cfg->verbose = 0;
   cfg->inner_tolerance = 1e-2;
+  cfg->graphs = 1;
 }
 
 fetidp_handle fetidp_create(const fetidp_config* cfg) {
@@ -947,7 +1132,7 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
         if (L.mode == mode)
           for (int64_t i = 0; i < L.n; ++i) {
             const TileWork& w = sch[q]->items[L.off + i];
-            fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (mode == 1 ? 2.0 : 0.0));
+            fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (mode == 1 ? 1.0 + (w.flags & 1) : 0.0));
           }
       if (k < cap) out[k++] = fl;
     }
@@ -957,6 +1142,40 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
 void fetidp_reset_profile(fetidp_handle h) {
   if (!h) return;
   for (int i = 0; i < fetidp_solver::kProfSlots; ++i) h->prof_ms[i] = h->prof_cnt[i] = 0.0;
+}
+
+int32_t fetidp_set_lambda0_model(fetidp_handle h, struct emag_mpnn* model) {
+  return guard(h, [&] {
+    h->lam_model = model;
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int64_t fetidp_get_dual_graph(fetidp_handle h, int32_t* row_ptr, int32_t* col, float* coef) {
+  if (!h) return -1;
+  int64_t nnz = -1;
+  const int32_t rc = guard(h, [&] {
+    if (!h->setup_done) throw InputError("get_dual_graph before setup");
+    h->build_dual_graph();
+    nnz = int64_t(h->g_col.size());
+    if (row_ptr) std::memcpy(row_ptr, h->g_rp.data(), sizeof(int32_t) * h->g_rp.size());
+    if (col) std::memcpy(col, h->g_col.data(), sizeof(int32_t) * h->g_col.size());
+    if (coef) std::memcpy(coef, h->g_coef.data(), sizeof(float) * h->g_coef.size());
+    return int32_t(FETIDP_SUCCESS);
+  });
+  return rc == FETIDP_SUCCESS ? nnz : -1;
+}
+
+int32_t fetidp_get_lambda_features(fetidp_handle h, float* features, double* scales) {
+  return guard(h, [&] {
+    if (!features) throw InputError("get_lambda_features: null output");
+    h->with_ops([&] { h->condense<double>(h->pool64, h->inv64, h->W64, h->f_glob.p); },
+                [&] { h->condense<float>(h->pool32, h->inv32, h->W32, h->f_glob.p); });
+    h->lambda_features(scales);
+    HIPCHECK(hipStreamSynchronize(h->st));
+    HIPCHECK(hipMemcpy(features, h->lam_X.p, sizeof(float) * h->topo.n_lam * kMpF, hipMemcpyDeviceToHost));
+    return int32_t(FETIDP_SUCCESS);
+  });
 }
 
 const char* fetidp_last_error(fetidp_handle h) { return h ? h->err.c_str() : "null handle"; }

@@ -98,5 +98,13 @@ void launch_axpy2(int64_t n, double alpha, const double* p, const double* q, dou
                   hipStream_t st);
 void launch_xpby(int64_t n, const double* z, double beta, double* p, hipStream_t st);
 void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t st);
+void launch_scale2(int64_t n, double alpha, double* x, double* y, hipStream_t st);
+// PCPG device scalars (slots of the scalar block)
+enum { kScRZ = 0, kScPQ = 1, kScAlpha = 2, kScBeta = 3, kScRR = 4, kScCount = 8 };
+void launch_pcpg_alpha(const double* partial, double* sc, double* host, hipStream_t st);
+void launch_pcpg_beta(const double* partial, double* sc, double* host, int init, hipStream_t st);
+void launch_axpy2_dev(int64_t n, const double* sc, const double* p, const double* q, double* lam,
+                      double* r, hipStream_t st);
+void launch_xpby_dev(int64_t n, const double* z, const double* sc, double* p, hipStream_t st);
 
 }  // namespace emag

@@ -1155,4 +1155,104 @@ void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_
   hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, st, n, alpha, x, y);
 }
 
+// ---------------------------------------------------------------------------
+// PCPG scalars on the device (so one iteration is a fixed kernel chain that a
+// HIP graph can hold): sc[kScRZ] = r.z of the current residual, then pq, alpha,
+// beta, rr.  `host` (pinned, mapped; may be null) receives the same slots for
+// the host's convergence checks.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, double* red) {
+  red[threadIdx.x] = part[threadIdx.x];
+  __syncthreads();
+  for (int s = kDotBlocks / 2; s >= 1; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+__global__ __launch_bounds__(kDotBlocks) void k_pcpg_alpha(const double* __restrict__ partial,
+                                                           double* __restrict__ sc, double* host) {
+  __shared__ double red[kDotBlocks];
+  const double pq = sum_partials(partial, red);
+  if (threadIdx.x == 0) {
+    const double alpha = fabs(pq) >= 1e-300 ? sc[kScRZ] / pq : 0.0;
+    sc[kScPQ] = pq;
+    sc[kScAlpha] = alpha;
+    if (host) {
+      host[kScPQ] = pq;
+      host[kScAlpha] = alpha;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kDotBlocks) void k_pcpg_beta(const double* __restrict__ partial,
+                                                          double* __restrict__ sc, double* host,
+                                                          int init) {
+  __shared__ double red[kDotBlocks];
+  const double rr = sum_partials(partial, red);
+  __syncthreads();
+  const double rz = sum_partials(partial + kDotBlocks, red);
+  if (threadIdx.x == 0) {
+    const double beta = init ? 0.0 : rz / sc[kScRZ];
+    sc[kScBeta] = beta;
+    sc[kScRZ] = rz;
+    sc[kScRR] = rr;
+    if (host) {
+      host[kScBeta] = beta;
+      host[kScRZ] = rz;
+      host[kScRR] = rr;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_axpy2_dev(int64_t n, const double* __restrict__ sc,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ q,
+                                                   double* __restrict__ lam, double* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = sc[kScAlpha];
+  lam[i] += alpha * p[i];
+  r[i] -= alpha * q[i];
+}
+
+__global__ __launch_bounds__(256) void k_xpby_dev(int64_t n, const double* __restrict__ z,
+                                                  const double* __restrict__ sc,
+                                                  double* __restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = z[i] + sc[kScBeta] * p[i];
+}
+
+void launch_pcpg_alpha(const double* partial, double* sc, double* host, hipStream_t st) {
+  hipLaunchKernelGGL(k_pcpg_alpha, dim3(1), dim3(kDotBlocks), 0, st, partial, sc, host);
+}
+void launch_pcpg_beta(const double* partial, double* sc, double* host, int init, hipStream_t st) {
+  hipLaunchKernelGGL(k_pcpg_beta, dim3(1), dim3(kDotBlocks), 0, st, partial, sc, host, init);
+}
+void launch_axpy2_dev(int64_t n, const double* sc, const double* p, const double* q, double* lam,
+                      double* r, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy2_dev, dim3((n + 255) / 256), dim3(256), 0, st, n, sc, p, q, lam, r);
+}
+void launch_xpby_dev(int64_t n, const double* z, const double* sc, double* p, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_xpby_dev, dim3((n + 255) / 256), dim3(256), 0, st, n, z, sc, p);
+}
+
+// x *= alpha, y *= alpha
+__global__ __launch_bounds__(256) void k_scale2(int64_t n, double alpha, double* __restrict__ x,
+                                               double* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] *= alpha;
+  y[i] *= alpha;
+}
+
+void launch_scale2(int64_t n, double alpha, double* x, double* y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale2, dim3((n + 255) / 256), dim3(256), 0, st, n, alpha, x, y);
+}
+
 }  // namespace emag

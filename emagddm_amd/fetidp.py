@@ -39,6 +39,8 @@ class SolverResult:
     factor_time_ms: float = 0.0
     symbolic_time_ms: float = 0.0
     fell_back_to_fp64: bool = False
+    guess_time_ms: float = 0.0
+    learned_guess: bool = False
 
 
 def _dp(a):
@@ -62,7 +64,7 @@ class FETIDPSolver:
 
     def __init__(self, precision="fp64", tolerance=1e-8, dual_tolerance=1e-11,
                  max_iterations=1000, max_refinements=12, profile=False,
-                 inner_tolerance=1e-2):
+                 inner_tolerance=1e-2, graphs=True):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.fetidp_config_default(C.byref(cfg))
@@ -73,12 +75,14 @@ class FETIDPSolver:
         cfg.max_refinements = max_refinements
         cfg.verbose = 2 if profile else 0
         cfg.inner_tolerance = inner_tolerance
+        cfg.graphs = 1 if graphs else 0
         self.precision = precision
         self._h = self._lib.fetidp_create(C.byref(cfg))
         if not self._h:
             raise RuntimeError("fetidp_create failed")
         self._keep = []
         self.sizes = None
+        self._model = None
 
     # --------------------------------------------------------------- plumbing
     def _check(self, rc, what):
@@ -155,7 +159,35 @@ class FETIDPSolver:
                             refinements=st.refinements, dual_residual=st.dual_residual,
                             initial_dual_residual=st.initial_dual_residual,
                             factor_time_ms=st.factor_ms, symbolic_time_ms=st.symbolic_ms,
-                            fell_back_to_fp64=bool(st.fell_back))
+                            fell_back_to_fp64=bool(st.fell_back), guess_time_ms=st.guess_ms,
+                            learned_guess=bool(st.learned_guess))
+
+    # --------------------------------------------------------------- learned lambda0
+    def set_lambda0_model(self, model):
+        """Attach a LambdaMPNN (or None): solve() then starts PCPG from its lambda0."""
+        self._check(self._lib.fetidp_set_lambda0_model(self._h, model.handle if model else None),
+                    "fetidp_set_lambda0_model")
+        self._model = model  # keep the model alive while attached
+
+    def dual_graph(self):
+        """(row_ptr, col, coef) of the multiplier interface graph."""
+        nnz = self._lib.fetidp_get_dual_graph(self._h, None, None, None)
+        if nnz < 0:
+            raise RuntimeError(f"fetidp_get_dual_graph: {self.last_error()}")
+        rp = np.zeros(self.sizes["n_lambda"] + 1, dtype=np.int32)
+        col = np.zeros(max(nnz, 1), dtype=np.int32)
+        coef = np.zeros(max(nnz, 1), dtype=np.float32)
+        self._lib.fetidp_get_dual_graph(self._h, _ip(rp), _ip(col),
+                                        coef.ctypes.data_as(C.POINTER(C.c_float)))
+        return rp, col[:nnz], coef[:nnz]
+
+    def lambda_features(self):
+        """(features (n_lambda, 16) float32, (rms d, rms z)) for the current factor and RHS."""
+        X = np.zeros((self.sizes["n_lambda"], 16), dtype=np.float32)
+        sc = np.zeros(2)
+        self._check(self._lib.fetidp_get_lambda_features(
+            self._h, X.ctypes.data_as(C.POINTER(C.c_float)), _dp(sc)), "fetidp_get_lambda_features")
+        return X, sc
 
     # --------------------------------------------------------------- introspection
     def topology(self):

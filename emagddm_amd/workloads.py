@@ -190,6 +190,37 @@ WORKLOADS = {
 }
 
 
+def _draw_materials(rng, materials: str, n_sub: int):
+    if materials == "uniform":
+        return np.array([1.0]), np.array([1e-2])
+    if materials == "nu_per_sub":
+        return 10.0 ** rng.uniform(-3, 0, size=n_sub), np.full(n_sub, 1e-2)
+    raise ValueError(materials)
+
+
+def _draw_loops(rng, loops):
+    if isinstance(loops, tuple):
+        count = int(rng.integers(loops[0], loops[1] + 1))
+        return random_loops(rng, count, 0.05, 0.2, 0.04)
+    return random_loops(rng, loops)
+
+
+def family_parameters(name: str, seed: int):
+    """(nu, kappa, loops) that make_workload(name, seed) draws, for reuse on the
+    mesh, partition and material map of seed 0.  Families whose material map
+    itself depends on the seed ("contrast") are not supported."""
+    c = WORKLOADS[name]
+    n, parts = c["n"], c["parts"]
+    if c["materials"] not in ("uniform", "nu_per_sub"):
+        raise ValueError(f"{name}: material map depends on the seed")
+    rng = np.random.default_rng(np.random.SeedSequence([seed, n, parts]))
+    if c["jitter"] is not None:  # consume the cut draws exactly as make_problem does
+        for _ in range(3):
+            jittered_widths(n, parts, c["jitter"][0], c["jitter"][1], rng)
+    nu, kappa = _draw_materials(rng, c["materials"], parts ** 3)
+    return nu, kappa, _draw_loops(rng, c["loops"])
+
+
 def make_problem(n: int, parts: int, jitter=None, materials="uniform", loops=2,
                  seed: int = 0, name: str = "custom") -> Problem:
     """Build one problem.  `seed` fixes cuts, materials and sources."""
@@ -204,14 +235,9 @@ def make_problem(n: int, parts: int, jitter=None, materials="uniform", loops=2,
     sub = box_partition(n, widths)
     n_sub = parts ** 3
     pec = pec_edges(n, tets)
-    if materials == "uniform":
-        mat = np.zeros(tets.shape[0], dtype=np.int32)
-        nu = np.array([1.0])
-        kappa = np.array([1e-2])
-    elif materials == "nu_per_sub":
-        mat = sub.copy()
-        nu = 10.0 ** rng.uniform(-3, 0, size=n_sub)
-        kappa = np.full(n_sub, 1e-2)
+    if materials in ("uniform", "nu_per_sub"):
+        mat = np.zeros(tets.shape[0], dtype=np.int32) if materials == "uniform" else sub.copy()
+        nu, kappa = _draw_materials(rng, materials, n_sub)
     elif materials == "contrast":
         nu_s = 10.0 ** rng.uniform(-3, 0, size=n_sub)
         ka_s = 10.0 ** rng.uniform(-4, 0, size=n_sub)
@@ -224,11 +250,7 @@ def make_problem(n: int, parts: int, jitter=None, materials="uniform", loops=2,
         kappa = np.concatenate([ka_s, ka_s])
     else:
         raise ValueError(materials)
-    if isinstance(loops, tuple):
-        count = int(rng.integers(loops[0], loops[1] + 1))
-        lps = random_loops(rng, count, 0.05, 0.2, 0.04)
-    else:
-        lps = random_loops(rng, loops)
+    lps = _draw_loops(rng, loops)
     jq = loop_source(quadrature_points(xyz, tets), lps)
     return Problem(name=name, n=n, xyz=xyz, tet2node=tets, tet_material=mat, nu=nu,
                    kappa=kappa, tet_subdomain=sub, n_sub=n_sub, pec_pairs=pec, jq=jq,

@@ -45,6 +45,9 @@ typedef struct fetidp_config {
   int32_t max_refinements;  /* FP32 mode: outer FP64 flexible-CG steps */
   int32_t verbose;          /* bit 1 (value 2): record per-kernel-class event timings */
   double inner_tolerance;   /* FP32 mode: PCPG stop ||r||/||d|| of each refinement step */
+  int32_t graphs;           /* 1: replay each PCPG iteration as one captured HIP graph
+                               (ignored while event profiling is on) */
+  int32_t reserved;
 } fetidp_config;
 
 typedef struct fetidp_stats {
@@ -58,6 +61,9 @@ typedef struct fetidp_stats {
   double factor_ms;         /* numeric setup: integration, assembly, factorization, coarse */
   double solve_ms;          /* RHS condensation, PCPG, recovery, residual check */
   double symbolic_ms;       /* host topology + plans + upload (fetidp_setup) */
+  double guess_ms;          /* learned lambda0: features + network (0 without a model) */
+  int32_t learned_guess;    /* 1: PCPG started from the attached model's lambda0 */
+  int32_t reserved;
 } fetidp_stats;
 
 typedef struct fetidp_solver* fetidp_handle;
@@ -88,7 +94,9 @@ int32_t fetidp_set_materials(fetidp_handle h, int32_t n_mat, const double* nu, c
  * (point q has barycentric weight (5+3sqrt5)/20 on local node q, (5-sqrt5)/20
  * on the others); F_e(i) = sum_q |T|/4 W_i(x_q).J_q.  Replaces the zero
  * MagneticVector3DIntegrator::computeSourceVector
- * (src/numeric/magnetic_vector_3d_integrator.cpp:343-364). */
+ * (src/numeric/magnetic_vector_3d_integrator.cpp:343-364).  After
+ * fetidp_factor a new source is assembled at once and reuses the factor
+ * (factor once, solve many). */
 int32_t fetidp_set_rhs_tet_J(fetidp_handle h, const double* jq);
 
 /* Assembled free-dof right-hand side f[n_free] (replaces set_rhs by J). */
@@ -142,6 +150,26 @@ int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S);
  * local modes 0..2 and coarse modes 0..2.  Returns the number of values written. */
 int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap);
 void fetidp_reset_profile(fetidp_handle h);
+
+/* Learned initial multiplier (include/emagddm_mpnn.h).  With a model attached
+ * and lambda0 == NULL, fetidp_solve starts PCPG from lambda0 = s_z * y, where
+ * y is the network's output on the interface graph for the features of
+ * fetidp_get_lambda_features; model == NULL detaches.  The handle does not own
+ * the model, which must outlive its use.  No reference counterpart: the
+ * reference always starts from zero (src/numeric/iterative_solvers.cpp:424-503). */
+struct emag_mpnn;
+int32_t fetidp_set_lambda0_model(fetidp_handle h, struct emag_mpnn* model);
+
+/* Interface graph of the multipliers: nodes = dual edges (lambda order), i ~ j
+ * when their mesh edges share a node and their unit tangents are not
+ * orthogonal; coef = t_i . t_j.  Returns nnz (< 0 on error); arrays
+ * row_ptr[n_lambda+1], col[nnz], coef[nnz] are filled when non-NULL. */
+int64_t fetidp_get_dual_graph(fetidp_handle h, int32_t* row_ptr, int32_t* col, float* coef);
+
+/* Network input features (n_lambda x 16, FP32) of the current factor and RHS,
+ * computed on the GPU from d, z = M^-1 d and the stiffness scaling; scales[0..1]
+ * = rms(d), rms(z) (the output scale s_z).  After fetidp_factor. */
+int32_t fetidp_get_lambda_features(fetidp_handle h, float* features, double* scales);
 
 const char* fetidp_last_error(fetidp_handle h);
 void fetidp_destroy(fetidp_handle h);

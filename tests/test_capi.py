@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "emagddm_fetidp.h")).read()
+    text = "".join(open(os.path.join(ROOT, "include", h)).read() for h in _lib.HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return set(re.findall(r"\b(fetidp_[A-Za-z_0-9]+|emag_[A-Za-z_0-9]+)\s*\(", text))
 
