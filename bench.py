@@ -20,6 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILE_STEPS = 2  # untimed event-profiled solves per variant (roofline and breakdown)
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes over the same workload, each scaled by what the counter reads for a 512 MiB
@@ -55,7 +56,7 @@ def run_variant(prob, variant, steps, warmup):
     from emagddm_amd import FETIDPSolver, SolverStatus
     from emagddm_amd.lambda0 import LambdaMPNN, default_weights
     precision, _, extra = variant.partition("+")
-    s = FETIDPSolver(precision=precision, profile=True)
+    s = FETIDPSolver(precision=precision, profile=False)  # timed run: graphs on, no events
     s.setup_problem(prob)
     model = None
     if extra == "learned":
@@ -70,7 +71,6 @@ def run_variant(prob, variant, steps, warmup):
     for _ in range(warmup):
         s.factor()
         r = s.solve()
-    s.reset_profile()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     results = []
@@ -82,7 +82,14 @@ def run_variant(prob, variant, steps, warmup):
     r = results[-1]
     ok = all(x.status == SolverStatus.SUCCESS and x.residual_norm <= 1e-8
              and not x.fell_back_to_fp64 for x in results)
+    # untimed profiled passes: per-kernel-class HIP events on the solver's stream
+    s.set_profile(True)
+    s.reset_profile()
+    for _ in range(PROFILE_STEPS):
+        s.factor()
+        s.solve()
     prof = s.profile()
+    s.set_profile(False)
     return dict(solver=s, model=model, result=r, ok=ok, s_per_solve=(t1 - t0) / steps,
                 factor_ms=r.factor_time_ms, solve_ms=r.solve_time_ms,
                 iterations=r.iterations, refinements=r.refinements,
@@ -95,7 +102,7 @@ def roofline(v, precision, workload):
     candidates = ["tile_column", "tile_schur", "tile_diag"]
     name = max(candidates, key=lambda k: prof[k][0])
     ms, launches = prof[name]
-    flops = prof["flops"][name] * v["steps"]
+    flops = prof["flops"][name] * PROFILE_STEPS
     if launches == 0 or ms == 0:
         return None, name
     per_launch_flops = flops / launches
@@ -107,7 +114,7 @@ def roofline(v, precision, workload):
     return dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                 frac=round(achieved / peak, 4), traffic=MEASURED_TRAFFIC.get((workload, kernel)),
                 kernel=kernel,
-                launches_per_step=launches // v["steps"],
+                launches_per_step=launches // PROFILE_STEPS,
                 flops_per_launch=per_launch_flops, avg_launch_ms=avg_ms), name
 
 
@@ -175,7 +182,7 @@ def main():
         print(f"[bench] {prec}: {v['s_per_solve']:.4f} s/solve its={v['iterations']} "
               f"ref={v['refinements']} res={v['residual']:.2e} ok={v['ok']}", file=sys.stderr,
               flush=True)
-        prof = {k: (round(ms / a.steps, 3), n // a.steps) for k, (ms, n) in
+        prof = {k: (round(ms / PROFILE_STEPS, 3), n // PROFILE_STEPS) for k, (ms, n) in
                 ((k, v["profile"][k]) for k in v["solver"].PROFILE_SLOTS)}
         print(f"[bench] {prec} per-step event ms (ms, launches): {prof}", file=sys.stderr,
               flush=True)

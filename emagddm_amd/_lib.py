@@ -33,7 +33,8 @@ class Stats(C.Structure):
 
 
 EXPORTS = [
-    "fetidp_config_default", "fetidp_create", "fetidp_setup", "fetidp_set_materials",
+    "fetidp_config_default", "fetidp_create", "fetidp_set_config", "fetidp_get_config",
+    "fetidp_setup", "fetidp_set_materials",
     "fetidp_set_rhs_tet_J", "fetidp_set_rhs_free", "fetidp_factor", "fetidp_solve",
     "fetidp_get_sizes", "fetidp_get_topology", "fetidp_get_global_nnz",
     "fetidp_get_global_system", "fetidp_apply_dual_operator", "fetidp_apply_preconditioner",
@@ -70,6 +71,8 @@ def load():
     lib.fetidp_config_default.restype = None
     lib.fetidp_create.argtypes = [C.POINTER(Config)]
     lib.fetidp_create.restype = P
+    lib.fetidp_set_config.argtypes = [P, C.POINTER(Config)]
+    lib.fetidp_get_config.argtypes = [P, C.POINTER(Config)]
     lib.fetidp_setup.argtypes = [P, C.c_int64, dp, C.c_int64, ip, ip, C.c_int32, dp, dp, ip,
                                  C.c_int64, ip]
     lib.fetidp_set_materials.argtypes = [P, C.c_int32, dp, dp]

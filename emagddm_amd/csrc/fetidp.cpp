@@ -187,6 +187,8 @@ struct fetidp_solver {
     for (hipEvent_t e : prof_pool) (void)hipEventDestroy(e);
     mp_graph_free(lam_graph);
     if (pcpg_exec) (void)hipGraphExecDestroy(pcpg_exec);
+    for (hipEvent_t e : guess_ev)
+      if (e) (void)hipEventDestroy(e);
     if (hsc) (void)hipHostFree(hsc);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -263,38 +265,32 @@ struct fetidp_solver {
     }
   }
 
-  // network features from the condensed dual RHS dv; returns the output scale
-  // s_z = rms(M^-1 d) (0 when d vanishes)
-  double lambda_features(double* scales) {
+  // network features from the condensed dual RHS dv, all on the stream:
+  // z = M^-1 d, fsc = {1/rms(d), 1/rms(z), rms(z), rms(d)}, features into lam_X
+  DBuf<double> fsc;
+  hipEvent_t guess_ev[2] = {nullptr, nullptr};
+  void lambda_features() {
     ensure_lam_device();
     const int64_t nl = topo.n_lam;
+    fsc.alloc(4);
     apply_M(dv.p, z.p);
     const double* a[2] = {dv.p, z.p};
-    double o[2];
-    dots(2, a, a, nl, o);
-    const double sd = std::sqrt(o[0] / double(nl)), sz = std::sqrt(o[1] / double(nl));
-    if (scales) { scales[0] = sd; scales[1] = sz; }
-    if (!(sd > 0.0) || !(sz > 0.0)) {
-      HIPCHECK(hipMemsetAsync(lam_X.p, 0, sizeof(float) * nl * kMpF, st));
-      return 0.0;
-    }
-    launch_lambda_features(nl, dv.p, z.p, 1.0 / sd, 1.0 / sz, rho.p, lam_pos.p, lam_dir.p, lam_X.p, st);
-    return sz;
+    launch_dots(nl, 2, a, a, partial.p, st);
+    launch_feature_scales(partial.p, kDotBlocks, nl, fsc.p, st);
+    launch_lambda_features(nl, dv.p, z.p, fsc.p, rho.p, lam_pos.p, lam_dir.p, lam_X.p, st);
   }
 
-  // lam = s_z * network(features)
+  // lam = s_z * network(features); timed with events, no host synchronization
   void learned_guess() {
-    HIPCHECK(hipStreamSynchronize(st));  // time the guess alone, not queued condensation
-    const double t0 = now_ms();
-    const double sz = lambda_features(nullptr);
-    if (sz > 0.0) {
-      mpnn_forward(lam_model, lam_graph, 1, lam_X.p, lam_y.p, st);
-      launch_lambda_from_y(topo.n_lam, lam_y.p, sz, lam.p, st);
-    } else {
-      lam.zero(st);
+    if (!guess_ev[0]) {
+      HIPCHECK(hipEventCreate(&guess_ev[0]));
+      HIPCHECK(hipEventCreate(&guess_ev[1]));
     }
-    HIPCHECK(hipStreamSynchronize(st));
-    last_guess_ms = now_ms() - t0;
+    HIPCHECK(hipEventRecord(guess_ev[0], st));
+    lambda_features();
+    mpnn_forward(lam_model, lam_graph, 1, lam_X.p, lam_y.p, st);
+    launch_lambda_from_y(topo.n_lam, lam_y.p, fsc.p, lam.p, st);
+    HIPCHECK(hipEventRecord(guess_ev[1], st));
     last_learned = true;
   }
 
@@ -824,6 +820,11 @@ struct fetidp_solver {
                         : solve_impl<double>(pool64, inv64, W64, h_lam0, S);
     if (h_x) HIPCHECK(hipMemcpyAsync(h_x, xtot.p, sizeof(double) * topo.n_free, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    if (last_learned) {
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, guess_ev[0], guess_ev[1]));
+      last_guess_ms = ms;
+    }
     prof_collect();
     S.solve_ms = now_ms() - t0;
     S.status = status;
@@ -909,24 +910,47 @@ void fetidp_config_default(fetidp_config* cfg) {
   cfg->graphs = 1;
 }
 
+// Copy of `in` with unset fields at their defaults; false for an unknown precision.
+static bool normalized_config(const fetidp_config* in, fetidp_config& out) {
+  if (in) out = *in;
+  else fetidp_config_default(&out);
+  if (out.precision != FETIDP_FP64 && out.precision != FETIDP_FP32_REFINE) return false;
+  if (out.max_iterations <= 0) out.max_iterations = 1000;
+  if (!(out.tolerance > 0)) out.tolerance = 1e-8;
+  if (!(out.dual_tolerance > 0)) out.dual_tolerance = 1e-11;
+  if (out.max_refinements <= 0) out.max_refinements = 12;
+  if (!(out.inner_tolerance > 0)) out.inner_tolerance = 1e-2;
+  return true;
+}
+
 fetidp_handle fetidp_create(const fetidp_config* cfg) {
   try {
+    fetidp_config c;
+    if (!normalized_config(cfg, c)) return nullptr;
     auto* h = new fetidp_solver();
-    if (cfg) h->cfg = *cfg;
-    else fetidp_config_default(&h->cfg);
-    if (h->cfg.precision != FETIDP_FP64 && h->cfg.precision != FETIDP_FP32_REFINE) {
-      delete h;
-      return nullptr;
-    }
-    if (h->cfg.max_iterations <= 0) h->cfg.max_iterations = 1000;
-    if (!(h->cfg.tolerance > 0)) h->cfg.tolerance = 1e-8;
-    if (!(h->cfg.dual_tolerance > 0)) h->cfg.dual_tolerance = 1e-11;
-    if (h->cfg.max_refinements <= 0) h->cfg.max_refinements = 12;
-    if (!(h->cfg.inner_tolerance > 0)) h->cfg.inner_tolerance = 1e-2;
+    h->cfg = c;
     return h;
   } catch (...) {
     return nullptr;
   }
+}
+
+int32_t fetidp_set_config(fetidp_handle h, const fetidp_config* cfg) {
+  return guard(h, [&] {
+    fetidp_config c;
+    if (!cfg || !normalized_config(cfg, c)) throw InputError("set_config: bad configuration");
+    if (c.precision != h->cfg.precision) h->factored = false;  // factors are per precision
+    h->cfg = c;
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_get_config(fetidp_handle h, fetidp_config* cfg) {
+  return guard(h, [&] {
+    if (!cfg) throw InputError("get_config: null output");
+    *cfg = h->cfg;
+    return int32_t(FETIDP_SUCCESS);
+  });
 }
 
 int32_t fetidp_setup(fetidp_handle h, int64_t n_nodes, const double* xyz, int64_t n_tets,
@@ -1171,9 +1195,15 @@ int32_t fetidp_get_lambda_features(fetidp_handle h, float* features, double* sca
     if (!features) throw InputError("get_lambda_features: null output");
     h->with_ops([&] { h->condense<double>(h->pool64, h->inv64, h->W64, h->f_glob.p); },
                 [&] { h->condense<float>(h->pool32, h->inv32, h->W32, h->f_glob.p); });
-    h->lambda_features(scales);
+    h->lambda_features();
     HIPCHECK(hipStreamSynchronize(h->st));
     HIPCHECK(hipMemcpy(features, h->lam_X.p, sizeof(float) * h->topo.n_lam * kMpF, hipMemcpyDeviceToHost));
+    if (scales) {
+      double f[4];
+      HIPCHECK(hipMemcpy(f, h->fsc.p, sizeof(f), hipMemcpyDeviceToHost));
+      scales[0] = f[3];
+      scales[1] = f[2];
+    }
     return int32_t(FETIDP_SUCCESS);
   });
 }

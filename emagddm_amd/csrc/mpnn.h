@@ -42,9 +42,12 @@ namespace emag {
 // y[u] = network(X)[u] for u < batch * g.n, on stream st (model workspace grows as needed).
 void mpnn_forward(emag_mpnn* m, const MpGraph& g, int64_t batch, const float* X, float* y,
                   hipStream_t st);
-// Solver glue: node features of the interface multipliers (see kernel comment).
-void launch_lambda_features(int64_t nl, const double* d, const double* z, double inv_sd,
-                            double inv_sz, const double* rho, const int64_t* lam_pos,
-                            const float* dir, float* X, hipStream_t st);
-void launch_lambda_from_y(int64_t nl, const float* y, double scale, double* lam, hipStream_t st);
+// Solver glue (all device-side, no host round trip): fsc = {1/s_d, 1/s_z, s_z, s_d}
+// from the dot partials of d.d and z.z; node features of the interface
+// multipliers (see kernel comment); lam = s_z * y.
+void launch_feature_scales(const double* partial, int nparts, int64_t nl, double* fsc, hipStream_t st);
+void launch_lambda_features(int64_t nl, const double* d, const double* z, const double* fsc,
+                            const double* rho, const int64_t* lam_pos, const float* dir, float* X,
+                            hipStream_t st);
+void launch_lambda_from_y(int64_t nl, const float* y, const double* fsc, double* lam, hipStream_t st);
 }  // namespace emag

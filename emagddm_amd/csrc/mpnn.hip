@@ -367,16 +367,44 @@ __global__ void k_mpnn_probe(const float* a, const float* b, float* d) {
 //   lr = log10(rho_0 / rho_1) / 3, |t| = |unit tangent| per axis
 //   [dd, zz, dd w, zz w, dd lr, zz lr, zz|tx|, zz|ty|, zz|tz|, dd|tx|, dd|ty|, dd|tz|,
 //    zz|lr|, dd|lr|, 0, 0]
+// fsc = {1/s_d, 1/s_z, s_z, s_d} from the partial sums of d.d and z.z
+// (nparts per pair; 0 in the inverses when a norm vanishes)
+__global__ __launch_bounds__(256) void k_feature_scales(const double* __restrict__ partial, int nparts,
+                                                        int64_t nl, double* __restrict__ fsc) {
+  __shared__ double red[2][256];
+  for (int q = 0; q < 2; ++q) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) v += partial[q * nparts + i];
+    red[q][threadIdx.x] = v;
+  }
+  __syncthreads();
+  for (int s = 128; s >= 1; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double sd = sqrt(red[0][0] / double(nl)), sz = sqrt(red[1][0] / double(nl));
+    const bool ok = sd > 0.0 && sz > 0.0;
+    fsc[0] = ok ? 1.0 / sd : 0.0;
+    fsc[1] = ok ? 1.0 / sz : 0.0;
+    fsc[2] = ok ? sz : 0.0;
+    fsc[3] = sd;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_lambda_features(int64_t nl, const double* __restrict__ d,
                                                          const double* __restrict__ z,
-                                                         double inv_sd, double inv_sz,
+                                                         const double* __restrict__ fsc,
                                                          const double* __restrict__ rho,
                                                          const int64_t* __restrict__ lam_pos,
                                                          const float* __restrict__ dir,
                                                          float* __restrict__ X) {
   const int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (l >= nl) return;
-  const double dd = d[l] * inv_sd, zz = z[l] * inv_sz;
+  const double dd = d[l] * fsc[0], zz = z[l] * fsc[1];
   const double r0 = rho[lam_pos[2 * l]], r1 = rho[lam_pos[2 * l + 1]];
   const double w = r1 / (r0 + r1) - 0.5;
   const double lr = log10(r0 / r1) / 3.0;
@@ -401,9 +429,10 @@ __global__ __launch_bounds__(256) void k_lambda_features(int64_t nl, const doubl
 }
 
 __global__ __launch_bounds__(256) void k_lambda_from_y(int64_t nl, const float* __restrict__ y,
-                                                       double scale, double* __restrict__ lam) {
+                                                       const double* __restrict__ fsc,
+                                                       double* __restrict__ lam) {
   const int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (l < nl) lam[l] = scale * double(y[l]);
+  if (l < nl) lam[l] = fsc[2] * double(y[l]);
 }
 
 template <typename T>
@@ -581,17 +610,21 @@ void mpnn_forward(emag_mpnn* m, const MpGraph& g, int64_t batch, const float* X,
   m->forward(g, batch * g.n, X, y, st);
 }
 
-void launch_lambda_features(int64_t nl, const double* d, const double* z, double inv_sd, double inv_sz,
+void launch_feature_scales(const double* partial, int nparts, int64_t nl, double* fsc, hipStream_t st) {
+  hipLaunchKernelGGL(k_feature_scales, dim3(1), dim3(256), 0, st, partial, nparts, nl, fsc);
+}
+
+void launch_lambda_features(int64_t nl, const double* d, const double* z, const double* fsc,
                             const double* rho, const int64_t* lam_pos, const float* dir, float* X,
                             hipStream_t st) {
   if (nl <= 0) return;
-  hipLaunchKernelGGL(k_lambda_features, dim3((nl + 255) / 256), dim3(256), 0, st, nl, d, z, inv_sd,
-                     inv_sz, rho, lam_pos, dir, X);
+  hipLaunchKernelGGL(k_lambda_features, dim3((nl + 255) / 256), dim3(256), 0, st, nl, d, z, fsc, rho,
+                     lam_pos, dir, X);
 }
 
-void launch_lambda_from_y(int64_t nl, const float* y, double scale, double* lam, hipStream_t st) {
+void launch_lambda_from_y(int64_t nl, const float* y, const double* fsc, double* lam, hipStream_t st) {
   if (nl <= 0) return;
-  hipLaunchKernelGGL(k_lambda_from_y, dim3((nl + 255) / 256), dim3(256), 0, st, nl, y, scale, lam);
+  hipLaunchKernelGGL(k_lambda_from_y, dim3((nl + 255) / 256), dim3(256), 0, st, nl, y, fsc, lam);
 }
 
 }  // namespace emag

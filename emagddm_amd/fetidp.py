@@ -84,6 +84,18 @@ class FETIDPSolver:
         self.sizes = None
         self._model = None
 
+    # --------------------------------------------------------------- configuration
+    def get_config(self):
+        cfg = _lib.Config()
+        self._check(self._lib.fetidp_get_config(self._h, C.byref(cfg)), "fetidp_get_config")
+        return cfg
+
+    def set_profile(self, on: bool):
+        """Switch per-kernel-class event timing (disables graph replay while on)."""
+        cfg = self.get_config()
+        cfg.verbose = (cfg.verbose | 2) if on else (cfg.verbose & ~2)
+        self._check(self._lib.fetidp_set_config(self._h, C.byref(cfg)), "fetidp_set_config")
+
     # --------------------------------------------------------------- plumbing
     def _check(self, rc, what):
         if rc != 0:

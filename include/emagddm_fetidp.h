@@ -74,6 +74,12 @@ void fetidp_config_default(fetidp_config* cfg);
  * (include/numeric/em_solver_factory.h:213-279). NULL on failure. */
 fetidp_handle fetidp_create(const fetidp_config* cfg);
 
+/* Change the configuration of an existing handle (as IterativeSolverConfig is
+ * set on a solver, include/numeric/iterative_solvers.h:59-75).  A precision
+ * change drops the current factor (call fetidp_factor again). */
+int32_t fetidp_set_config(fetidp_handle h, const fetidp_config* cfg);
+int32_t fetidp_get_config(fetidp_handle h, fetidp_config* cfg);
+
 /* Symbolic setup.  Mesh, materials, partition and PEC edges; replaces the
  * EMMeshData -> GlobalEdgeIDGenerator -> EMDOFManager chain
  * (src/fe_em/global_edge_id_generator.cpp:30-110, src/fe_em/em_dof_manager.cpp:29-65)
