@@ -88,6 +88,9 @@ struct fetidp_solver {
   Plans plans;
   bool setup_done = false, factored = false, rhs_from_J = false, rhs_set = false;
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;  // side stream of the factorization (mode-3 launches)
+  hipStream_t st3 = nullptr;  // operator extraction beside the coarse factorization
+  hipEvent_t ev_main = nullptr, ev_side = nullptr, ev_extract = nullptr;
   int32_t n_mat = 0;
   double symbolic_ms = 0.0, last_factor_ms = 0.0, upload_ms = 0.0;
   bool device_ready = false;
@@ -156,14 +159,18 @@ struct fetidp_solver {
   }
   template <typename Fn>
   void timed(int slot, Fn&& fn) {
+    timed_on(slot, st, fn);
+  }
+  template <typename Fn>
+  void timed_on(int slot, hipStream_t stream, Fn&& fn) {
     if (!profiling()) {
       fn();
       return;
     }
     hipEvent_t a = prof_event(), b = prof_event();
-    HIPCHECK(hipEventRecord(a, st));
+    HIPCHECK(hipEventRecord(a, stream));
     fn();
-    HIPCHECK(hipEventRecord(b, st));
+    HIPCHECK(hipEventRecord(b, stream));
     prof_pending.push_back({slot, {a, b}});
   }
   void prof_collect() {
@@ -190,6 +197,10 @@ struct fetidp_solver {
     for (hipEvent_t e : guess_ev)
       if (e) (void)hipEventDestroy(e);
     if (hsc) (void)hipHostFree(hsc);
+    for (hipEvent_t e : {ev_main, ev_side, ev_extract})
+      if (e) (void)hipEventDestroy(e);
+    if (st3) (void)hipStreamDestroy(st3);
+    if (st2) (void)hipStreamDestroy(st2);
     if (st) (void)hipStreamDestroy(st);
   }
 
@@ -336,6 +347,14 @@ struct fetidp_solver {
     if (device_ready) return;
     const double t0 = now_ms();
     if (!st) HIPCHECK(hipStreamCreate(&st));
+    if (!st2) {
+      HIPCHECK(hipStreamCreate(&st2));
+      HIPCHECK(hipStreamCreate(&st3));
+      HIPCHECK(hipEventCreateWithFlags(&ev_main, hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&ev_side, hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&ev_extract, hipEventDisableTiming));
+      HIPCHECK(hipEventRecord(ev_side, st2));  // a recorded, complete event to start from
+    }
     xyz.upload(h_xyz_);
     t2n.upload(h_t2n_);
     mat.upload(h_mat_);
@@ -467,12 +486,39 @@ struct fetidp_solver {
   }
 
   // ---------------------------------------------------------------- factor
+  // Launches run in order on st, except mode 3 (column item I = J + 1 plus the
+  // factorization of the next diagonal tile), which runs on st2 beside the
+  // mode-1 launch of the same column J:
+  //   mode 3 of J waits for everything issued on st before it (column J-1,
+  //   diagonal J); the mode-1 launch of J waits for mode 3 of J-1 only;
+  //   every other launch, and the end of the schedule, wait for all of st2.
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& rowbase,
                     const DBuf<int64_t>& rabs, const DBuf<int64_t>& inv_base, T* pool, T* inv,
                     int* failp, float* P) {
     const int slot0 = P ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
+    bool after_side = false;      // the previous launch was a mode-3 launch on st2
     for (const TileLaunch& L : S.launches) {
+      if (L.mode == 3 && profiling()) {
+        // profiled runs keep every launch on st, so event timings are per kernel
+        timed(slot0 + 2, [&] {
+          launch_tile_mode<T>(3, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st);
+        });
+        continue;
+      }
+      if (L.mode == 3) {
+        HIPCHECK(hipEventRecord(ev_main, st));
+        HIPCHECK(hipStreamWaitEvent(st2, ev_main, 0));
+        HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));  // before ev_side moves on to this launch
+        timed_on(slot0 + 2, st2, [&] {
+          launch_tile_mode<T>(3, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st2);
+        });
+        HIPCHECK(hipEventRecord(ev_side, st2));
+        after_side = true;
+        continue;
+      }
+      if (!after_side) HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
+      after_side = false;
       if (L.mode < 0) {
         timed(5, [&] { launch_extract_P<T>(topo.n_sub, plans.max_ndp, devsub.p, pool, rowabs.p, ftile.p, P, st); });
         continue;
@@ -481,6 +527,7 @@ struct fetidp_solver {
         launch_tile_mode<T>(L.mode, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st);
       });
     }
+    HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
   }
 
   template <typename T>
@@ -514,7 +561,16 @@ struct fetidp_solver {
       run_schedule<T>(plans.local_sched, local_items, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, P.p);
     else
       run_schedule<T>(plans.local_sched_ll, local_items_ll, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, P.p);
-    timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st); });
+    // W extraction (st3) and the coarse problem (st, st2) only read the local
+    // factors, so they overlap; st waits for st3 before the end of the factor.
+    if (profiling()) {
+      timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st); });
+    } else {
+      HIPCHECK(hipEventRecord(ev_main, st));
+      HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
+      launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st3);
+      HIPCHECK(hipEventRecord(ev_extract, st3));
+    }
     // coarse problem in FP64
     cpool.zero(st);
     launch_gather_pool<T>(plans.coarse.size(), coarse_target.p, coarse_ptr.p, coarse_srcpos.p,
@@ -522,6 +578,7 @@ struct fetidp_solver {
     cfail.zero(st);
     run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, nullptr);
     launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, coarse_rowabs.p, coarse_ftile.p, sinv.p, st);
+    if (!profiling()) HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
     HIPCHECK(hipStreamSynchronize(st));
     prof_collect();
     std::vector<int> hf(topo.n_sub);
@@ -1153,10 +1210,10 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
     for (int mode = 0; mode < 3; ++mode) {
       double fl = 0.0;
       for (const TileLaunch& L : sch[q]->launches)
-        if (L.mode == mode)
+        if (L.mode == mode || (mode == 2 && L.mode == 3))  // mode 3 is timed with the diagonals
           for (int64_t i = 0; i < L.n; ++i) {
             const TileWork& w = sch[q]->items[L.off + i];
-            fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (mode == 1 ? 1.0 + (w.flags & 1) : 0.0));
+            fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (L.mode == 1 || L.mode == 3 ? 1.0 + (w.flags & 1) : 0.0));
           }
       if (k < cap) out[k++] = fl;
     }

@@ -390,12 +390,35 @@ __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
   __syncthreads();
 }
 
+// Cholesky of the 64x64 tile staged in s (LDS, ld kLd) by wave 0 (registers +
+// readlane) and its inverse into si (LDS); the workgroup then stores L (upper
+// part zeroed) to C and L^-1 to Li, and flags a non-positive pivot in *fail.
+template <typename T>
+__device__ __forceinline__ void factor_diag_tile(T* s, T* si, T* __restrict__ C, T* __restrict__ Li,
+                                                 int* fail, int tid, int wave, int lane) {
+  __shared__ T rdg[64];
+  if (wave == 0) {
+    const int bad = wave_potrf_inv<T>(s, si, rdg, lane);
+    if (bad && lane == 0) atomicOr(fail, 1);
+  }
+  __syncthreads();
+  for (int e = tid; e < kTileElems; e += 256) {
+    const int i = e & 63, j = e >> 6;
+    C[e] = (i >= j) ? s[j * kLd + i] : T(0);
+    Li[e] = si[j * kLd + i];
+  }
+}
+
 // MODE 0: C_IJ -= sum_K L_IK L_JK^T                                (Schur updates)
 // MODE 1: same, then X = C L_JJ^-T with the stored inverse diagonal tile; with
 //         flags & 1 the diagonal tile of row I also takes its share C_II -= X X^T
 //         (column step, I > J; every WG of a launch owns a distinct row I)
 // MODE 2: same (empty K range when the diagonal was updated incrementally), then
 //         Cholesky of the diagonal tile and its inverse
+// MODE 3: MODE 1 for the item I = J + 1, whose share completes C_II, then the
+//         Cholesky and inverse of C_II for the next column step (run on a second
+//         stream beside the MODE 1 launch of the same column; ~256 VGPRs, so it
+//         is kept out of MODE 1)
 // The inverse diagonal tile of column J of matrix m lives at inv + inv_base[m] + J*4096.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict__ work,
@@ -418,12 +441,13 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) acc[x][y] = acc_t{0, 0, 0, 0};
-  {
-    __shared__ T sA[2 * 64 * 16], sB[2 * 64 * 16];
-    panel_mma<T>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
-                 pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, sA, sB, tid,
-                 pb0, qb0, lr, lk);
-  }
+  // LDS: the panel staging buffers are dead after panel_mma; the same space then
+  // holds the inverse tile of a diagonal factorization.
+  constexpr int kStage = 2 * 64 * 16, kPanelLds = 2 * kStage > 64 * kLd ? 2 * kStage : 64 * kLd;
+  __shared__ T lds_panel[kPanelLds];
+  panel_mma<T>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
+               pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, lds_panel,
+               lds_panel + kStage, tid, pb0, qb0, lr, lk);
 
   if constexpr (MODE == 0) {
 #pragma unroll
@@ -447,7 +471,7 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
           s[j * kLd + i] = C[j * 64 + i] - acc[x][y][r];
         }
     __syncthreads();
-    if constexpr (MODE == 1) {
+    if constexpr (MODE == 1 || MODE == 3) {
       // X[i][j] = sum_k C'[i][k] Linv[j][k]:  D2[p=j][q=i], A op = Linv (p,k), B op = C' (q,k)
       const T* Li = inv + inv_base[w.mat] + (int64_t)w.J * kTileElems;
       acc_t a2[2][2];
@@ -467,7 +491,7 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
             C[j * 64 + i] = a2[x][y][r];
             s[j * kLd + i] = a2[x][y][r];
           }
-      if (!(w.flags & 1)) return;  // uniform per workgroup
+      if (MODE == 1 && !(w.flags & 1)) return;  // uniform per workgroup
       __syncthreads();
       // C_II -= X X^T
       acc_t a3[2][2];
@@ -477,30 +501,35 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
         for (int y = 0; y < 2; ++y) a3[x][y] = acc_t{0, 0, 0, 0};
       tile_mma<T, kLd, kLd>(a3, s, s, pb0, qb0, lr, lk);
       T* Cd = pool + ra[w.I] + (int64_t)w.I * kTileElems;
+      if constexpr (MODE == 1) {
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+          for (int y = 0; y < 2; ++y)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-            Cd[j * 64 + i] -= a3[x][y][r];
-          }
+            for (int r = 0; r < 4; ++r) {
+              const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
+              Cd[j * 64 + i] -= a3[x][y][r];
+            }
+      } else {
+        // C_II is complete: stage it in s once every wave has finished reading X
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
+              s[j * kLd + i] = Cd[j * 64 + i] - a3[x][y][r];
+            }
+        __syncthreads();
+        factor_diag_tile<T>(s, lds_panel, Cd, inv + inv_base[w.mat] + (int64_t)w.I * kTileElems,
+                            fail + w.mat, tid, wave, lane);
+      }
     } else {
-      // Cholesky of the 64x64 tile and its inverse by wave 0 (registers + readlane)
-      __shared__ T si[64 * kLd];
-      __shared__ T rdg[64];
-      if (wave == 0) {
-        const int bad = wave_potrf_inv<T>(s, si, rdg, lane);
-        if (bad && lane == 0) atomicOr(fail + w.mat, 1);
-      }
-      __syncthreads();
-      T* Li = inv + inv_base[w.mat] + (int64_t)w.J * kTileElems;
-      for (int e = tid; e < kTileElems; e += 256) {
-        const int i = e & 63, j = e >> 6;
-        C[e] = (i >= j) ? s[j * kLd + i] : T(0);
-        Li[e] = si[j * kLd + i];
-      }
+      factor_diag_tile<T>(s, lds_panel, C, inv + inv_base[w.mat] + (int64_t)w.J * kTileElems,
+                          fail + w.mat, tid, wave, lane);
     }
   }
 }
@@ -514,8 +543,10 @@ void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* row
     hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
   else if (mode == 1)
     hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
-  else
+  else if (mode == 2)
     hipLaunchKernelGGL((k_tile_update<T, 2>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+  else
+    hipLaunchKernelGGL((k_tile_update<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
 }
 template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
                                        const int64_t*, double*, double*, int*, hipStream_t);

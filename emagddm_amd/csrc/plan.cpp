@@ -45,9 +45,9 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
   TileLaunch L{mode, int64_t(S.items.size()), int64_t(list.size())};
   S.items.insert(S.items.end(), list.begin(), list.end());
   S.launches.push_back(L);
-  // mode 1 adds the TRSM product (and the diagonal-tile share when flagged)
+  // modes 1 and 3 add the TRSM product (and the diagonal-tile share when flagged)
   for (const TileWork& w : list) {
-    const double extra = mode == 1 ? 1.0 + (w.flags & 1) : 0.0;
+    const double extra = (mode == 1 || mode == 3) ? 1.0 + (w.flags & 1) : 0.0;
     S.flops += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + extra);
   }
 }
@@ -59,14 +59,23 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
 // carry an empty K range and Schur lists skip the diagonal -- short critical path.
 // Left-looking diagonal (inc = false): one rounding per diagonal entry and Schur
 // step, used for FP32 factors where repeated rounding of the diagonal matters.
+// With incremental diagonals, the item (j + 1, j) of column j gives C_{j+1,j+1}
+// its last share, so it runs as a mode-3 launch that also factors that tile,
+// on a second stream beside the mode-1 launch of the other rows (the diagonal
+// launch of column j + 1 disappears).  Per column: [mode 2][mode 3][mode 1].
 TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc, const int32_t* ft) {
   TileSchedule S;
   auto f = [&](int I) { return ft ? ft[I] : 0; };
   for (int j = 0; j < Tp; ++j) {
-    add_tiles(S, 2, {TileWork{0, j, j, inc ? 0 : f(j), inc ? 0 : j, 0}});
-    std::vector<TileWork> below;
+    if (!(inc && j > 0 && f(j) <= j - 1))  // else factored by column j-1's mode-3 item
+      add_tiles(S, 2, {TileWork{0, j, j, inc ? 0 : f(j), inc ? 0 : j, 0}});
+    std::vector<TileWork> next, below;
     for (int I = j + 1; I < Ta; ++I)
-      if (f(I) <= j) below.push_back(TileWork{0, I, j, std::max(f(I), f(j)), j, inc ? 1 : 0});
+      if (f(I) <= j) {
+        const TileWork w{0, I, j, std::max(f(I), f(j)), j, inc ? 1 : 0};
+        (inc && I == j + 1 && I < Tp ? next : below).push_back(w);
+      }
+    add_tiles(S, 3, next);
     add_tiles(S, 1, below);
   }
   std::vector<TileWork> schur;
@@ -337,16 +346,21 @@ void build_plans(const Topology& T, Plans& P) {
     }
     // phase 1: interior columns; rows of D' tiles are structurally zero here
     for (int j = 0; j < maxTi; ++j) {
-      std::vector<TileWork> diag, below;
+      std::vector<TileWork> diag, next, below;
       for (int64_t s = 0; s < ns; ++s) {
         const SubLayout& L = T.sub[s];
         if (j >= L.Ti) continue;
         const int32_t* ft = &P.ftile[P.ftile_off[s]];
-        diag.push_back(TileWork{int32_t(s), j, j, inc ? 0 : ft[j], inc ? 0 : j, 0});
+        if (!(inc && j > 0 && ft[j] <= j - 1))  // else factored by column j-1's mode-3 item
+          diag.push_back(TileWork{int32_t(s), j, j, inc ? 0 : ft[j], inc ? 0 : j, 0});
         for (int I = j + 1; I < L.Tc; ++I)
-          if (ft[I] <= j) below.push_back(TileWork{int32_t(s), I, j, std::max(ft[I], ft[j]), j, fl});
+          if (ft[I] <= j) {
+            const TileWork w{int32_t(s), I, j, std::max(ft[I], ft[j]), j, fl};
+            (inc && I == j + 1 && I < L.Ti ? next : below).push_back(w);
+          }
       }
       add_tiles(S, 2, diag);
+      add_tiles(S, 3, next);
       add_tiles(S, 1, below);
     }
     std::vector<TileWork> schur1;
@@ -364,18 +378,24 @@ void build_plans(const Topology& T, Plans& P) {
     S.launches.push_back(TileLaunch{-1, 0, 0});  // snapshot S_DD for the preconditioner
     // phase 2: dual columns
     for (int j = 0; j < maxTd; ++j) {
-      std::vector<TileWork> diag, below;
+      std::vector<TileWork> diag, next, below;
       for (int64_t s = 0; s < ns; ++s) {
         const SubLayout& L = T.sub[s];
         const int col = L.Ti + j;
         if (col >= L.Tr) continue;
         const int32_t* ft = &P.ftile[P.ftile_off[s]];
-        diag.push_back(TileWork{int32_t(s), col, col, inc ? 0 : std::max(L.Ti, ft[col]), inc ? 0 : col, 0});
+        // the first dual diagonal keeps a launch of its own: the S_DD snapshot must
+        // see it unfactored
+        if (!(inc && col > L.Ti && ft[col] <= col - 1))
+          diag.push_back(TileWork{int32_t(s), col, col, inc ? 0 : std::max(L.Ti, ft[col]), inc ? 0 : col, 0});
         for (int I = col + 1; I < L.T; ++I)
-          if (ft[I] <= col)
-            below.push_back(TileWork{int32_t(s), I, col, std::max(L.Ti, std::max(ft[I], ft[col])), col, fl});
+          if (ft[I] <= col) {
+            const TileWork w{int32_t(s), I, col, std::max(L.Ti, std::max(ft[I], ft[col])), col, fl};
+            (inc && I == col + 1 && I < L.Tr ? next : below).push_back(w);
+          }
       }
       add_tiles(S, 2, diag);
+      add_tiles(S, 3, next);
       add_tiles(S, 1, below);
     }
     std::vector<TileWork> schur2;
@@ -408,6 +428,12 @@ void build_plans(const Topology& T, Plans& P) {
       check(w.J <= w.I && w.I < L.T && w.k1 <= w.J && w.J >= ft[w.I], "local tile");
       check(w.k0 >= w.k1 || (w.k0 >= ft[w.I] && w.k0 >= ft[w.J]), "local tile panel");
     }
+  // a mode-3 item completes and factors the diagonal tile of the next column
+  for (const TileSchedule* S : {&P.local_sched, &P.local_sched_ll, &P.coarse_sched})
+    for (const TileLaunch& L : S->launches)
+      if (L.mode == 3)
+        for (int64_t i = L.off; i < L.off + L.n; ++i)
+          check(S->items[i].I == S->items[i].J + 1 && (S->items[i].flags & 1), "mode-3 item");
   {
     const int Ta = int(2 * P.coarse_ncp / kTile);
     for (const TileWork& w : P.coarse_sched.items) {
