@@ -113,6 +113,8 @@ struct fetidp_solver {
   DBuf<int64_t> rowabs, coarse_rowabs;
   DBuf<int32_t> ftile, coarse_ftile;
   DBuf<DevSub> devsub;
+  DBuf<SymTile> sym_tiles;
+  DBuf<double> sympart;
   DBuf<int32_t> vec_dof;
   DBuf<double> vec_w;
   DBuf<int32_t> d_lam;
@@ -396,6 +398,8 @@ struct fetidp_solver {
       coarse_inv_base.upload(std::vector<int64_t>{0});
     }
     devsub.upload(plans.devsub);
+    sym_tiles.upload(plans.sym_tiles);
+    sympart.alloc(size_t(std::max<int64_t>(topo.sp_elems, 1)));
     vec_dof.upload(plans.vec_dof);
     vec_w.upload(plans.vec_w);
     d_lam.upload(topo.d_lam);
@@ -564,11 +568,11 @@ struct fetidp_solver {
     // W extraction (st3) and the coarse problem (st, st2) only read the local
     // factors, so they overlap; st waits for st3 before the end of the factor.
     if (profiling()) {
-      timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st); });
+      timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st); });
     } else {
       HIPCHECK(hipEventRecord(ev_main, st));
       HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
-      launch_extract_W<T>(topo.n_sub, plans.max_wrows, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st3);
+      launch_extract_W<T>(topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st3);
       HIPCHECK(hipEventRecord(ev_extract, st3));
     }
     // coarse problem in FP64
@@ -632,7 +636,7 @@ struct fetidp_solver {
 
   template <typename T>
   void apply_F(const DBuf<T>& W, const double* pin, double* qout) {
-    timed(9, [&] { launch_gemv_dual<T>(topo.n_sub, plans.max_wrows, devsub.p, W.p, 0, d_lam.p, d_sgn.p, pin, ybuf.p, st, plans.max_ndp); });
+    timed(9, [&] { apply_W<T>(W, pin); });
     timed(10, [&] { apply_F_rest<T>(W, qout); });
   }
   template <typename T>
@@ -642,8 +646,18 @@ struct fetidp_solver {
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
+  // ybuf = W x with x = B^T p gathered per subdomain: Z from its lower tiles, then Phi^T rows
+  template <typename T>
+  void apply_W(const DBuf<T>& W, const double* pin) {
+    launch_symv<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, W.p, 0,
+                   d_lam.p, d_sgn.p, pin, sympart.p, ybuf.p, st);
+    launch_gemv_phi<T>(topo.n_sub, plans.max_ncp, devsub.p, W.p, d_lam.p, d_sgn.p, pin, ybuf.p, st, plans.max_ndp);
+  }
   void apply_M(const double* rin, double* zout) {
-    timed(11, [&] { launch_gemv_dual<float>(topo.n_sub, plans.max_ndp, devsub.p, Pf.p, 1, d_lam.p, d_sgnD.p, rin, dbuf.p, st, plans.max_ndp); });
+    timed(11, [&] {
+      launch_symv<float>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, Pf.p, 1,
+                         d_lam.p, d_sgnD.p, rin, sympart.p, dbuf.p, st);
+    });
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgnD.p, dbuf.p, zout, st);
   }
 
@@ -665,7 +679,7 @@ struct fetidp_solver {
   // recover the global free-dof solution from lambda into xout
   template <typename T>
   void recover(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, double* xout) {
-    launch_gemv_dual<T>(topo.n_sub, plans.max_wrows, devsub.p, W.p, 0, d_lam.p, d_sgn.p, lam.p, ybuf.p, st, plans.max_ndp);
+    apply_W<T>(W, lam.p);
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
     launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, gc.p, uc.p, st);
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);

@@ -42,7 +42,7 @@ template <typename T>
 void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, const int64_t* rowabs,
                       const int32_t* ftile, float* P, hipStream_t st);
 template <typename T>
-void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool,
+void launch_extract_W(int ns, int max_ncp, int max_ndp, const DevSub* subs, const T* pool,
                       const int64_t* rowabs, const int32_t* ftile, T* W, hipStream_t st);
 void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, const int64_t* rowabs,
                          const int32_t* ftile, double* sinv, hipStream_t st);
@@ -64,10 +64,15 @@ void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void*
                     double* half, hipStream_t st);
 
 // y_s = M_s x_s with x_s[k] = scale(slot) * v[lam[slot]], rows_s = ndp (+ ncp if with_c)
+// Phi^T rows of W times the gathered dual vector (out at y_off + ndp)
 template <typename T>
-void launch_gemv_dual(int ns, int max_rows, const DevSub* subs, const T* M, int which,
-                      const int32_t* d_lam, const double* d_scale, const double* v, double* y,
-                      hipStream_t st, int max_ndp);
+void launch_gemv_phi(int ns, int max_ncp, const DevSub* subs, const T* W, const int32_t* d_lam,
+                     const double* d_scale, const double* v, double* y, hipStream_t st, int max_ndp);
+// symmetric operator (which 0: Z of W, out at y_off; which 1: P, out at dvec_off)
+template <typename T>
+void launch_symv(int64_t ntiles, const SymTile* tiles, int ns, int max_Td, const DevSub* subs,
+                 const T* M, int which, const int32_t* d_lam, const double* d_scale,
+                 const double* v, double* part, double* out, hipStream_t st);
 // out[slot] = alpha*y[...] + beta*(Phi u)[k] + gamma*vec[o_a + k]
 template <typename T>
 void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* y,

@@ -566,6 +566,25 @@ __device__ __forceinline__ T pool_sym(const T* P, const int64_t* ra, const int32
   return P[ra[hi >> 6] + (int64_t)(lo >> 6) * kTileElems + (lo & 63) * 64 + (hi & 63)];
 }
 
+// elements of a lower-tile symmetric store of order n (device copy of sym_tile_elems)
+__device__ __forceinline__ int64_t sym_elems(int64_t n) {
+  const int64_t t = n >> 6;
+  return t * (t + 1) / 2 * kTileElems;
+}
+
+// (row, column) of element e of a lower-tile symmetric store (tile (I, J <= I) at
+// (I(I+1)/2 + J) * 4096, row-major in the tile)
+__device__ __forceinline__ void sym_tile_rc(int64_t e, int& row, int& col) {
+  const int64_t t = e >> 12;
+  int I = int((sqrt(8.0 * double(t) + 1.0) - 1.0) * 0.5);
+  while (int64_t(I) * (I + 1) / 2 > t) --I;
+  while (int64_t(I + 1) * (I + 2) / 2 <= t) ++I;
+  const int J = int(t - int64_t(I) * (I + 1) / 2);
+  const int w = int(e & 4095);
+  row = I * 64 + (w >> 6);
+  col = J * 64 + (w & 63);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_extract_P(const DevSub* __restrict__ subs,
                                                    const T* __restrict__ pool,
@@ -574,8 +593,9 @@ __global__ __launch_bounds__(256) void k_extract_P(const DevSub* __restrict__ su
                                                    float* __restrict__ P) {
   const DevSub S = subs[blockIdx.y];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)S.ndp * S.ndp) return;
-  const int a = int(e / S.ndp), b = int(e % S.ndp);
+  if (e >= sym_elems(S.ndp)) return;
+  int a, b;
+  sym_tile_rc(e, a, b);
   P[S.p_off + e] = (float)pool_sym(pool, rowabs + S.ft_off, ftile + S.ft_off, S.o_d + a, S.o_d + b);
 }
 
@@ -585,7 +605,7 @@ __global__ __launch_bounds__(256) void k_extract_P(const DevSub* __restrict__ su
 template <typename T>
 void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, const int64_t* rowabs,
                       const int32_t* ftile, float* P, hipStream_t st) {
-  const int64_t n = (int64_t)max_ndp * max_ndp;
+  const int64_t n = sym_tile_elems(max_ndp);
   if (n <= 0 || ns <= 0) return;
   hipLaunchKernelGGL(k_extract_P<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, rowabs,
                      ftile, P);
@@ -595,7 +615,7 @@ template void launch_extract_P<double>(int, int, const DevSub*, const double*, c
 template void launch_extract_P<float>(int, int, const DevSub*, const float*, const int64_t*,
                                       const int32_t*, float*, hipStream_t);
 
-// W = [Z; Phi^T] from the trailing block: Z = -A(D',D'), Phi = -A(D',c)
+// W = Z (lower tiles) then Phi^T from the trailing block: Z = -A(D',D'), Phi = -A(D',c)
 template <typename T>
 __global__ __launch_bounds__(256) void k_extract_W(const DevSub* __restrict__ subs,
                                                    const T* __restrict__ pool,
@@ -604,23 +624,27 @@ __global__ __launch_bounds__(256) void k_extract_W(const DevSub* __restrict__ su
                                                    T* __restrict__ W) {
   const DevSub S = subs[blockIdx.y];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int rows = S.ndp + S.ncp;
-  if (e >= (int64_t)rows * S.ndp) return;
-  const int row = int(e / S.ndp), k = int(e % S.ndp);
+  const int64_t zt = sym_elems(S.ndp);
+  if (e >= zt + (int64_t)S.ncp * S.ndp) return;
   const int64_t* ra = rowabs + S.ft_off;
   const int32_t* ft = ftile + S.ft_off;
   T v;
-  if (row < S.ndp)
-    v = -pool_sym(pool, ra, ft, S.o_a + row, S.o_a + k);
-  else
-    v = -pool_sym(pool, ra, ft, S.o_a + k, S.o_c + (row - S.ndp));
+  if (e < zt) {  // Z = -A(D', D') as lower tiles
+    int a, b;
+    sym_tile_rc(e, a, b);
+    v = -pool_sym(pool, ra, ft, S.o_a + a, S.o_a + b);
+  } else {       // Phi^T = -A(c, D') rows, row-major ncp x ndp
+    const int64_t f = e - zt;
+    const int row = int(f / S.ndp), k = int(f % S.ndp);
+    v = -pool_sym(pool, ra, ft, S.o_a + k, S.o_c + row);
+  }
   W[S.w_off + e] = v;
 }
 
 template <typename T>
-void launch_extract_W(int ns, int max_rows, int max_ndp, const DevSub* subs, const T* pool,
+void launch_extract_W(int ns, int max_ncp, int max_ndp, const DevSub* subs, const T* pool,
                       const int64_t* rowabs, const int32_t* ftile, T* W, hipStream_t st) {
-  const int64_t n = (int64_t)max_rows * max_ndp;
+  const int64_t n = sym_tile_elems(max_ndp) + (int64_t)max_ncp * max_ndp;
   if (n <= 0 || ns <= 0) return;
   hipLaunchKernelGGL(k_extract_W<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, rowabs,
                      ftile, W);
@@ -855,20 +879,21 @@ template <> struct Seg<float> {
   }
 };
 
-// which = 0: M = W_s ((ndp+ncp) x ndp), out at y_off; which = 1: M = P_s (ndp x ndp), out at dvec_off.
-// Rows are ndp-strided with ndp a multiple of 64 and row bases 256-byte aligned, so every lane
-// reads whole 16-byte segments; columns in [nd, ndp) meet zeros in the gathered vector.
-// Each wave walks two rows at once so four segment loads are in flight per lane.
+// Phi^T rows of W (ncp x ndp, row-major after the Z tiles) times the gathered
+// x = d_scale * v[d_lam] of each subdomain, out at y_off + ndp.  Rows are
+// ndp-strided with ndp a multiple of 64 and row bases 256-byte aligned, so every
+// lane reads whole 16-byte segments; columns in [nd, ndp) meet zeros in x.  Each
+// wave walks two rows at once so four segment loads are in flight per lane.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemv_dual(const DevSub* __restrict__ subs,
-                                                   const T* __restrict__ M, int which,
-                                                   const int32_t* __restrict__ d_lam,
-                                                   const double* __restrict__ d_scale,
-                                                   const double* __restrict__ v, double* __restrict__ y) {
+__global__ __launch_bounds__(256) void k_gemv_phi(const DevSub* __restrict__ subs,
+                                                  const T* __restrict__ W,
+                                                  const int32_t* __restrict__ d_lam,
+                                                  const double* __restrict__ d_scale,
+                                                  const double* __restrict__ v, double* __restrict__ y) {
   constexpr int V = Seg<T>::V;
   extern __shared__ double xs[];
   const DevSub S = subs[blockIdx.y];
-  const int rows = which == 0 ? S.ndp + S.ncp : S.ndp;
+  const int rows = S.ncp;
   const int r0 = blockIdx.x * kGemvRows;
   if (r0 >= rows) return;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -878,9 +903,9 @@ __global__ __launch_bounds__(256) void k_gemv_dual(const DevSub* __restrict__ su
     xs[k] = l >= 0 ? d_scale[slot] * v[l] : 0.0;
   }
   __syncthreads();
-  const T* base = M + (which == 0 ? S.w_off : S.p_off);
+  const T* base = W + S.w_off + sym_elems(S.ndp);
   const int kend = (S.nd + V - 1) / V * V;  // <= ndp
-  double* out = y + (which == 0 ? S.y_off : S.dvec_off);
+  double* out = y + S.y_off + S.ndp;
   for (int rr = 2 * wv; rr < kGemvRows; rr += 8) {
     const int row0 = r0 + rr;
     if (row0 >= rows) break;
@@ -914,17 +939,101 @@ __global__ __launch_bounds__(256) void k_gemv_dual(const DevSub* __restrict__ su
 }
 
 template <typename T>
-void launch_gemv_dual(int ns, int max_rows, const DevSub* subs, const T* M, int which,
-                      const int32_t* d_lam, const double* d_scale, const double* v, double* y,
-                      hipStream_t st, int max_ndp) {
-  if (ns <= 0 || max_rows <= 0) return;
-  hipLaunchKernelGGL(k_gemv_dual<T>, dim3((max_rows + kGemvRows - 1) / kGemvRows, ns), dim3(256),
-                     max_ndp * sizeof(double), st, subs, M, which, d_lam, d_scale, v, y);
+void launch_gemv_phi(int ns, int max_ncp, const DevSub* subs, const T* W, const int32_t* d_lam,
+                     const double* d_scale, const double* v, double* y, hipStream_t st, int max_ndp) {
+  if (ns <= 0 || max_ncp <= 0) return;
+  hipLaunchKernelGGL(k_gemv_phi<T>, dim3((max_ncp + kGemvRows - 1) / kGemvRows, ns), dim3(256),
+                     max_ndp * sizeof(double), st, subs, W, d_lam, d_scale, v, y);
 }
-template void launch_gemv_dual<double>(int, int, const DevSub*, const double*, int, const int32_t*,
-                                       const double*, const double*, double*, hipStream_t, int);
-template void launch_gemv_dual<float>(int, int, const DevSub*, const float*, int, const int32_t*,
+template void launch_gemv_phi<double>(int, int, const DevSub*, const double*, const int32_t*,
                                       const double*, const double*, double*, hipStream_t, int);
+template void launch_gemv_phi<float>(int, int, const DevSub*, const float*, const int32_t*,
+                                     const double*, const double*, double*, hipStream_t, int);
+
+// Symmetric dual operator (Z of W, or P) times the gathered x, from its lower
+// tiles: one work-group per tile (I, J).  It reads the tile once and writes two
+// 64-vectors of partial sums, M_IJ x_J at slot (I, J) and (for I > J) M_IJ^T x_I
+// at slot (J, I) of the subdomain's (ndp/64)^2 partial array; k_symv_reduce then
+// sums slots (I, 0 .. Td-1) in order.  Wave w holds tile rows 16w .. 16w+15, lane
+// = column, so each tile row is one 512-byte (FP64) coalesced load.
+template <typename T>
+__global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ tiles,
+                                                   const DevSub* __restrict__ subs,
+                                                   const T* __restrict__ M, int which,
+                                                   const int32_t* __restrict__ d_lam,
+                                                   const double* __restrict__ d_scale,
+                                                   const double* __restrict__ v,
+                                                   double* __restrict__ part) {
+  __shared__ double xJ[64], xI[64];
+  __shared__ double prod[64 * 65];
+  __shared__ double colp[4][64];
+  const SymTile tl = tiles[blockIdx.x];
+  const DevSub S = subs[tl.s];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid < 128) {
+    const int blk = (tid >> 6) ? tl.I : tl.J;
+    const int64_t slot = S.dvec_off + blk * 64 + lane;
+    const int32_t l = d_lam[slot];
+    const double xv = l >= 0 ? d_scale[slot] * v[l] : 0.0;
+    if (tid >> 6) xI[lane] = xv;
+    else xJ[lane] = xv;
+  }
+  __syncthreads();
+  const T* A = M + (which == 0 ? S.w_off : S.p_off) + ((int64_t)tl.I * (tl.I + 1) / 2 + tl.J) * kTileElems;
+  const double xc = xJ[lane];
+  double cs = 0.0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int r = 16 * wv + q;
+    const double a = (double)A[r * 64 + lane];
+    prod[r * 65 + lane] = a * xc;
+    cs += a * xI[r];
+  }
+  const bool off = tl.I != tl.J;  // uniform per work-group
+  if (off) colp[wv][lane] = cs;
+  __syncthreads();
+  const int Td = S.ndp >> 6;
+  const int r = tid >> 2, qq = tid & 3;
+  double rs = 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) rs += prod[r * 65 + 16 * qq + c];
+  rs += __shfl_xor(rs, 1);
+  rs += __shfl_xor(rs, 2);
+  if (qq == 0) part[S.sp_off + ((int64_t)tl.I * Td + tl.J) * 64 + r] = rs;
+  if (off && tid < 64)
+    part[S.sp_off + ((int64_t)tl.J * Td + tl.I) * 64 + tid] =
+        (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
+}
+
+// out[I*64 + r] = sum_{J < Td} part[(I, J)][r], at y_off (which 0) or dvec_off (which 1)
+__global__ __launch_bounds__(64) void k_symv_reduce(const DevSub* __restrict__ subs,
+                                                    const double* __restrict__ part, int which,
+                                                    double* __restrict__ out) {
+  const DevSub S = subs[blockIdx.y];
+  const int Td = S.ndp >> 6, I = blockIdx.x;
+  if (I >= Td) return;
+  const int r = threadIdx.x;
+  const double* p = part + S.sp_off + (int64_t)I * Td * 64 + r;
+  double sum = 0.0;
+  for (int J = 0; J < Td; ++J) sum += p[(int64_t)J * 64];
+  out[(which == 0 ? S.y_off : S.dvec_off) + I * 64 + r] = sum;
+}
+
+template <typename T>
+void launch_symv(int64_t ntiles, const SymTile* tiles, int ns, int max_Td, const DevSub* subs,
+                 const T* M, int which, const int32_t* d_lam, const double* d_scale,
+                 const double* v, double* part, double* out, hipStream_t st) {
+  if (ntiles <= 0 || ns <= 0) return;
+  hipLaunchKernelGGL(k_symv_tile<T>, dim3(ntiles), dim3(256), 0, st, tiles, subs, M, which, d_lam,
+                     d_scale, v, part);
+  hipLaunchKernelGGL(k_symv_reduce, dim3(max_Td, ns), dim3(64), 0, st, subs, part, which, out);
+}
+template void launch_symv<double>(int64_t, const SymTile*, int, int, const DevSub*, const double*,
+                                  int, const int32_t*, const double*, const double*, double*,
+                                  double*, hipStream_t);
+template void launch_symv<float>(int64_t, const SymTile*, int, int, const DevSub*, const float*, int,
+                                 const int32_t*, const double*, const double*, double*, double*,
+                                 hipStream_t);
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__ subs,
@@ -941,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__
   double r = 0.0;
   if (k < S.nd) {
     double s = 0.0;
-    const T* ph = W + S.w_off + (int64_t)S.ndp * S.ndp;  // Phi^T rows
+    const T* ph = W + S.w_off + sym_elems(S.ndp);  // Phi^T rows
     for (int c = 0; c < S.nc; ++c) s += (double)ph[(int64_t)c * S.ndp + k] * uc[c_loc[S.cvec_off + c]];
     r = beta * s;
     if (alpha != 0.0) r += alpha * y[S.y_off + k];

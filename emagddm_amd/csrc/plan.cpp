@@ -473,8 +473,12 @@ void build_plans(const Topology& T, Plans& P) {
     D.pool_off = L.pool_off; D.vec_off = L.vec_off; D.dvec_off = L.dvec_off;
     D.w_off = L.w_off; D.p_off = L.p_off; D.y_off = L.y_off; D.cvec_off = L.cvec_off;
     D.ft_off = P.ftile_off[s];
+    D.sp_off = L.sp_off;
+    for (int I = 0; I < L.ndp / kTile; ++I)
+      for (int J = 0; J <= I; ++J) P.sym_tiles.push_back(SymTile{int32_t(s), I, J});
     D.pool_off = P.pool_base[s];
     P.max_ndp = std::max(P.max_ndp, L.ndp);
+    P.max_Td = std::max(P.max_Td, L.ndp / kTile);
     P.max_ncp = std::max(P.max_ncp, L.ncp);
     P.max_m = std::max(P.max_m, L.m);
     P.max_wrows = std::max(P.max_wrows, L.ndp + L.ncp);

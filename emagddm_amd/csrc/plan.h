@@ -68,6 +68,8 @@ struct Plans {
   std::vector<int32_t> coarse_ftile;
   std::vector<int64_t> coarse_rowabs;
   int64_t inv_elems = 0, coarse_inv_elems = 0;
+  std::vector<SymTile> sym_tiles;  // every lower tile of every subdomain's Z / P
+  int max_Td = 0;                  // max ndp / 64
 };
 
 // Packed index of (a, b), a <= b, in the 21-entry upper triangle of a 6x6.

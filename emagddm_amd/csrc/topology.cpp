@@ -193,7 +193,7 @@ std::string build_topology(const MeshInput& in, Topology& T) {
   }
 
   // ---- padded layouts and offsets ------------------------------------------
-  int64_t pool = 0, vec = 0, dvec = 0, wel = 0, pel = 0, yel = 0, cvec = 0;
+  int64_t pool = 0, vec = 0, dvec = 0, wel = 0, pel = 0, yel = 0, cvec = 0, spel = 0;
   for (int64_t s = 0; s < ns; ++s) {
     SubLayout& L = T.sub[s];
     L.nip = int32_t(round_up(L.ni, kTile));
@@ -214,9 +214,11 @@ std::string build_topology(const MeshInput& in, Topology& T) {
     L.dvec_off = dvec;
     dvec += L.ndp;
     L.w_off = wel;
-    wel += int64_t(L.ndp + L.ncp) * L.ndp;
+    wel += sym_tile_elems(L.ndp) + int64_t(L.ncp) * L.ndp;
     L.p_off = pel;
-    pel += int64_t(L.ndp) * L.ndp;
+    pel += sym_tile_elems(L.ndp);
+    L.sp_off = spel;
+    spel += int64_t(L.ndp / kTile) * (L.ndp / kTile) * kTile;
     L.y_off = yel;
     yel += L.ndp + L.ncp;
     L.cvec_off = cvec;
@@ -227,6 +229,7 @@ std::string build_topology(const MeshInput& in, Topology& T) {
   T.dvec_elems = dvec;
   T.w_elems = wel;
   T.p_elems = pel;
+  T.sp_elems = spel;
   T.y_elems = yel;
   T.cvec_elems = cvec;
 

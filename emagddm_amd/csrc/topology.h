@@ -28,9 +28,23 @@ struct DevSub {
   int32_t nd, nc, ndp, ncp, o_d, o_c, o_a, m, T, Ti, Tr, Tc;
   int64_t pool_off, vec_off, dvec_off, w_off, p_off, y_off, cvec_off;
   int64_t ft_off;  // offset of this subdomain's envelope (first nonzero tile per tile row)
+  int64_t sp_off;  // offset of its (ndp/64)^2 partial 64-vectors of the symmetric products
 };
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// One lower tile (I, J) of subdomain s's symmetric dual operators (Z and P).
+struct SymTile {
+  int32_t s, I, J;
+};
+
+// Elements of a symmetric n x n operator (n a multiple of 64) stored as its lower
+// 64x64 tiles: tile (I, J <= I) at (I(I+1)/2 + J) * 4096, row-major inside the tile,
+// diagonal tiles full.
+inline int64_t sym_tile_elems(int64_t n) {
+  const int64_t t = n / kTile;
+  return t * (t + 1) / 2 * kTileElems;
+}
 
 // Local padded layout of one subdomain's augmented matrix:
 //   [ i (interior) | D (dual) | c (primal) | D' (copy of D, identity-coupled) ]
@@ -44,8 +58,9 @@ struct SubLayout {
   int64_t pool_off = 0;                // element offset of tile (0,0) in the factor pool
   int64_t vec_off = 0;                 // offset of the length-m local vector
   int64_t dvec_off = 0;                // offset into the concatenated D space (ndp each)
-  int64_t w_off = 0;                   // offset of W = [Z; Phi^T] ((ndp+ncp) x ndp)
-  int64_t p_off = 0;                   // offset of P = S_DD (ndp x ndp)
+  int64_t w_off = 0;                   // offset of W: Z (ndp x ndp, lower tiles) then Phi^T (ncp x ndp)
+  int64_t p_off = 0;                   // offset of P = S_DD (ndp x ndp, lower tiles)
+  int64_t sp_off = 0;                  // offset of the symmetric-product partials ((ndp/64)^2 x 64)
   int64_t y_off = 0;                   // offset into the W-output space ((ndp+ncp) each)
   int64_t cvec_off = 0;                // offset into the concatenated local primal space (ncp each)
 };
@@ -71,7 +86,7 @@ struct Topology {
   std::vector<int64_t> sub_tet_ptr;   // CSR: tets of each subdomain (ascending)
   std::vector<int32_t> sub_tet;
   int64_t pool_elems = 0, vec_elems = 0, dvec_elems = 0, w_elems = 0, p_elems = 0,
-          y_elems = 0, cvec_elems = 0;
+          y_elems = 0, cvec_elems = 0, sp_elems = 0;
 
   // dual maps
   std::vector<int32_t> d_lam;      // (dvec_elems) lambda index per D slot, -1 on padding
