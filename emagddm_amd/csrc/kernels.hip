@@ -460,7 +460,11 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
           C[j * 64 + i] -= acc[x][y][r];
         }
   } else {
-    __shared__ T s[64 * kLd];
+    // staging tile: in mode 1 it reuses the (dead) panel buffers, so the kernel
+    // needs 33 KB of LDS and three work-groups fit a CU; modes 2 and 3 keep the
+    // panel space for the inverse tile and need their own
+    __shared__ T s_own[MODE >= 2 ? 64 * kLd : 1];
+    T* s = MODE >= 2 ? s_own : lds_panel;
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
