@@ -421,12 +421,12 @@ __device__ __forceinline__ void factor_diag_tile(T* s, T* si, T* __restrict__ C,
 //         is kept out of MODE 1)
 // The inverse diagonal tile of column J of matrix m lives at inv + inv_base[m] + J*4096.
 template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict__ work,
-                                                     const int64_t* __restrict__ rowbase,
-                                                     const int64_t* __restrict__ rowabs,
-                                                     const int64_t* __restrict__ inv_base,
-                                                     T* __restrict__ pool, T* __restrict__ inv,
-                                                     int* __restrict__ fail) {
+__device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
+                                            const int64_t* __restrict__ rowbase,
+                                            const int64_t* __restrict__ rowabs,
+                                            const int64_t* __restrict__ inv_base,
+                                            T* __restrict__ pool, T* __restrict__ inv,
+                                            int* __restrict__ fail) {
   using acc_t = typename Mfma<T>::acc_t;
   const TileWork w = work[blockIdx.x];
   // envelope-packed storage: tile (I, J) at ra[I] + J * 4096
@@ -538,6 +538,27 @@ __global__ __launch_bounds__(256) void k_tile_update(const TileWork* __restrict_
   }
 }
 
+// Modes 0 and 1 are the throughput kernels: capped at 128 registers per lane so four
+// waves share each SIMD.  Modes 2 and 3 hold a 64x64 tile in one wave's registers.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256, 4) void k_tile_update(const TileWork* __restrict__ work,
+                                                        const int64_t* __restrict__ rowbase,
+                                                        const int64_t* __restrict__ rowabs,
+                                                        const int64_t* __restrict__ inv_base,
+                                                        T* __restrict__ pool, T* __restrict__ inv,
+                                                        int* __restrict__ fail) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail);
+}
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_tile_factor(const TileWork* __restrict__ work,
+                                                     const int64_t* __restrict__ rowbase,
+                                                     const int64_t* __restrict__ rowabs,
+                                                     const int64_t* __restrict__ inv_base,
+                                                     T* __restrict__ pool, T* __restrict__ inv,
+                                                     int* __restrict__ fail) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail);
+}
+
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
                       const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
@@ -548,9 +569,9 @@ void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* row
   else if (mode == 1)
     hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_tile_update<T, 2>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_factor<T, 2>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
   else
-    hipLaunchKernelGGL((k_tile_update<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_factor<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
 }
 template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
                                        const int64_t*, double*, double*, int*, hipStream_t);
