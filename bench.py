@@ -32,6 +32,9 @@ WORKLOAD_TEXT = {
     "cube16": "16^3-hex Kuhn cube (24,576 tets, 26,416 free edges), 2x2x2 subdomains, "
               "nu=1, kappa=1e-2, 2 Gaussian current loops, PEC",
     "cube64c": "64^3-hex Kuhn cube, 8x8x8 subdomains, log-uniform nu/kappa contrast + sphere",
+    "cube32l": "32^3-hex Kuhn cube, 4x4x4 subdomains, nu log-uniform per subdomain, 1-4 loops",
+    "cube96j": "96^3-hex Kuhn cube (5,308,416 tets), 12x12x12 jittered subdomains 6-10 hexes "
+               "wide, log-uniform nu/kappa contrast + sphere",
 }
 
 
@@ -45,18 +48,20 @@ def parse():
                     help="comma list of fp64 | fp32, each optionally +learned (lambda0 network)")
     ap.add_argument("--cpu-sample-iters", type=int, default=600)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--max-iterations", type=int, default=1000)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="copy this many MiB device-to-device once before the variants, so a "
                          "counter profile sees a copy of known size")
     return ap.parse_args()
 
 
-def run_variant(prob, variant, steps, warmup):
+def run_variant(prob, variant, steps, warmup, max_iterations=1000):
     import torch
     from emagddm_amd import FETIDPSolver, SolverStatus
     from emagddm_amd.lambda0 import LambdaMPNN, default_weights
     precision, _, extra = variant.partition("+")
-    s = FETIDPSolver(precision=precision, profile=False)  # timed run: graphs on, no events
+    s = FETIDPSolver(precision=precision, profile=False,  # timed run: graphs on, no events
+                     max_iterations=max_iterations)
     s.setup_problem(prob)
     model = None
     if extra == "learned":
@@ -174,7 +179,7 @@ def main():
     variants = {}
     for prec in a.variants.split(","):
         try:
-            v = run_variant(prob, prec, a.steps, a.warmup)
+            v = run_variant(prob, prec, a.steps, a.warmup, a.max_iterations)
         except RuntimeError as e:
             print(f"[bench] {prec}: failed: {e}", file=sys.stderr, flush=True)
             continue
@@ -216,6 +221,7 @@ def main():
                              residual=v["residual"], reached_1e8=v["ok"],
                              fell_back_to_fp64=v["result"].fell_back_to_fp64,
                              learned_lambda0=v["result"].learned_guess,
+                             precond_fp64=v["result"].precond_fp64,
                              guess_ms=round(v["result"].guess_time_ms, 3),
                              initial_dual_residual=v["result"].initial_dual_residual)
                      for k, v in variants.items()},

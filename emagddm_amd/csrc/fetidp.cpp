@@ -126,7 +126,10 @@ struct fetidp_solver {
   DBuf<int64_t> free_a;
   DBuf<double> pool64, inv64, W64;
   DBuf<float> pool32, inv32, W32;
-  DBuf<float> Pf;  // Dirichlet preconditioner blocks, FP32 for both precisions
+  DBuf<float> Pf;   // Dirichlet preconditioner blocks in FP32 ...
+  DBuf<double> Pd;  // ... or in FP64 when FP32 rounding could make them indefinite
+  bool p64 = false;
+  double h_min = 0.0;  // shortest mesh edge (conditioning estimate)
   DBuf<int> fail;
   DBuf<double> cpool, cinv, sinv;
   DBuf<int> cfail;
@@ -335,6 +338,16 @@ struct fetidp_solver {
     build_plans(topo, plans);
     n_mat = nmat;
     h_xyz_.assign(h_xyz, h_xyz + 3 * n_nodes);
+    h_min = 1e300;
+    for (int64_t e = 0; e < topo.n_edges; ++e) {
+      const int64_t a = topo.edge_nodes[2 * e], b = topo.edge_nodes[2 * e + 1];
+      double d2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        const double d = h_xyz[3 * b + c] - h_xyz[3 * a + c];
+        d2 += d * d;
+      }
+      h_min = std::min(h_min, std::sqrt(d2));
+    }
     h_t2n_.assign(h_t2n, h_t2n + 4 * n_tets);
     h_mat_.assign(h_mat, h_mat + n_tets);
     h_nu_.assign(h_nu, h_nu + nmat);
@@ -499,8 +512,8 @@ struct fetidp_solver {
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& rowbase,
                     const DBuf<int64_t>& rabs, const DBuf<int64_t>& inv_base, T* pool, T* inv,
-                    int* failp, float* P) {
-    const int slot0 = P ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
+                    int* failp, bool local) {
+    const int slot0 = local ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     bool after_side = false;      // the previous launch was a mode-3 launch on st2
     for (const TileLaunch& L : S.launches) {
       if (L.mode == 3 && profiling()) {
@@ -524,7 +537,10 @@ struct fetidp_solver {
       if (!after_side) HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
       after_side = false;
       if (L.mode < 0) {
-        timed(5, [&] { launch_extract_P<T>(topo.n_sub, plans.max_ndp, devsub.p, pool, rowabs.p, ftile.p, P, st); });
+        timed(5, [&] {
+          if (p64) launch_extract_P<T, double>(topo.n_sub, plans.max_ndp, devsub.p, pool, rowabs.p, ftile.p, Pd.p, st);
+          else launch_extract_P<T, float>(topo.n_sub, plans.max_ndp, devsub.p, pool, rowabs.p, ftile.p, Pf.p, st);
+        });
         continue;
       }
       timed(slot0 + L.mode, [&] {
@@ -534,8 +550,23 @@ struct fetidp_solver {
     HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
   }
 
+  // FP64 Dirichlet blocks when cond(S_DD) ~ max nu / (min kappa h^2) could exceed
+  // what FP32 keeps positive definite (relative rounding 6e-8); cfg.precond_precision
+  // 1 / 2 forces FP32 / FP64.
+  bool precond_fp64() const {
+    if (cfg.precond_precision == 1) return false;
+    if (cfg.precond_precision == 2) return true;
+    double nu_max = 0.0, ka_min = 1e300;
+    for (int32_t m = 0; m < n_mat; ++m) {
+      nu_max = std::max(nu_max, h_nu_[m]);
+      ka_min = std::min(ka_min, h_kappa_[m]);
+    }
+    if (!(ka_min > 0.0) || !(h_min > 0.0)) return true;
+    return nu_max / (ka_min * h_min * h_min) > 1e6;
+  }
+
   template <typename T>
-  void factor_impl(DBuf<T>& pool, DBuf<T>& inv, DBuf<T>& W, DBuf<float>& P) {
+  void factor_impl(DBuf<T>& pool, DBuf<T>& inv, DBuf<T>& W) {
     const int64_t nt = topo.n_tets;
     // element matrices A_e = K_e + M_e (and F_e when the RHS is given by J)
     timed(0, [&] {
@@ -550,7 +581,13 @@ struct fetidp_solver {
     pool.alloc(size_t(plans.pool_elems));
     inv.alloc(size_t(plans.inv_elems));
     W.alloc(size_t(topo.w_elems));
-    P.alloc(size_t(topo.p_elems));
+    if (p64) {
+      Pf.release();
+      Pd.alloc(size_t(topo.p_elems));
+    } else {
+      Pd.release();
+      Pf.alloc(size_t(topo.p_elems));
+    }
     pool.zero(st);
     timed(1, [&] {
       if constexpr (sizeof(T) == 8)
@@ -562,9 +599,9 @@ struct fetidp_solver {
                    d_sgn.p, d_sgnD.p, d_half.p, st);
     fail.zero(st);
     if constexpr (sizeof(T) == 8)
-      run_schedule<T>(plans.local_sched, local_items, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, P.p);
+      run_schedule<T>(plans.local_sched, local_items, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, true);
     else
-      run_schedule<T>(plans.local_sched_ll, local_items_ll, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, P.p);
+      run_schedule<T>(plans.local_sched_ll, local_items_ll, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, true);
     // W extraction (st3) and the coarse problem (st, st2) only read the local
     // factors, so they overlap; st waits for st3 before the end of the factor.
     if (profiling()) {
@@ -580,7 +617,7 @@ struct fetidp_solver {
     launch_gather_pool<T>(plans.coarse.size(), coarse_target.p, coarse_ptr.p, coarse_srcpos.p,
                           pool.p, cpool.p, st);
     cfail.zero(st);
-    run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, nullptr);
+    run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, false);
     launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, coarse_rowabs.p, coarse_ftile.p, sinv.p, st);
     if (!profiling()) HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
     HIPCHECK(hipStreamSynchronize(st));
@@ -601,20 +638,21 @@ struct fetidp_solver {
     factored = false;
     const double t0 = now_ms();
     fell_back = false;
+    p64 = precond_fp64();
     if (fp32()) {
       pool64.release(); inv64.release(); W64.release();
       try {
-        factor_impl<float>(pool32, inv32, W32, Pf);
+        factor_impl<float>(pool32, inv32, W32);
       } catch (const NumericError& e) {
         // FP32 factor lost positive definiteness: refactor in FP64 (solve follows)
         fell_back = true;
         fallback_reason = e.what();
         pool32.release(); inv32.release(); W32.release();
-        factor_impl<double>(pool64, inv64, W64, Pf);
+        factor_impl<double>(pool64, inv64, W64);
       }
     } else {
       pool32.release(); inv32.release(); W32.release();
-      factor_impl<double>(pool64, inv64, W64, Pf);
+      factor_impl<double>(pool64, inv64, W64);
     }
     factored = true;
     return now_ms() - t0;
@@ -655,8 +693,12 @@ struct fetidp_solver {
   }
   void apply_M(const double* rin, double* zout) {
     timed(11, [&] {
-      launch_symv<float>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, Pf.p, 1,
-                         d_lam.p, d_sgnD.p, rin, sympart.p, dbuf.p, st);
+      if (p64)
+        launch_symv<double>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, Pd.p, 1,
+                            d_lam.p, d_sgnD.p, rin, sympart.p, dbuf.p, st);
+      else
+        launch_symv<float>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, Pf.p, 1,
+                           d_lam.p, d_sgnD.p, rin, sympart.p, dbuf.p, st);
     });
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgnD.p, dbuf.p, zout, st);
   }
@@ -727,7 +769,7 @@ struct fetidp_solver {
       pcpg_iteration_kernels<T>(W);
       return;
     }
-    const void* key[3] = {W.p, Pf.p, sinv.p};
+    const void* key[3] = {W.p, p64 ? (const void*)Pd.p : (const void*)Pf.p, sinv.p};
     if (!pcpg_exec || pcpg_key_t != int(sizeof(T)) || std::memcmp(key, pcpg_key, sizeof(key))) {
       if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));
       pcpg_exec = nullptr;
@@ -901,6 +943,7 @@ struct fetidp_solver {
     S.status = status;
     S.guess_ms = last_guess_ms;
     S.learned_guess = last_learned ? 1 : 0;
+    S.precond_fp64 = p64 ? 1 : 0;
     return status;
   }
 

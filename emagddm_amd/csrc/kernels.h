@@ -38,9 +38,9 @@ void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* row
 // MFMA lane-layout probe (one wave, 512 doubles of output).
 void launch_mfma_probe(int is_f32, double* out);
 
-template <typename T>
+template <typename T, typename TP>
 void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, const int64_t* rowabs,
-                      const int32_t* ftile, float* P, hipStream_t st);
+                      const int32_t* ftile, TP* P, hipStream_t st);
 template <typename T>
 void launch_extract_W(int ns, int max_ncp, int max_ndp, const DevSub* subs, const T* pool,
                       const int64_t* rowabs, const int32_t* ftile, T* W, hipStream_t st);

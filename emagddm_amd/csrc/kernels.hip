@@ -610,35 +610,39 @@ __device__ __forceinline__ void sym_tile_rc(int64_t e, int& row, int& col) {
   col = J * 64 + (w & 63);
 }
 
-template <typename T>
+template <typename T, typename TP>
 __global__ __launch_bounds__(256) void k_extract_P(const DevSub* __restrict__ subs,
                                                    const T* __restrict__ pool,
                                                    const int64_t* __restrict__ rowabs,
                                                    const int32_t* __restrict__ ftile,
-                                                   float* __restrict__ P) {
+                                                   TP* __restrict__ P) {
   const DevSub S = subs[blockIdx.y];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= sym_elems(S.ndp)) return;
   int a, b;
   sym_tile_rc(e, a, b);
-  P[S.p_off + e] = (float)pool_sym(pool, rowabs + S.ft_off, ftile + S.ft_off, S.o_d + a, S.o_d + b);
+  P[S.p_off + e] = (TP)pool_sym(pool, rowabs + S.ft_off, ftile + S.ft_off, S.o_d + a, S.o_d + b);
 }
 
-// The Dirichlet preconditioner is stored in FP32 for both factor precisions: it
-// only steers PCPG, the dual operator F and the residual checks stay in the
-// factor precision / FP64.
-template <typename T>
+// The Dirichlet preconditioner blocks are stored in FP32 (half the bytes per
+// PCPG iteration; they only steer PCPG) unless the problem is so ill-conditioned
+// that FP32 rounding could make them indefinite; then in FP64.
+template <typename T, typename TP>
 void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, const int64_t* rowabs,
-                      const int32_t* ftile, float* P, hipStream_t st) {
+                      const int32_t* ftile, TP* P, hipStream_t st) {
   const int64_t n = sym_tile_elems(max_ndp);
   if (n <= 0 || ns <= 0) return;
-  hipLaunchKernelGGL(k_extract_P<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, rowabs,
-                     ftile, P);
+  hipLaunchKernelGGL((k_extract_P<T, TP>), dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool,
+                     rowabs, ftile, P);
 }
-template void launch_extract_P<double>(int, int, const DevSub*, const double*, const int64_t*,
-                                       const int32_t*, float*, hipStream_t);
-template void launch_extract_P<float>(int, int, const DevSub*, const float*, const int64_t*,
-                                      const int32_t*, float*, hipStream_t);
+template void launch_extract_P<double, float>(int, int, const DevSub*, const double*, const int64_t*,
+                                              const int32_t*, float*, hipStream_t);
+template void launch_extract_P<float, float>(int, int, const DevSub*, const float*, const int64_t*,
+                                             const int32_t*, float*, hipStream_t);
+template void launch_extract_P<double, double>(int, int, const DevSub*, const double*, const int64_t*,
+                                               const int32_t*, double*, hipStream_t);
+template void launch_extract_P<float, double>(int, int, const DevSub*, const float*, const int64_t*,
+                                              const int32_t*, double*, hipStream_t);
 
 // W = Z (lower tiles) then Phi^T from the trailing block: Z = -A(D',D'), Phi = -A(D',c)
 template <typename T>

@@ -41,6 +41,7 @@ class SolverResult:
     fell_back_to_fp64: bool = False
     guess_time_ms: float = 0.0
     learned_guess: bool = False
+    precond_fp64: bool = False
 
 
 def _dp(a):
@@ -64,7 +65,7 @@ class FETIDPSolver:
 
     def __init__(self, precision="fp64", tolerance=1e-8, dual_tolerance=1e-11,
                  max_iterations=1000, max_refinements=12, profile=False,
-                 inner_tolerance=1e-2, graphs=True):
+                 inner_tolerance=1e-2, graphs=True, precond_precision="auto"):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.fetidp_config_default(C.byref(cfg))
@@ -76,6 +77,7 @@ class FETIDPSolver:
         cfg.verbose = 2 if profile else 0
         cfg.inner_tolerance = inner_tolerance
         cfg.graphs = 1 if graphs else 0
+        cfg.precond_precision = {"auto": 0, "fp32": 1, "fp64": 2}[precond_precision]
         self.precision = precision
         self._h = self._lib.fetidp_create(C.byref(cfg))
         if not self._h:
@@ -89,6 +91,13 @@ class FETIDPSolver:
         cfg = _lib.Config()
         self._check(self._lib.fetidp_get_config(self._h, C.byref(cfg)), "fetidp_get_config")
         return cfg
+
+    def set_config_values(self, **fields):
+        """Change configuration fields (names of fetidp_config) on the live handle."""
+        cfg = self.get_config()
+        for k, v in fields.items():
+            setattr(cfg, k, v)
+        self._check(self._lib.fetidp_set_config(self._h, C.byref(cfg)), "fetidp_set_config")
 
     def set_profile(self, on: bool):
         """Switch per-kernel-class event timing (disables graph replay while on)."""
@@ -172,7 +181,8 @@ class FETIDPSolver:
                             initial_dual_residual=st.initial_dual_residual,
                             factor_time_ms=st.factor_ms, symbolic_time_ms=st.symbolic_ms,
                             fell_back_to_fp64=bool(st.fell_back), guess_time_ms=st.guess_ms,
-                            learned_guess=bool(st.learned_guess))
+                            learned_guess=bool(st.learned_guess),
+                            precond_fp64=bool(st.precond_fp64))
 
     # --------------------------------------------------------------- learned lambda0
     def set_lambda0_model(self, model):

@@ -47,7 +47,8 @@ typedef struct fetidp_config {
   double inner_tolerance;   /* FP32 mode: PCPG stop ||r||/||d|| of each refinement step */
   int32_t graphs;           /* 1: replay each PCPG iteration as one captured HIP graph
                                (ignored while event profiling is on) */
-  int32_t reserved;
+  int32_t precond_precision; /* Dirichlet blocks: 0 auto (FP64 when max nu / (min kappa
+                               h_min^2) > 1e6, else FP32), 1 FP32, 2 FP64 */
 } fetidp_config;
 
 typedef struct fetidp_stats {
@@ -63,7 +64,7 @@ typedef struct fetidp_stats {
   double symbolic_ms;       /* host topology + plans + upload (fetidp_setup) */
   double guess_ms;          /* learned lambda0: features + network (0 without a model) */
   int32_t learned_guess;    /* 1: PCPG started from the attached model's lambda0 */
-  int32_t reserved;
+  int32_t precond_fp64;     /* 1: the Dirichlet blocks were kept in FP64 */
 } fetidp_stats;
 
 typedef struct fetidp_solver* fetidp_handle;

@@ -94,3 +94,30 @@ def test_solve_matches_direct(gpu_available, kind, precision):
     assert res.residual_norm <= 1e-9
     assert solvers.rel_residual(asm["A"], res.x, asm["f"]) <= 1.01e-9
     assert err <= (1e-8 if precision == "fp64" else 1e-7)
+
+
+def test_contrast_64_subdomains_matches_oracle_iterations(gpu_available):
+    """Per-subdomain nu/kappa contrast with a low-nu sphere cutting subdomains: the
+    FP64 Dirichlet blocks (chosen automatically: max nu / (min kappa h^2) > 1e6)
+    keep PCPG on the oracle's iteration count; the answer matches the direct solve."""
+    from oracle import fetidp as ofetidp
+    p = W.make_problem(16, 4, None, "contrast", seed=7, name="c16")
+    print(f"[contrast] 16^3 4^3 subdomains tets={p.n_tets}", flush=True)
+    F = ofetidp.DenseFETIDP(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.tet_subdomain,
+                            p.pec_pairs, p.jq)
+    out = F.solve(tol=1e-10, max_iter=2000)
+    xd = solvers.direct(F.A, F.f)
+    # one PCPG stage to the same dual tolerance (global residual check switched off)
+    s = FETIDPSolver(tolerance=1.0, dual_tolerance=1e-10, max_iterations=2000)
+    s.setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    r = s.solve()
+    assert r.status == SolverStatus.SUCCESS and r.precond_fp64
+    # a long ill-conditioned CG run: rounding order moves the count a little
+    assert abs(r.iterations - out["iterations"]) <= 0.1 * out["iterations"]
+    # the default stop rule reaches the 1e-8 global residual and the direct answer
+    s.set_config_values(tolerance=1e-8)
+    r = s.solve()
+    assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-8
+    assert np.linalg.norm(r.x - xd) <= 1e-7 * np.linalg.norm(xd)
