@@ -21,7 +21,8 @@ class Config(C.Structure):
     _fields_ = [("precision", C.c_int32), ("max_iterations", C.c_int32),
                 ("tolerance", C.c_double), ("dual_tolerance", C.c_double),
                 ("max_refinements", C.c_int32), ("verbose", C.c_int32),
-                ("inner_tolerance", C.c_double), ("graphs", C.c_int32), ("precond_precision", C.c_int32)]
+                ("inner_tolerance", C.c_double), ("graphs", C.c_int32), ("precond_precision", C.c_int32),
+                ("scaling", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -29,7 +30,8 @@ class Stats(C.Structure):
                 ("fell_back", C.c_int32), ("residual_norm", C.c_double),
                 ("dual_residual", C.c_double), ("initial_dual_residual", C.c_double),
                 ("factor_ms", C.c_double), ("solve_ms", C.c_double), ("symbolic_ms", C.c_double),
-                ("guess_ms", C.c_double), ("learned_guess", C.c_int32), ("precond_fp64", C.c_int32)]
+                ("guess_ms", C.c_double), ("learned_guess", C.c_int32), ("precond_fp64", C.c_int32),
+                ("deluxe", C.c_int32), ("reserved", C.c_int32)]
 
 
 EXPORTS = [

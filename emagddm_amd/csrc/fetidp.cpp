@@ -115,6 +115,17 @@ struct fetidp_solver {
   DBuf<DevSub> devsub;
   DBuf<SymTile> sym_tiles;
   DBuf<double> sympart;
+  // deluxe scaling (interface faces)
+  bool deluxe = false;
+  DBuf<FaceDev> face_dev;
+  DBuf<DevSub> face_sub;
+  DBuf<int32_t> face_map_lam, face_map_self, d_self;
+  DBuf<int64_t> face_slot0, face_slot1;
+  DBuf<SymTile> face_tiles;
+  DBuf<TileWork> face_items;
+  DBuf<int64_t> face_rowbase, face_rowabs, face_inv_base;
+  DBuf<double> fpool, finv, fops, fpart, fv1, fv2, xslot;
+  DBuf<int> ffail;
   DBuf<int32_t> vec_dof;
   DBuf<double> vec_w;
   DBuf<int32_t> d_lam;
@@ -413,6 +424,21 @@ struct fetidp_solver {
     devsub.upload(plans.devsub);
     sym_tiles.upload(plans.sym_tiles);
     sympart.alloc(size_t(std::max<int64_t>(topo.sp_elems, 1)));
+    {
+      const FacePlan& F = plans.faces;
+      face_dev.upload(F.dev);
+      face_sub.upload(F.sub);
+      face_map_lam.upload(F.map_lam);
+      face_map_self.upload(F.map_self);
+      face_slot0.upload(F.slot0);
+      face_slot1.upload(F.slot1);
+      face_tiles.upload(F.tiles);
+      face_items.upload(F.sched.items);
+      face_rowbase.upload(F.rowbase);
+      face_rowabs.upload(F.rowabs);
+      face_inv_base.upload(F.inv_off);
+      d_self.upload(plans.d_self);
+    }
     vec_dof.upload(plans.vec_dof);
     vec_w.upload(plans.vec_w);
     d_lam.upload(topo.d_lam);
@@ -553,7 +579,24 @@ struct fetidp_solver {
   // FP64 Dirichlet blocks when cond(S_DD) ~ max nu / (min kappa h^2) could exceed
   // what FP32 keeps positive definite (relative rounding 6e-8); cfg.precond_precision
   // 1 / 2 forces FP32 / FP64.
+  // Deluxe scaling when the coefficient ratio kappa/nu varies by more than 1e4
+  // between materials: diagonal stiffness weights then misjudge the gradient
+  // (kappa-only) modes on the faces.  cfg.scaling 1 / 2 forces stiffness / deluxe.
+  bool use_deluxe() const {
+    if (cfg.scaling == 1) return false;
+    if (cfg.scaling == 2) return true;
+    double lo = 1e300, hi = 0.0;
+    for (int32_t m = 0; m < n_mat; ++m) {
+      const double q = h_kappa_[m] / h_nu_[m];
+      lo = std::min(lo, q);
+      hi = std::max(hi, q);
+    }
+    if (!(lo > 0.0)) return hi > 0.0;
+    return hi / lo > 1e4;
+  }
+
   bool precond_fp64() const {
+    if (deluxe) return true;  // the face blocks are formed and inverted in FP64
     if (cfg.precond_precision == 1) return false;
     if (cfg.precond_precision == 2) return true;
     double nu_max = 0.0, ka_min = 1e300;
@@ -619,6 +662,7 @@ struct fetidp_solver {
     cfail.zero(st);
     run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, false);
     launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, coarse_rowabs.p, coarse_ftile.p, sinv.p, st);
+    if (deluxe) deluxe_setup();
     if (!profiling()) HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
     HIPCHECK(hipStreamSynchronize(st));
     prof_collect();
@@ -629,6 +673,12 @@ struct fetidp_solver {
     for (int s = 0; s < topo.n_sub; ++s)
       if (hf[s]) throw NumericError("subdomain " + std::to_string(s) + ": K_rr not positive definite");
     if (hc) throw NumericError("coarse Schur complement not positive definite");
+    if (deluxe) {
+      std::vector<int> ff(size_t(plans.faces.n));
+      HIPCHECK(hipMemcpy(ff.data(), ffail.p, sizeof(int) * ff.size(), hipMemcpyDeviceToHost));
+      for (size_t f = 0; f < ff.size(); ++f)
+        if (ff[f]) throw NumericError("face " + std::to_string(f) + ": S_s + S_t not positive definite");
+    }
   }
 
   double factor() {
@@ -638,6 +688,7 @@ struct fetidp_solver {
     factored = false;
     const double t0 = now_ms();
     fell_back = false;
+    deluxe = use_deluxe() && plans.faces.n > 0;
     p64 = precond_fp64();
     if (fp32()) {
       pool64.release(); inv64.release(); W64.release();
@@ -684,6 +735,48 @@ struct fetidp_solver {
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
+  // Per face: [[A_F, I], [I, 0]] from the FP64 Dirichlet blocks, eliminate A_F with the
+  // batched tile kernels, keep A_F^-1 and B_F as lower tiles.
+  void deluxe_setup() {
+    const FacePlan& F = plans.faces;
+    fpool.alloc(size_t(F.pool_elems));
+    finv.alloc(size_t(std::max<int64_t>(F.inv_elems, 1)));
+    fops.alloc(size_t(F.op_elems));
+    fpart.alloc(size_t(std::max<int64_t>(F.sp_elems, 1)));
+    fv1.alloc(size_t(F.vec_elems));
+    fv2.alloc(size_t(F.vec_elems));
+    ffail.alloc(size_t(F.n));
+    if (xslot.n != size_t(topo.dvec_elems)) {
+      xslot.alloc(size_t(topo.dvec_elems));
+      xslot.zero(st);  // padding slots stay zero; real slots are rewritten every application
+    }
+    launch_face_setup(F.n, F.max_T, face_dev.p, devsub.p, Pd.p, face_slot0.p, face_slot1.p, fpool.p, st);
+    ffail.zero(st);
+    run_schedule<double>(F.sched, face_items, face_rowbase, face_rowabs, face_inv_base, fpool.p, finv.p,
+                         ffail.p, false);
+    launch_face_extract(F.n, F.max_T, face_dev.p, devsub.p, Pd.p, face_slot1.p, fpool.p, fops.p, st);
+  }
+
+  // z = M^-1 r with deluxe weights: x_s0 = D_F r, x_s1 = D_F r - r (D_F = A_F^-1 B_F),
+  // y = S_DD x per subdomain, z = D_F^T (y_s0 + y_s1) - y_s1 on every face.
+  void apply_M_deluxe(const double* rin, double* zout) {
+    const FacePlan& F = plans.faces;
+    const int64_t nt = int64_t(F.tiles.size());
+    launch_symv<double>(nt, face_tiles.p, int(F.n), F.max_T, face_sub.p, fops.p, 0, face_map_lam.p, nullptr,
+                        rin, fpart.p, fv1.p, st);                                  // B r
+    launch_symv<double>(nt, face_tiles.p, int(F.n), F.max_T, face_sub.p, fops.p, 1, face_map_self.p, nullptr,
+                        fv1.p, fpart.p, fv2.p, st);                                // A^-1 B r
+    launch_deluxe_split(F.vec_elems, face_map_lam.p, face_slot0.p, face_slot1.p, fv2.p, rin, xslot.p, st);
+    launch_symv<double>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, Pd.p, 1,
+                        d_self.p, nullptr, xslot.p, sympart.p, dbuf.p, st);       // y = S x
+    launch_deluxe_sum(F.vec_elems, face_map_lam.p, face_slot0.p, face_slot1.p, dbuf.p, fv1.p, st);
+    launch_symv<double>(nt, face_tiles.p, int(F.n), F.max_T, face_sub.p, fops.p, 1, face_map_self.p, nullptr,
+                        fv1.p, fpart.p, fv2.p, st);                                // A^-1 u
+    launch_symv<double>(nt, face_tiles.p, int(F.n), F.max_T, face_sub.p, fops.p, 0, face_map_self.p, nullptr,
+                        fv2.p, fpart.p, fv1.p, st);                                // B A^-1 u
+    launch_deluxe_out(F.vec_elems, face_map_lam.p, face_slot1.p, fv1.p, dbuf.p, zout, st);
+  }
+
   // ybuf = W x with x = B^T p gathered per subdomain: Z from its lower tiles, then Phi^T rows
   template <typename T>
   void apply_W(const DBuf<T>& W, const double* pin) {
@@ -692,6 +785,10 @@ struct fetidp_solver {
     launch_gemv_phi<T>(topo.n_sub, plans.max_ncp, devsub.p, W.p, d_lam.p, d_sgn.p, pin, ybuf.p, st, plans.max_ndp);
   }
   void apply_M(const double* rin, double* zout) {
+    if (deluxe) {
+      timed(11, [&] { apply_M_deluxe(rin, zout); });
+      return;
+    }
     timed(11, [&] {
       if (p64)
         launch_symv<double>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, Pd.p, 1,
@@ -770,7 +867,8 @@ struct fetidp_solver {
       return;
     }
     const void* key[3] = {W.p, p64 ? (const void*)Pd.p : (const void*)Pf.p, sinv.p};
-    if (!pcpg_exec || pcpg_key_t != int(sizeof(T)) || std::memcmp(key, pcpg_key, sizeof(key))) {
+    const int key_t = int(sizeof(T)) | (deluxe ? 256 : 0);
+    if (!pcpg_exec || pcpg_key_t != key_t || std::memcmp(key, pcpg_key, sizeof(key))) {
       if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));
       pcpg_exec = nullptr;
       hipGraph_t g = nullptr;
@@ -781,7 +879,7 @@ struct fetidp_solver {
       (void)hipGraphDestroy(g);
       HIPCHECK(e);
       std::memcpy(pcpg_key, key, sizeof(key));
-      pcpg_key_t = int(sizeof(T));
+      pcpg_key_t = key_t;
     }
     HIPCHECK(hipGraphLaunch(pcpg_exec, st));
   }
@@ -841,6 +939,7 @@ struct fetidp_solver {
     return FETIDP_MAX_ITER_REACHED;
   }
   double prev_ = 0.0;
+  double first_init = -1.0;  // initial dual residual of the first PCPG stage of a solve
 
   template <typename T>
   int solve_impl(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W,
@@ -850,12 +949,13 @@ struct fetidp_solver {
     ref_hist.clear();
     int its = 0, status = FETIDP_SUCCESS;
     double dn = 0.0, res = 0.0;
+    first_init = -1.0;
     if (!fp32()) {
       condense<T>(pool, inv, W, f_glob.p);
       if (h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
       else if (lam_model) learned_guess();
       else lam.zero(st);
-      double tol = cfg.dual_tolerance;
+      double tol = cfg.dual_tolerance, prev = 1e300;
       bool fresh = true;
       for (int round = 0; round < 6; ++round) {
         status = pcpg<T>(W, tol, cfg.max_iterations - its, fresh, its, dn);
@@ -864,10 +964,28 @@ struct fetidp_solver {
         res = global_residual(x.p, f_glob.p);
         ref_hist.push_back(res);
         if (status != FETIDP_SUCCESS || res <= cfg.tolerance) break;
+        if (res > 0.5 * prev) break;  // a tighter dual tolerance no longer helps
+        prev = res;
         tol *= 0.1;
       }
-      S.refinements = 0;
       HIPCHECK(hipMemcpyAsync(xtot.p, x.p, sizeof(double) * nf, hipMemcpyDeviceToDevice, st));
+      first_init = init_dual;
+      // FP64 iterative refinement with the same factor when the recovered solution has
+      // a residual floor (ill-conditioned local problems): x += FETI-DP^-1 (f - K x);
+      // rglob holds f - K xtot from the last residual evaluation
+      int k = 0;
+      while (status == FETIDP_SUCCESS && res > cfg.tolerance && k < cfg.max_refinements) {
+        condense<T>(pool, inv, W, rglob.p);
+        lam.zero(st);
+        status = pcpg<T>(W, cfg.dual_tolerance, cfg.max_iterations - its, true, its, dn);
+        if (status != FETIDP_SUCCESS) break;
+        recover<T>(pool, inv, W, x.p);
+        launch_axpy(nf, 1.0, x.p, xtot.p, st);
+        res = global_residual(xtot.p, f_glob.p);
+        ref_hist.push_back(res);
+        ++k;
+      }
+      S.refinements = k;
     } else {
       // FP64 flexible CG on the assembled system, preconditioned by the FP32
       // FETI-DP solve (inner PCPG to cfg.inner_tolerance): the outer recurrence,
@@ -920,7 +1038,7 @@ struct fetidp_solver {
     S.iterations = its;
     S.residual_norm = res;
     S.dual_residual = last_dual;
-    S.initial_dual_residual = init_dual;
+    S.initial_dual_residual = first_init >= 0.0 ? first_init : init_dual;
     return status;
   }
 
@@ -944,6 +1062,7 @@ struct fetidp_solver {
     S.guess_ms = last_guess_ms;
     S.learned_guess = last_learned ? 1 : 0;
     S.precond_fp64 = p64 ? 1 : 0;
+    S.deluxe = deluxe ? 1 : 0;
     return status;
   }
 

@@ -104,6 +104,19 @@ void launch_axpy2(int64_t n, double alpha, const double* p, const double* q, dou
 void launch_xpby(int64_t n, const double* z, double beta, double* p, hipStream_t st);
 void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t st);
 void launch_scale2(int64_t n, double alpha, double* x, double* y, hipStream_t st);
+// deluxe scaling (FP64 Dirichlet blocks P)
+void launch_face_setup(int64_t nfaces, int max_tf, const FaceDev* faces, const DevSub* subs,
+                       const double* P, const int64_t* slot0, const int64_t* slot1, double* fpool,
+                       hipStream_t st);
+void launch_face_extract(int64_t nfaces, int max_tf, const FaceDev* faces, const DevSub* subs,
+                         const double* P, const int64_t* slot1, const double* fpool, double* ops,
+                         hipStream_t st);
+void launch_deluxe_split(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
+                         const double* g, const double* r, double* xs, hipStream_t st);
+void launch_deluxe_sum(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
+                       const double* y, double* u, hipStream_t st);
+void launch_deluxe_out(int64_t nv, const int32_t* map_lam, const int64_t* slot1, const double* h,
+                       const double* y, double* z, hipStream_t st);
 // PCPG device scalars (slots of the scalar block)
 enum { kScRZ = 0, kScPQ = 1, kScAlpha = 2, kScBeta = 3, kScRR = 4, kScCount = 8 };
 void launch_pcpg_alpha(const double* partial, double* sc, double* host, hipStream_t st);

@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ t
     const int blk = (tid >> 6) ? tl.I : tl.J;
     const int64_t slot = S.dvec_off + blk * 64 + lane;
     const int32_t l = d_lam[slot];
-    const double xv = l >= 0 ? d_scale[slot] * v[l] : 0.0;
+    const double xv = l >= 0 ? (d_scale ? d_scale[slot] : 1.0) * v[l] : 0.0;
     if (tid >> 6) xI[lane] = xv;
     else xJ[lane] = xv;
   }
@@ -1408,6 +1408,158 @@ void launch_axpy2_dev(int64_t n, const double* sc, const double* p, const double
 void launch_xpby_dev(int64_t n, const double* z, const double* sc, double* p, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_xpby_dev, dim3((n + 255) / 256), dim3(256), 0, st, n, z, sc, p);
+}
+
+// ---------------------------------------------------------------------------
+// Deluxe scaling: per interface face F, A_F = S_s0,FF + S_s1,FF and B_F = S_s1,FF
+// ---------------------------------------------------------------------------
+// element (a, b) of subdomain s's FP64 Dirichlet block (lower-tile store), a, b
+// absolute D slots of s
+__device__ __forceinline__ double p_elem(const DevSub& S, const double* __restrict__ P, int64_t sa,
+                                         int64_t sb) {
+  int i = int(sa - S.dvec_off), j = int(sb - S.dvec_off);
+  if (i < j) {
+    const int t = i;
+    i = j;
+    j = t;
+  }
+  const int64_t I = i >> 6, J = j >> 6;
+  return P[S.p_off + (I * (I + 1) / 2 + J) * kTileElems + (i & 63) * 64 + (j & 63)];
+}
+
+// [[A_F, I], [I, 0]] of every face as dense lower tiles, column-major inside a tile
+// (the factor-pool convention); diagonal tiles are filled symmetric, padding rows of
+// A_F get a unit diagonal and no coupling.
+__global__ __launch_bounds__(256) void k_face_pool(const FaceDev* __restrict__ faces,
+                                                   const DevSub* __restrict__ subs,
+                                                   const double* __restrict__ P,
+                                                   const int64_t* __restrict__ slot0,
+                                                   const int64_t* __restrict__ slot1,
+                                                   double* __restrict__ fpool) {
+  const FaceDev F = faces[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ta = 2 * F.tf;
+  if (e >= ta * (ta + 1) / 2 * kTileElems) return;
+  const int64_t t = e >> 12;
+  int I = int((sqrt(8.0 * double(t) + 1.0) - 1.0) * 0.5);
+  while (int64_t(I) * (I + 1) / 2 > t) --I;
+  while (int64_t(I + 1) * (I + 2) / 2 <= t) ++I;
+  const int J = int(t - int64_t(I) * (I + 1) / 2);
+  const int w = int(e & 4095);
+  const int r = I * 64 + (w & 63), c = J * 64 + (w >> 6);
+  const int a = r > c ? r : c, b = r > c ? c : r;
+  const int nfp = F.tf * 64;
+  double v = 0.0;
+  if (a < nfp) {
+    if (a < F.nf && b < F.nf) {
+      const int64_t ka = F.vec_off + a, kb = F.vec_off + b;
+      v = p_elem(subs[F.s0], P, slot0[ka], slot0[kb]) + p_elem(subs[F.s1], P, slot1[ka], slot1[kb]);
+    } else {
+      v = a == b ? 1.0 : 0.0;
+    }
+  } else if (b < nfp) {
+    v = (a - nfp == b && b < F.nf) ? 1.0 : 0.0;
+  }
+  fpool[F.pool_base + e] = v;
+}
+
+// A_F^-1 = -(trailing block of the factored pool) and B_F, as lower tiles row-major
+// inside a tile (the symmetric-product layout)
+__global__ __launch_bounds__(256) void k_face_extract(const FaceDev* __restrict__ faces,
+                                                      const DevSub* __restrict__ subs,
+                                                      const double* __restrict__ P,
+                                                      const int64_t* __restrict__ slot1,
+                                                      const double* __restrict__ fpool,
+                                                      double* __restrict__ ops) {
+  const FaceDev F = faces[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nfp = F.tf * 64;
+  if (e >= sym_elems(nfp)) return;
+  int i, j;
+  sym_tile_rc(e, i, j);
+  const int a = (i > j ? i : j) + nfp, b = (i > j ? j : i) + nfp;
+  ops[F.ainv_off + e] =
+      -fpool[F.pool_base + ((int64_t)(a >> 6) * ((a >> 6) + 1) / 2 + (b >> 6)) * kTileElems +
+             (int64_t)(b & 63) * 64 + (a & 63)];
+  ops[F.b_off + e] = (i < F.nf && j < F.nf)
+                         ? p_elem(subs[F.s1], P, slot1[F.vec_off + i], slot1[F.vec_off + j])
+                         : 0.0;
+}
+
+void launch_face_setup(int64_t nfaces, int max_tf, const FaceDev* faces, const DevSub* subs,
+                       const double* P, const int64_t* slot0, const int64_t* slot1, double* fpool,
+                       hipStream_t st) {
+  if (nfaces <= 0) return;
+  const int64_t ta = 2 * int64_t(max_tf);
+  const int64_t n = ta * (ta + 1) / 2 * kTileElems;
+  hipLaunchKernelGGL(k_face_pool, dim3((n + 255) / 256, nfaces), dim3(256), 0, st, faces, subs, P,
+                     slot0, slot1, fpool);
+}
+
+void launch_face_extract(int64_t nfaces, int max_tf, const FaceDev* faces, const DevSub* subs,
+                         const double* P, const int64_t* slot1, const double* fpool, double* ops,
+                         hipStream_t st) {
+  if (nfaces <= 0) return;
+  const int64_t n = sym_tile_elems(int64_t(max_tf) * 64);
+  hipLaunchKernelGGL(k_face_extract, dim3((n + 255) / 256, nfaces), dim3(256), 0, st, faces, subs,
+                     P, slot1, fpool, ops);
+}
+
+// x (D slots) from g = D_F r on every face: side s0 takes g, side s1 takes g - r
+__global__ __launch_bounds__(256) void k_deluxe_split(int64_t nv, const int32_t* __restrict__ map_lam,
+                                                      const int64_t* __restrict__ slot0,
+                                                      const int64_t* __restrict__ slot1,
+                                                      const double* __restrict__ g,
+                                                      const double* __restrict__ r,
+                                                      double* __restrict__ xs) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= nv) return;
+  const int32_t l = map_lam[k];
+  if (l < 0) return;
+  xs[slot0[k]] = g[k];
+  xs[slot1[k]] = g[k] - r[l];
+}
+
+// u = y(s0 side) + y(s1 side) on every face (0 on padding)
+__global__ __launch_bounds__(256) void k_deluxe_sum(int64_t nv, const int32_t* __restrict__ map_lam,
+                                                    const int64_t* __restrict__ slot0,
+                                                    const int64_t* __restrict__ slot1,
+                                                    const double* __restrict__ y,
+                                                    double* __restrict__ u) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= nv) return;
+  u[k] = map_lam[k] >= 0 ? y[slot0[k]] + y[slot1[k]] : 0.0;
+}
+
+// z = h - y(s1 side), scattered to the multipliers
+__global__ __launch_bounds__(256) void k_deluxe_out(int64_t nv, const int32_t* __restrict__ map_lam,
+                                                    const int64_t* __restrict__ slot1,
+                                                    const double* __restrict__ h,
+                                                    const double* __restrict__ y,
+                                                    double* __restrict__ z) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= nv) return;
+  const int32_t l = map_lam[k];
+  if (l >= 0) z[l] = h[k] - y[slot1[k]];
+}
+
+void launch_deluxe_split(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
+                         const double* g, const double* r, double* xs, hipStream_t st) {
+  if (nv > 0)
+    hipLaunchKernelGGL(k_deluxe_split, dim3((nv + 255) / 256), dim3(256), 0, st, nv, map_lam, slot0,
+                       slot1, g, r, xs);
+}
+void launch_deluxe_sum(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
+                       const double* y, double* u, hipStream_t st) {
+  if (nv > 0)
+    hipLaunchKernelGGL(k_deluxe_sum, dim3((nv + 255) / 256), dim3(256), 0, st, nv, map_lam, slot0, slot1,
+                       y, u);
+}
+void launch_deluxe_out(int64_t nv, const int32_t* map_lam, const int64_t* slot1, const double* h,
+                       const double* y, double* z, hipStream_t st) {
+  if (nv > 0)
+    hipLaunchKernelGGL(k_deluxe_out, dim3((nv + 255) / 256), dim3(256), 0, st, nv, map_lam, slot1, h,
+                       y, z);
 }
 
 // x *= alpha, y *= alpha

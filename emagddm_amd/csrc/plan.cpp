@@ -88,6 +88,109 @@ TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc, const int32_t* ft)
   return S;
 }
 
+TileSchedule batched_dense_schedule(const std::vector<int>& Tp) {
+  TileSchedule S;
+  int maxTp = 0;
+  for (int t : Tp) maxTp = std::max(maxTp, t);
+  for (int j = 0; j < maxTp; ++j) {
+    std::vector<TileWork> diag, next, below;
+    for (int m = 0; m < int(Tp.size()); ++m) {
+      const int tp = Tp[m], ta = 2 * tp;
+      if (j >= tp) continue;
+      if (j == 0) diag.push_back(TileWork{m, 0, 0, 0, 0, 0});
+      for (int I = j + 1; I < ta; ++I) {
+        const TileWork w{m, I, j, 0, j, 1};
+        (I == j + 1 && I < tp ? next : below).push_back(w);
+      }
+    }
+    add_tiles(S, 2, diag);
+    add_tiles(S, 3, next);
+    add_tiles(S, 1, below);
+  }
+  std::vector<TileWork> schur;
+  for (int m = 0; m < int(Tp.size()); ++m) {
+    const int tp = Tp[m], ta = 2 * tp;
+    for (int J = tp; J < ta; ++J)
+      for (int I = J + 1; I < ta; ++I) schur.push_back(TileWork{m, I, J, 0, tp, 0});
+  }
+  add_tiles(S, 0, schur);
+  return S;
+}
+
+namespace {
+// Faces of the partition and their dense factorization plan (see FacePlan).
+void build_faces(const Topology& T, Plans& P) {
+  FacePlan& F = P.faces;
+  F = FacePlan();
+  const int64_t nl = T.n_lam;
+  std::vector<int32_t> slot_sub(T.dvec_elems, -1);
+  for (int64_t s = 0; s < T.n_sub; ++s)
+    for (int k = 0; k < T.sub[s].ndp; ++k) slot_sub[T.sub[s].dvec_off + k] = int32_t(s);
+  // multipliers sorted by (s0, s1, lambda)
+  std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> key(nl);
+  for (int64_t l = 0; l < nl; ++l)
+    key[l] = {{slot_sub[T.lam_pos[2 * l]], slot_sub[T.lam_pos[2 * l + 1]]}, int32_t(l)};
+  std::sort(key.begin(), key.end());
+  std::vector<int> tp_list;
+  int64_t vec = 0, op = 0, sp = 0, pool = 0, inv = 0;
+  for (int64_t a = 0; a < nl;) {
+    int64_t b = a;
+    while (b < nl && key[b].first == key[a].first) ++b;
+    FaceDev d{};
+    d.s0 = key[a].first.first;
+    d.s1 = key[a].first.second;
+    d.nf = int32_t(b - a);
+    const int nfp = int(round_up(d.nf, kTile));
+    d.tf = nfp / kTile;
+    d.vec_off = vec;
+    d.pool_base = pool;
+    d.ainv_off = op;
+    d.b_off = op + sym_tile_elems(nfp);
+    DevSub u{};
+    u.ndp = nfp;
+    u.nd = d.nf;
+    u.dvec_off = vec;
+    u.y_off = vec;
+    u.w_off = d.b_off;
+    u.p_off = d.ainv_off;
+    u.sp_off = sp;
+    const int f = int(F.dev.size());
+    for (int i = 0; i < nfp; ++i) {
+      const bool real = i < d.nf;
+      const int32_t l = real ? key[a + i].second : -1;
+      F.map_lam.push_back(l);
+      F.map_self.push_back(real ? int32_t(vec + i) : -1);
+      F.slot0.push_back(real ? T.lam_pos[2 * l] : -1);
+      F.slot1.push_back(real ? T.lam_pos[2 * l + 1] : -1);
+    }
+    for (int I = 0; I < d.tf; ++I)
+      for (int J = 0; J <= I; ++J) F.tiles.push_back(SymTile{f, I, J});
+    // [[A, I], [I, 0]]: 2 tf tile rows, dense lower tiles, tile (I, J) at base + (I(I+1)/2 + J) 4096
+    const int ta = 2 * d.tf;
+    F.rowbase.push_back(int64_t(F.rowabs.size()));
+    for (int I = 0; I < ta; ++I) F.rowabs.push_back(pool + tile_offset(I, 0));
+    F.inv_off.push_back(inv);
+    vec += nfp;
+    op += 2 * sym_tile_elems(nfp);
+    sp += int64_t(d.tf) * d.tf * kTile;
+    pool += int64_t(ta) * (ta + 1) / 2 * kTileElems;
+    inv += int64_t(d.tf) * kTileElems;
+    F.max_T = std::max(F.max_T, d.tf);
+    tp_list.push_back(d.tf);
+    F.dev.push_back(d);
+    F.sub.push_back(u);
+    a = b;
+  }
+  F.n = int64_t(F.dev.size());
+  F.vec_elems = vec;
+  F.op_elems = op;
+  F.sp_elems = sp;
+  F.pool_elems = pool;
+  F.inv_elems = inv;
+  F.sched = batched_dense_schedule(tp_list);
+}
+}  // namespace
+
 void build_plans(const Topology& T, Plans& P) {
   P = Plans();
   const int64_t nt = T.n_tets, ns = T.n_sub, nf = T.n_free;
@@ -412,6 +515,11 @@ void build_plans(const Topology& T, Plans& P) {
     add_tiles(S, 0, schur2);
   }
 
+  P.d_self.assign(T.dvec_elems, -1);
+  for (int64_t k = 0; k < T.dvec_elems; ++k)
+    if (T.d_lam[k] >= 0) P.d_self[k] = int32_t(k);
+  build_faces(T, P);
+
   // ---- bounds validation of every index the kernels will use ------------------
   auto check = [](bool ok, const char* what) {
     if (!ok) throw std::runtime_error(std::string("plan out of range: ") + what);
@@ -444,6 +552,18 @@ void build_plans(const Topology& T, Plans& P) {
     }
   }
   for (int64_t y : P.c_ypos) check(y >= 0 && y < T.y_elems, "coarse y position");
+  {
+    const FacePlan& F = P.faces;
+    for (const TileWork& w : F.sched.items) {
+      check(w.mat >= 0 && w.mat < F.n, "face matrix");
+      const int ta = 2 * F.dev[w.mat].tf;
+      check(w.J <= w.I && w.I < ta && w.k1 <= w.J && w.k0 >= 0, "face tile");
+    }
+    for (const SymTile& t : F.tiles) check(t.s >= 0 && t.s < F.n && t.J <= t.I && t.I < F.dev[t.s].tf, "face sym tile");
+    int64_t nreal = 0;
+    for (int32_t l : F.map_lam) nreal += l >= 0;
+    check(nreal == T.n_lam, "faces cover every multiplier once");
+  }
   for (int64_t v : P.c_vpos) check(v >= 0 && v < T.vec_elems, "coarse v position");
 
   // ---- inverse diagonal tiles: one per factored tile column ---------------------

@@ -42,6 +42,27 @@ struct TileSchedule {
   double flops = 0.0;  // algorithmic flops of the tile updates
 };
 
+// Interface faces for deluxe scaling: the multipliers shared by one subdomain pair
+// (s0 < s1).  Per face F the solver forms A_F = S_s0,FF + S_s1,FF from the two
+// Dirichlet Schur blocks, inverts it by a partial Cholesky of [[A_F, I], [I, 0]]
+// (trailing block -A_F^-1) and keeps B_F = S_s1,FF; the scaled jump operator then
+// applies D_F = A_F^-1 B_F and its transpose through symmetric tile products.
+struct FacePlan {
+  int64_t n = 0;
+  std::vector<FaceDev> dev;
+  std::vector<DevSub> sub;            // per face, for the symmetric-product kernels:
+                                      //   ndp = padded order, dvec_off = y_off = vec_off,
+                                      //   w_off = b_off, p_off = ainv_off, sp_off
+  std::vector<int32_t> map_lam;       // face-vector slot -> multiplier (-1 padding)
+  std::vector<int32_t> map_self;      // face-vector slot -> itself (-1 padding)
+  std::vector<int64_t> slot0, slot1;  // face-vector slot -> D slot in s0 / s1 (-1 padding)
+  std::vector<SymTile> tiles;
+  int64_t vec_elems = 0, op_elems = 0, sp_elems = 0, pool_elems = 0, inv_elems = 0;
+  int max_T = 0;                      // max padded order / 64
+  std::vector<int64_t> rowabs, rowbase, inv_off;
+  TileSchedule sched;                 // eliminates the A_F block of every face
+};
+
 struct Plans {
   NzPlan pool;        // element matrices -> subdomain factor pools (augmented, tiled)
   NzPlan rhs_global;  // element F -> global free-dof RHS
@@ -70,6 +91,8 @@ struct Plans {
   int64_t inv_elems = 0, coarse_inv_elems = 0;
   std::vector<SymTile> sym_tiles;  // every lower tile of every subdomain's Z / P
   int max_Td = 0;                  // max ndp / 64
+  std::vector<int32_t> d_self;     // D slot -> itself (-1 on padding)
+  FacePlan faces;
 };
 
 // Packed index of (a, b), a <= b, in the 21-entry upper triangle of a 6x6.
@@ -82,6 +105,10 @@ void build_plans(const Topology& T, Plans& P);
 // ft (optional): first nonzero tile column of every tile row (envelope).
 TileSchedule single_matrix_schedule(int Tp, int Ta, bool incremental_diagonal,
                                     const int32_t* ft = nullptr);
+
+// Incremental-diagonal schedule of a batch of dense matrices: matrix m has 2 Tp[m]
+// tile rows and its first Tp[m] tile columns are eliminated.
+TileSchedule batched_dense_schedule(const std::vector<int>& Tp);
 
 // Dense lower-triangular tile layout (used for matrices without an envelope).
 inline int64_t tile_offset(int64_t I, int64_t J) { return (I * (I + 1) / 2 + J) * kTileElems; }

@@ -33,6 +33,14 @@ struct DevSub {
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// One interface face (multipliers shared by subdomains s0 < s1) as the deluxe-scaling
+// kernels see it.
+struct FaceDev {
+  int32_t s0, s1, nf, tf;       // subdomains, multipliers, padded order / 64
+  int64_t pool_base, vec_off;   // factor pool (dense lower tiles), face-vector offset
+  int64_t ainv_off, b_off;      // lower-tile stores of A_F^-1 and B_F
+};
+
 // One lower tile (I, J) of subdomain s's symmetric dual operators (Z and P).
 struct SymTile {
   int32_t s, I, J;

@@ -42,6 +42,7 @@ class SolverResult:
     guess_time_ms: float = 0.0
     learned_guess: bool = False
     precond_fp64: bool = False
+    deluxe: bool = False
 
 
 def _dp(a):
@@ -65,7 +66,7 @@ class FETIDPSolver:
 
     def __init__(self, precision="fp64", tolerance=1e-8, dual_tolerance=1e-11,
                  max_iterations=1000, max_refinements=12, profile=False,
-                 inner_tolerance=1e-2, graphs=True, precond_precision="auto"):
+                 inner_tolerance=1e-2, graphs=True, precond_precision="auto", scaling="auto"):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.fetidp_config_default(C.byref(cfg))
@@ -78,6 +79,7 @@ class FETIDPSolver:
         cfg.inner_tolerance = inner_tolerance
         cfg.graphs = 1 if graphs else 0
         cfg.precond_precision = {"auto": 0, "fp32": 1, "fp64": 2}[precond_precision]
+        cfg.scaling = {"auto": 0, "stiffness": 1, "deluxe": 2}[scaling]
         self.precision = precision
         self._h = self._lib.fetidp_create(C.byref(cfg))
         if not self._h:
@@ -182,7 +184,7 @@ class FETIDPSolver:
                             factor_time_ms=st.factor_ms, symbolic_time_ms=st.symbolic_ms,
                             fell_back_to_fp64=bool(st.fell_back), guess_time_ms=st.guess_ms,
                             learned_guess=bool(st.learned_guess),
-                            precond_fp64=bool(st.precond_fp64))
+                            precond_fp64=bool(st.precond_fp64), deluxe=bool(st.deluxe))
 
     # --------------------------------------------------------------- learned lambda0
     def set_lambda0_model(self, model):

@@ -49,6 +49,11 @@ typedef struct fetidp_config {
                                (ignored while event profiling is on) */
   int32_t precond_precision; /* Dirichlet blocks: 0 auto (FP64 when max nu / (min kappa
                                h_min^2) > 1e6, else FP32), 1 FP32, 2 FP64 */
+  int32_t scaling;          /* interface scaling of the preconditioner: 0 auto (deluxe when
+                               kappa/nu varies by more than 1e4 between materials, else
+                               stiffness), 1 stiffness rho_t/(rho_s+rho_t), 2 deluxe
+                               (S_s,FF + S_t,FF)^-1 S_t,FF per face (FP64) */
+  int32_t reserved;
 } fetidp_config;
 
 typedef struct fetidp_stats {
@@ -65,6 +70,8 @@ typedef struct fetidp_stats {
   double guess_ms;          /* learned lambda0: features + network (0 without a model) */
   int32_t learned_guess;    /* 1: PCPG started from the attached model's lambda0 */
   int32_t precond_fp64;     /* 1: the Dirichlet blocks were kept in FP64 */
+  int32_t deluxe;           /* 1: deluxe interface scaling was used */
+  int32_t reserved;
 } fetidp_stats;
 
 typedef struct fetidp_solver* fetidp_handle;

@@ -17,7 +17,8 @@ solver implements, in the plainest dense form, for small meshes:
 * F = sum B Z B^T + sum B Phi R S~^-1 R^T Phi^T B^T,
   d = sum B a_s - sum B Phi R S~^-1 f~_c,  a_s = [K_rr^-1 f_r]_D,
   f~_c = sum R^T (f_c - K_cr K_rr^-1 f_r).
-* Preconditioner M^-1 = sum B_D P B_D^T with stiffness scaling: the row of
+* Preconditioner M^-1 = sum B_D P B_D^T with stiffness scaling (or, with
+  scaling="deluxe", the face matrices D_F = (P_s0,FF + P_s1,FF)^-1 P_s1,FF): the row of
   subdomain s for a multiplier shared with t is weighted rho_t/(rho_s+rho_t),
   rho = diagonal of the local matrix at that edge.
 * PCPG = PCG on F lambda = d from lambda0 (default 0), stop at
@@ -40,7 +41,8 @@ from .solvers import SUCCESS, NUMERICAL_ERROR, DIVERGED, MAX_ITER_REACHED
 
 class DenseFETIDP:
     def __init__(self, xyz, tet2node, tet_material, nu, kappa, tet_subdomain,
-                 pec_pairs, jq):
+                 pec_pairs, jq, scaling="stiffness"):
+        self.scaling = scaling
         asm = assembly.assemble(xyz, tet2node, tet_material, nu, kappa, pec_pairs, jq)
         self.A, self.f = asm["A"], asm["f"]
         l2g, sgn = asm["l2g"], asm["signs"]
@@ -102,6 +104,46 @@ class DenseFETIDP:
             self.subs[s0]["D"][k0] = r1 / (r0 + r1)
             self.subs[s1]["D"][k1] = r0 / (r0 + r1)
         self._factor()
+        if scaling == "deluxe":
+            self._deluxe_setup()
+
+    def _deluxe_setup(self):
+        """Deluxe weights per interface face F = (s0 < s1): D_F = (S0 + S1)^-1 S1 with
+        S_k = P_k restricted to F (Dohrmann & Widlund 2016, "A BDDC algorithm with
+        deluxe scaling for three-dimensional H(curl) problems").  In the dual form the
+        scaled jump of side s0 applies D_F and that of side s1 applies -(I - D_F), so
+        sum_s B_D,s B_s^T = I; with diagonal face blocks this is stiffness scaling
+        with rho = diag(P).  No reference counterpart (the reference has no FETI-DP)."""
+        owners = {}
+        for s, S in enumerate(self.subs):
+            for k, l in enumerate(S["lam"]):
+                owners.setdefault(int(l), []).append((s, k))
+        pairs = {}
+        for l, lst in sorted(owners.items()):
+            (s0, k0), (s1, k1) = sorted(lst)
+            pairs.setdefault((s0, s1), []).append((l, k0, k1))
+        self.faces = []
+        for (s0, s1), items in sorted(pairs.items()):
+            lams = np.array([i[0] for i in items])
+            k0 = np.array([i[1] for i in items])
+            k1 = np.array([i[2] for i in items])
+            S0 = self.subs[s0]["P"][np.ix_(k0, k0)]
+            S1 = self.subs[s1]["P"][np.ix_(k1, k1)]
+            D0 = np.linalg.solve(S0 + S1, S1)
+            self.faces.append((s0, s1, lams, k0, k1, D0))
+
+    def _apply_prec_deluxe(self, r):
+        # x_s0 = D r_F, x_s1 = -(I - D) r_F;  z_F = D^T y_s0 - (I - D)^T y_s1
+        xs = [np.zeros(S["nd"]) for S in self.subs]
+        for (s0, s1, lams, k0, k1, D0) in self.faces:
+            g = D0 @ r[lams]
+            xs[s0][k0] += g
+            xs[s1][k1] += g - r[lams]
+        ys = [S["P"] @ x for S, x in zip(self.subs, xs)]
+        z = np.zeros(self.n_lam)
+        for (s0, s1, lams, k0, k1, D0) in self.faces:
+            z[lams] += D0.T @ (ys[s0][k0] + ys[s1][k1]) - ys[s1][k1]
+        return z
 
     def _factor(self):
         Sc = np.zeros((self.n_c, self.n_c))
@@ -159,6 +201,8 @@ class DenseFETIDP:
         return d
 
     def apply_prec(self, r):
+        if self.scaling == "deluxe":
+            return self._apply_prec_deluxe(r)
         z = np.zeros(self.n_lam)
         for S in self.subs:
             x = S["D"] * S["sgn"] * r[S["lam"]]

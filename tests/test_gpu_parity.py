@@ -49,7 +49,8 @@ def test_global_system_matches_oracle(gpu_available):
 def test_dual_operators_match_oracle(gpu_available, kind, precision):
     p = _problem(kind)
     print(f"[dual ops] {kind} {precision}", flush=True)
-    s = FETIDPSolver(precision=precision).setup_problem(p)
+    # stiffness scaling on both sides (deluxe has its own test below)
+    s = FETIDPSolver(precision=precision, scaling="stiffness").setup_problem(p)
     s.set_rhs_tet_J(p.jq)
     s.factor()
     o = ofetidp.DenseFETIDP(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.tet_subdomain,
@@ -97,9 +98,9 @@ def test_solve_matches_direct(gpu_available, kind, precision):
 
 
 def test_contrast_64_subdomains_matches_oracle_iterations(gpu_available):
-    """Per-subdomain nu/kappa contrast with a low-nu sphere cutting subdomains: the
-    FP64 Dirichlet blocks (chosen automatically: max nu / (min kappa h^2) > 1e6)
-    keep PCPG on the oracle's iteration count; the answer matches the direct solve."""
+    """Per-subdomain nu/kappa contrast with a low-nu sphere cutting subdomains, stiffness
+    scaling: the FP64 Dirichlet blocks (chosen automatically: max nu / (min kappa h^2)
+    > 1e6) keep PCPG on the oracle's iteration count; the answer matches the direct solve."""
     from oracle import fetidp as ofetidp
     p = W.make_problem(16, 4, None, "contrast", seed=7, name="c16")
     print(f"[contrast] 16^3 4^3 subdomains tets={p.n_tets}", flush=True)
@@ -108,7 +109,7 @@ def test_contrast_64_subdomains_matches_oracle_iterations(gpu_available):
     out = F.solve(tol=1e-10, max_iter=2000)
     xd = solvers.direct(F.A, F.f)
     # one PCPG stage to the same dual tolerance (global residual check switched off)
-    s = FETIDPSolver(tolerance=1.0, dual_tolerance=1e-10, max_iterations=2000)
+    s = FETIDPSolver(tolerance=1.0, dual_tolerance=1e-10, max_iterations=2000, scaling="stiffness")
     s.setup_problem(p)
     s.set_rhs_tet_J(p.jq)
     s.factor()
@@ -119,5 +120,47 @@ def test_contrast_64_subdomains_matches_oracle_iterations(gpu_available):
     # the default stop rule reaches the 1e-8 global residual and the direct answer
     s.set_config_values(tolerance=1e-8)
     r = s.solve()
+    assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-8
+    assert np.linalg.norm(r.x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("n,parts", [(8, 2), (12, 3)])
+def test_deluxe_preconditioner_matches_oracle(gpu_available, n, parts):
+    from oracle import fetidp as ofetidp
+    p = W.make_problem(n, parts, None, "contrast", seed=7, name="cd")
+    print(f"[deluxe] M^-1 {n}^3 {parts}^3", flush=True)
+    s = FETIDPSolver(scaling="deluxe")
+    s.setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    F = ofetidp.DenseFETIDP(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.tet_subdomain,
+                            p.pec_pairs, p.jq, scaling="deluxe")
+    rng = np.random.default_rng(3)
+    for _ in range(2):
+        r = rng.normal(size=F.n_lam)
+        z = s.apply_preconditioner(r)
+        zo = F.apply_prec(r)
+        assert np.linalg.norm(z - zo) <= 1e-10 * np.linalg.norm(zo)
+
+
+def test_deluxe_contrast_solve(gpu_available):
+    """Automatic deluxe scaling (kappa/nu spans > 1e4): PCPG count of the oracle's
+    deluxe iteration, 1e-8 residual, direct answer."""
+    from oracle import fetidp as ofetidp
+    p = W.make_problem(16, 4, None, "contrast", seed=7, name="c16")
+    print(f"[deluxe] solve 16^3 4^3 tets={p.n_tets}", flush=True)
+    F = ofetidp.DenseFETIDP(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.tet_subdomain,
+                            p.pec_pairs, p.jq, scaling="deluxe")
+    out = F.solve(tol=1e-10, max_iter=500)
+    s = FETIDPSolver(tolerance=1.0, dual_tolerance=1e-10)
+    s.setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    r = s.solve()
+    assert r.deluxe and r.precond_fp64 and r.status == SolverStatus.SUCCESS
+    assert abs(r.iterations - out["iterations"]) <= 2
+    s.set_config_values(tolerance=1e-8)
+    r = s.solve()
+    xd = solvers.direct(F.A, F.f)
     assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-8
     assert np.linalg.norm(r.x - xd) <= 1e-7 * np.linalg.norm(xd)
