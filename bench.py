@@ -25,7 +25,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE
 # passes over the same workload, each scaled by what the counter reads for a 512 MiB
 # device-to-device copy (FETCH x2.000, WRITE x1.000); the counters cannot be read live.
-MEASURED_TRAFFIC = {("cube48j", "k_tile_update<double,1>"): 1.057e9}
+MEASURED_TRAFFIC = {("cube48j", "k_tile_update<double,1>"): 1.0266e9}
 WORKLOAD_TEXT = {
     "cube48j": "48^3-hex Kuhn cube (663,552 tets, 753,552 free edges), 6x6x6 jittered "
                "subdomains 6-10 hexes wide, nu=1, kappa=1e-2, 2 Gaussian current loops, PEC",
@@ -49,13 +49,15 @@ def parse():
     ap.add_argument("--cpu-sample-iters", type=int, default=600)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-iterations", type=int, default=1000)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the (untimed) launch-variant autotuner")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="copy this many MiB device-to-device once before the variants, so a "
                          "counter profile sees a copy of known size")
     return ap.parse_args()
 
 
-def run_variant(prob, variant, steps, warmup, max_iterations=1000):
+def run_variant(prob, variant, steps, warmup, max_iterations=1000, autotune=True):
     import torch
     from emagddm_amd import FETIDPSolver, SolverStatus
     from emagddm_amd.lambda0 import LambdaMPNN, default_weights
@@ -72,6 +74,10 @@ def run_variant(prob, variant, steps, warmup, max_iterations=1000):
         s.set_lambda0_model(model)
     elif extra:
         raise RuntimeError(f"unknown variant {variant}")
+    tuning = None
+    if autotune:  # untimed, like the symbolic setup: picks the fastest launch variant
+        s.set_rhs_tet_J(prob.jq)
+        tuning = s.autotune(repetitions=1)
     s.set_rhs_tet_J(prob.jq)          # inputs now resident in HBM
     for _ in range(warmup):
         s.factor()
@@ -95,7 +101,7 @@ def run_variant(prob, variant, steps, warmup, max_iterations=1000):
         s.solve()
     prof = s.profile()
     s.set_profile(False)
-    return dict(solver=s, model=model, result=r, ok=ok, s_per_solve=(t1 - t0) / steps,
+    return dict(solver=s, model=model, tuning=tuning, result=r, ok=ok, s_per_solve=(t1 - t0) / steps,
                 factor_ms=r.factor_time_ms, solve_ms=r.solve_time_ms,
                 iterations=r.iterations, refinements=r.refinements,
                 residual=r.residual_norm, profile=prof, steps=steps)
@@ -179,7 +185,7 @@ def main():
     variants = {}
     for prec in a.variants.split(","):
         try:
-            v = run_variant(prob, prec, a.steps, a.warmup, a.max_iterations)
+            v = run_variant(prob, prec, a.steps, a.warmup, a.max_iterations, not a.no_autotune)
         except RuntimeError as e:
             print(f"[bench] {prec}: failed: {e}", file=sys.stderr, flush=True)
             continue
@@ -223,7 +229,11 @@ def main():
                              learned_lambda0=v["result"].learned_guess,
                              precond_fp64=v["result"].precond_fp64,
                              guess_ms=round(v["result"].guess_time_ms, 3),
-                             initial_dual_residual=v["result"].initial_dual_residual)
+                             initial_dual_residual=v["result"].initial_dual_residual,
+                             autotune=(None if v["tuning"] is None else dict(
+                                 chosen=list(min(v["tuning"], key=lambda t: t[3])[:3]),
+                                 variants_ms={f"side{a}_graph{b}_waves{c}": round(d, 2)
+                                              for a, b, c, d in v["tuning"]})))
                      for k, v in variants.items()},
         "speedup_vs_plain_fp64": (round(variants["fp64"]["s_per_solve"] / vb["s_per_solve"], 3)
                                   if "fp64" in variants else None),

@@ -22,7 +22,8 @@ class Config(C.Structure):
                 ("tolerance", C.c_double), ("dual_tolerance", C.c_double),
                 ("max_refinements", C.c_int32), ("verbose", C.c_int32),
                 ("inner_tolerance", C.c_double), ("graphs", C.c_int32), ("precond_precision", C.c_int32),
-                ("scaling", C.c_int32), ("reserved", C.c_int32)]
+                ("scaling", C.c_int32), ("side_stream", C.c_int32), ("tile_waves", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -45,7 +46,7 @@ EXPORTS = [
     "fetidp_get_coarse_matrix", "fetidp_get_profile", "fetidp_reset_profile",
     "fetidp_last_error", "fetidp_destroy", "emag_nedelec_integrate", "emag_tile_cholesky_test",
     "emag_mfma_layout_probe", "fetidp_set_lambda0_model", "fetidp_get_dual_graph",
-    "fetidp_get_lambda_features",
+    "fetidp_get_lambda_features", "fetidp_autotune",
     "emag_mpnn_create", "emag_mpnn_destroy", "emag_mpnn_last_error", "emag_mpnn_num_params",
     "emag_mpnn_get_params", "emag_mpnn_set_params", "emag_mpnn_set_graph", "emag_mpnn_forward",
     "emag_mpnn_gradient", "emag_mpnn_train_step", "emag_mpnn_mfma_probe",
@@ -109,6 +110,7 @@ def load():
     lib.fetidp_get_dual_graph.argtypes = [P, ip, ip, fp]
     lib.fetidp_get_dual_graph.restype = C.c_int64
     lib.fetidp_get_lambda_features.argtypes = [P, fp, dp]
+    lib.fetidp_autotune.argtypes = [P, C.c_int32, dp, C.c_int32]
     lib.emag_mpnn_create.argtypes = [C.c_uint64]
     lib.emag_mpnn_create.restype = P
     lib.emag_mpnn_destroy.argtypes = [P]

@@ -542,7 +542,7 @@ struct fetidp_solver {
     const int slot0 = local ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     bool after_side = false;      // the previous launch was a mode-3 launch on st2
     for (const TileLaunch& L : S.launches) {
-      if (L.mode == 3 && profiling()) {
+      if (L.mode == 3 && (profiling() || !cfg.side_stream)) {
         // profiled runs keep every launch on st, so event timings are per kernel
         timed(slot0 + 2, [&] {
           launch_tile_mode<T>(3, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st);
@@ -570,7 +570,8 @@ struct fetidp_solver {
         continue;
       }
       timed(slot0 + L.mode, [&] {
-        launch_tile_mode<T>(L.mode, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st);
+        launch_tile_mode<T>(L.mode, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st,
+                            cfg.tile_waves == 0 ? 0 : 4);
       });
     }
     HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
@@ -1066,6 +1067,61 @@ struct fetidp_solver {
     return status;
   }
 
+  // Autotuner: time factor + solve for every combination of the launch choices that
+  // leave the arithmetic unchanged (side stream for the diagonal factorizations, PCPG
+  // graph replay, throughput tile kernels capped at 128 registers or allocated freely),
+  // check that
+  // every combination returns bitwise the same solution, keep the fastest.
+  // out[3k..3k+2] = (side_stream, graphs, tile_waves), out[... ] times in milliseconds:
+  // per variant 4 values {side_stream, graphs, tile_waves, ms}.
+  int autotune(int reps, double* out, int cap) {
+    if (!factored && !rhs_set) throw InputError("autotune needs a right-hand side (set_rhs_*) first");
+    reps = std::max(reps, 1);
+    const fetidp_config base = cfg;
+    std::vector<double> ref;
+    double best = 1e300;
+    int best_v = -1, k = 0, nv = 0;
+    for (int v = 0; v < 8; ++v) {
+      cfg.side_stream = (v >> 0) & 1;
+      cfg.graphs = (v >> 1) & 1;
+      cfg.tile_waves = (v >> 2) & 1 ? 0 : 4;
+      fetidp_stats S{};
+      factor();
+      solve(nullptr, nullptr, S);  // warm-up (graph capture, allocations)
+      HIPCHECK(hipStreamSynchronize(st));
+      const double t0 = now_ms();
+      for (int r = 0; r < reps; ++r) {
+        factor();
+        solve(nullptr, nullptr, S);
+      }
+      const double ms = (now_ms() - t0) / reps;
+      std::vector<double> xh(size_t(topo.n_free));
+      HIPCHECK(hipMemcpy(xh.data(), xtot.p, sizeof(double) * xh.size(), hipMemcpyDeviceToHost));
+      if (ref.empty()) ref = xh;
+      else if (std::memcmp(ref.data(), xh.data(), sizeof(double) * xh.size()) != 0) {
+        cfg = base;
+        throw NumericError("autotune: launch variant changed the solution");
+      }
+      if (k + 4 <= cap && out) {
+        out[k++] = cfg.side_stream;
+        out[k++] = cfg.graphs;
+        out[k++] = cfg.tile_waves;
+        out[k++] = ms;
+      }
+      ++nv;
+      if (ms < best) {
+        best = ms;
+        best_v = v;
+      }
+    }
+    cfg = base;
+    cfg.side_stream = (best_v >> 0) & 1;
+    cfg.graphs = (best_v >> 1) & 1;
+    cfg.tile_waves = (best_v >> 2) & 1 ? 0 : 4;
+    factor();
+    return nv;
+  }
+
   template <typename F64, typename F32>
   void with_ops(F64 f64, F32 f32) {
     if (!factored) throw InputError("operators need fetidp_factor first");
@@ -1141,6 +1197,8 @@ void fetidp_config_default(fetidp_config* cfg) {
   cfg->verbose = 0;
   cfg->inner_tolerance = 1e-2;
   cfg->graphs = 1;
+  cfg->side_stream = 1;
+  cfg->tile_waves = 4;
 }
 
 // Copy of `in` with unset fields at their defaults; false for an unknown precision.
@@ -1153,6 +1211,7 @@ static bool normalized_config(const fetidp_config* in, fetidp_config& out) {
   if (!(out.dual_tolerance > 0)) out.dual_tolerance = 1e-11;
   if (out.max_refinements <= 0) out.max_refinements = 12;
   if (!(out.inner_tolerance > 0)) out.inner_tolerance = 1e-2;
+  if (out.tile_waves != 0) out.tile_waves = 4;
   return true;
 }
 
@@ -1439,6 +1498,15 @@ int32_t fetidp_get_lambda_features(fetidp_handle h, float* features, double* sca
     }
     return int32_t(FETIDP_SUCCESS);
   });
+}
+
+int32_t fetidp_autotune(fetidp_handle h, int32_t repetitions, double* timings, int32_t cap) {
+  int32_t n = 0;
+  const int32_t rc = guard(h, [&] {
+    n = h->autotune(repetitions, timings, cap);
+    return int32_t(FETIDP_SUCCESS);
+  });
+  return rc == FETIDP_SUCCESS ? n : -rc;
 }
 
 const char* fetidp_last_error(fetidp_handle h) { return h ? h->err.c_str() : "null handle"; }

@@ -549,6 +549,17 @@ __global__ __launch_bounds__(256, 4) void k_tile_update(const TileWork* __restri
                                                         int* __restrict__ fail) {
   tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail);
 }
+// Same code with the compiler's own register allocation (no occupancy floor): the
+// autotuner's alternative for the throughput modes.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_tile_update_free(const TileWork* __restrict__ work,
+                                                           const int64_t* __restrict__ rowbase,
+                                                           const int64_t* __restrict__ rowabs,
+                                                           const int64_t* __restrict__ inv_base,
+                                                           T* __restrict__ pool, T* __restrict__ inv,
+                                                           int* __restrict__ fail) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail);
+}
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_tile_factor(const TileWork* __restrict__ work,
                                                      const int64_t* __restrict__ rowbase,
@@ -562,9 +573,13 @@ __global__ __launch_bounds__(256) void k_tile_factor(const TileWork* __restrict_
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
                       const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
-                      hipStream_t st) {
+                      hipStream_t st, int waves) {
   if (n <= 0) return;
-  if (mode == 0)
+  if (mode == 0 && waves == 0)
+    hipLaunchKernelGGL((k_tile_update_free<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+  else if (mode == 1 && waves == 0)
+    hipLaunchKernelGGL((k_tile_update_free<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+  else if (mode == 0)
     hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
   else if (mode == 1)
     hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
@@ -574,9 +589,9 @@ void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* row
     hipLaunchKernelGGL((k_tile_factor<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
 }
 template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                       const int64_t*, double*, double*, int*, hipStream_t);
+                                       const int64_t*, double*, double*, int*, hipStream_t, int);
 template void launch_tile_mode<float>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                      const int64_t*, float*, float*, int*, hipStream_t);
+                                      const int64_t*, float*, float*, int*, hipStream_t, int);
 
 // ---------------------------------------------------------------------------
 // Operator extraction after the partial factorization

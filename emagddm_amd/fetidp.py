@@ -94,6 +94,16 @@ class FETIDPSolver:
         self._check(self._lib.fetidp_get_config(self._h, C.byref(cfg)), "fetidp_get_config")
         return cfg
 
+    def autotune(self, repetitions=2):
+        """Time the launch variants (side stream, graph replay, tile-kernel waves) on the
+        current problem, keep the fastest; returns [(side_stream, graphs, waves, ms)]."""
+        buf = np.zeros(64)
+        n = self._lib.fetidp_autotune(self._h, repetitions, _dp(buf), 64)
+        if n < 0:
+            raise RuntimeError(f"fetidp_autotune: {SolverStatus(-n).name}: {self.last_error()}")
+        rows = buf[:4 * n].reshape(n, 4)
+        return [(int(a), int(b), int(c), float(d)) for a, b, c, d in rows]
+
     def set_config_values(self, **fields):
         """Change configuration fields (names of fetidp_config) on the live handle."""
         cfg = self.get_config()

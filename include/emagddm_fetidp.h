@@ -53,6 +53,10 @@ typedef struct fetidp_config {
                                kappa/nu varies by more than 1e4 between materials, else
                                stiffness), 1 stiffness rho_t/(rho_s+rho_t), 2 deluxe
                                (S_s,FF + S_t,FF)^-1 S_t,FF per face (FP64) */
+  int32_t side_stream;      /* 1: diagonal-tile factorizations overlap the column steps on a
+                               second stream (default); 0: one stream */
+  int32_t tile_waves;       /* throughput tile kernels: 4 = capped at 128 registers for four
+                               waves per SIMD (default), 0 = the compiler's own allocation */
   int32_t reserved;
 } fetidp_config;
 
@@ -184,6 +188,15 @@ int64_t fetidp_get_dual_graph(fetidp_handle h, int32_t* row_ptr, int32_t* col, f
  * computed on the GPU from d, z = M^-1 d and the stiffness scaling; scales[0..1]
  * = rms(d), rms(z) (the output scale s_z).  After fetidp_factor. */
 int32_t fetidp_get_lambda_features(fetidp_handle h, float* features, double* scales);
+
+/* Autotuner: times `repetitions` factor + solve runs of the current problem for every
+ * combination of the launch choices that keep the arithmetic identical (side_stream,
+ * graphs, tile_waves), checks that all return bitwise the same solution, and keeps
+ * the fastest in the handle's configuration (then refactors).  timings[4k..4k+3] =
+ * {side_stream, graphs, tile_waves, ms per factor + solve} for variant k (up to cap
+ * values).  Returns the number of variants, or -status on error.  Needs a RHS.
+ * No reference counterpart. */
+int32_t fetidp_autotune(fetidp_handle h, int32_t repetitions, double* timings, int32_t cap);
 
 const char* fetidp_last_error(fetidp_handle h);
 void fetidp_destroy(fetidp_handle h);
