@@ -39,7 +39,7 @@ EXPORTS = [
     "fetidp_config_default", "fetidp_create", "fetidp_set_config", "fetidp_get_config",
     "fetidp_setup", "fetidp_set_materials",
     "fetidp_set_rhs_tet_J", "fetidp_set_rhs_free", "fetidp_factor", "fetidp_solve",
-    "fetidp_get_sizes", "fetidp_get_topology", "fetidp_get_global_nnz",
+    "fetidp_get_sizes", "fetidp_get_choices", "fetidp_get_topology", "fetidp_get_global_nnz",
     "fetidp_get_global_system", "fetidp_apply_dual_operator", "fetidp_apply_preconditioner",
     "fetidp_get_dual_rhs", "fetidp_get_lambda", "fetidp_get_history",
     "fetidp_get_refinement_history",
@@ -84,6 +84,7 @@ def load():
     lib.fetidp_factor.argtypes = [P]
     lib.fetidp_solve.argtypes = [P, dp, dp, C.POINTER(Stats)]
     lib.fetidp_get_sizes.argtypes = [P, lp]
+    lib.fetidp_get_choices.argtypes = [P, ip, ip, dp]
     lib.fetidp_get_topology.argtypes = [P, ip, ip, ip, ip]
     lib.fetidp_get_global_nnz.argtypes = [P]
     lib.fetidp_get_global_nnz.restype = C.c_int64

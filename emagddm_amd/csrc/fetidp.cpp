@@ -141,6 +141,11 @@ struct fetidp_solver {
   DBuf<double> Pd;  // ... or in FP64 when FP32 rounding could make them indefinite
   bool p64 = false;
   double h_min = 0.0;  // shortest mesh edge (conditioning estimate)
+  // Largest overlap of neighbouring subdomains' bounding boxes (thinnest side of the
+  // intersection box) in mean edge lengths: 0 for axis-aligned box partitions (about
+  // 0.5 on a box partition of a mesh whose nodes moved by up to h/4), several for graph
+  // or Voronoi partitions, whose staircase faces stiffness scaling handles badly.
+  double box_overlap = 0.0;
   DBuf<int> fail;
   DBuf<double> cpool, cinv, sinv;
   DBuf<int> cfail;
@@ -359,6 +364,7 @@ struct fetidp_solver {
       }
       h_min = std::min(h_min, std::sqrt(d2));
     }
+    box_overlap = partition_box_overlap(h_xyz, h_t2n, h_sub);
     h_t2n_.assign(h_t2n, h_t2n + 4 * n_tets);
     h_mat_.assign(h_mat, h_mat + n_tets);
     h_nu_.assign(h_nu, h_nu + nmat);
@@ -583,9 +589,44 @@ struct fetidp_solver {
   // Deluxe scaling when the coefficient ratio kappa/nu varies by more than 1e4
   // between materials: diagonal stiffness weights then misjudge the gradient
   // (kappa-only) modes on the faces.  cfg.scaling 1 / 2 forces stiffness / deluxe.
+  double partition_box_overlap(const double* h_xyz, const int32_t* h_t2n, const int32_t* h_sub) const {
+    const int64_t ns = topo.n_sub;
+    std::vector<double> lo(size_t(3 * ns), 1e300), hi(size_t(3 * ns), -1e300);
+    double hsum = 0.0;
+    for (int64_t t = 0; t < topo.n_tets; ++t) {
+      const int64_t s = h_sub[t];
+      for (int v = 0; v < 4; ++v)
+        for (int c = 0; c < 3; ++c) {
+          const double x = h_xyz[3 * int64_t(h_t2n[4 * t + v]) + c];
+          lo[3 * s + c] = std::min(lo[3 * s + c], x);
+          hi[3 * s + c] = std::max(hi[3 * s + c], x);
+        }
+    }
+    for (int64_t e = 0; e < topo.n_edges; ++e) {
+      const int64_t a = topo.edge_nodes[2 * e], b = topo.edge_nodes[2 * e + 1];
+      double d2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        const double d = h_xyz[3 * b + c] - h_xyz[3 * a + c];
+        d2 += d * d;
+      }
+      hsum += std::sqrt(d2);
+    }
+    const double h_mean = hsum / double(std::max<int64_t>(topo.n_edges, 1));
+    double worst = 0.0;
+    for (const FaceDev& f : plans.faces.dev) {
+      double thin = 1e300;
+      for (int c = 0; c < 3; ++c)
+        thin = std::min(thin, std::min(hi[3 * f.s0 + c], hi[3 * f.s1 + c]) -
+                                  std::max(lo[3 * f.s0 + c], lo[3 * f.s1 + c]));
+      worst = std::max(worst, thin);
+    }
+    return h_mean > 0.0 ? worst / h_mean : 0.0;
+  }
+
   bool use_deluxe() const {
     if (cfg.scaling == 1) return false;
     if (cfg.scaling == 2) return true;
+    if (box_overlap > 1.0) return true;  // not a box partition
     double lo = 1e300, hi = 0.0;
     for (int32_t m = 0; m < n_mat; ++m) {
       const double q = h_kappa_[m] / h_nu_[m];
@@ -596,8 +637,8 @@ struct fetidp_solver {
     return hi / lo > 1e4;
   }
 
-  bool precond_fp64() const {
-    if (deluxe) return true;  // the face blocks are formed and inverted in FP64
+  bool precond_fp64(bool dl) const {
+    if (dl) return true;  // the face blocks are formed and inverted in FP64
     if (cfg.precond_precision == 1) return false;
     if (cfg.precond_precision == 2) return true;
     double nu_max = 0.0, ka_min = 1e300;
@@ -690,7 +731,7 @@ struct fetidp_solver {
     const double t0 = now_ms();
     fell_back = false;
     deluxe = use_deluxe() && plans.faces.n > 0;
-    p64 = precond_fp64();
+    p64 = precond_fp64(deluxe);
     if (fp32()) {
       pool64.release(); inv64.release(); W64.release();
       try {
@@ -1231,7 +1272,10 @@ int32_t fetidp_set_config(fetidp_handle h, const fetidp_config* cfg) {
   return guard(h, [&] {
     fetidp_config c;
     if (!cfg || !normalized_config(cfg, c)) throw InputError("set_config: bad configuration");
-    if (c.precision != h->cfg.precision) h->factored = false;  // factors are per precision
+    // factors are per precision; the preconditioner blocks per scaling and precision
+    if (c.precision != h->cfg.precision || c.scaling != h->cfg.scaling ||
+        c.precond_precision != h->cfg.precond_precision)
+      h->factored = false;
     h->cfg = c;
     return int32_t(FETIDP_SUCCESS);
   });
@@ -1311,6 +1355,18 @@ int32_t fetidp_get_sizes(fetidp_handle h, int64_t* out) {
     out[7] = T.w_elems + T.p_elems;
     out[8] = h->plans.coarse_ncp;
     out[9] = h->plans.max_m;
+    return int32_t(FETIDP_SUCCESS);
+  });
+}
+
+int32_t fetidp_get_choices(fetidp_handle h, int32_t* deluxe, int32_t* precond_fp64,
+                           double* box_overlap) {
+  return guard(h, [&] {
+    if (!h->setup_done) throw InputError("get_choices before setup");
+    const bool dl = h->use_deluxe() && h->plans.faces.n > 0;
+    if (deluxe) *deluxe = dl ? 1 : 0;
+    if (precond_fp64) *precond_fp64 = h->precond_fp64(dl) ? 1 : 0;
+    if (box_overlap) *box_overlap = h->box_overlap;
     return int32_t(FETIDP_SUCCESS);
   });
 }

@@ -162,6 +162,15 @@ class FETIDPSolver:
                    prob.tet_subdomain, prob.pec_pairs)
         return self
 
+    def choices(self):
+        """Preconditioner choices resolved from the configuration and the setup (host
+        only): {"deluxe": bool, "precond_fp64": bool, "box_overlap": float}."""
+        dl, p64, ov = C.c_int32(), C.c_int32(), C.c_double()
+        self._check(self._lib.fetidp_get_choices(self._h, C.byref(dl), C.byref(p64), C.byref(ov)),
+                    "fetidp_get_choices")
+        return {"deluxe": bool(dl.value), "precond_fp64": bool(p64.value),
+                "box_overlap": float(ov.value)}
+
     def set_materials(self, nu, kappa):
         nu, kappa = _f64(nu), _f64(kappa)
         self._check(self._lib.fetidp_set_materials(self._h, nu.size, _dp(nu), _dp(kappa)),

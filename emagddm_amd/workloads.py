@@ -257,6 +257,30 @@ def make_problem(n: int, parts: int, jitter=None, materials="uniform", loops=2,
                    cuts=tuple(tuple(w) for w in widths))
 
 
+def irregular_problem(n: int, n_seeds: int, seed: int = 0, perturb: float = 0.25,
+                      name: str = "irregular") -> Problem:
+    """General geometry and partition: interior nodes of the Kuhn cube moved by up to
+    `perturb`·h per coordinate (non-Kuhn element shapes), and each tet assigned to the
+    nearest of `n_seeds` random points (Voronoi cells: staircase faces, subdomain
+    edges along arbitrary lines, corners shared by three or more subdomains)."""
+    base = make_problem(n, 1, seed=seed, name=name)
+    rng = np.random.default_rng(np.random.SeedSequence([seed, n, n_seeds, 7]))
+    xyz = base.xyz.copy()
+    inside = np.all((xyz > 1e-9) & (xyz < 1 - 1e-9), axis=1)
+    xyz[inside] += rng.uniform(-perturb, perturb, size=(int(inside.sum()), 3)) / n
+    cent = xyz[base.tet2node].mean(axis=1)
+    pts = rng.uniform(0, 1, size=(n_seeds, 3))
+    near = np.argmin(((cent[:, None, :] - pts[None]) ** 2).sum(-1), axis=1)
+    _, sub = np.unique(near, return_inverse=True)
+    sub = sub.astype(np.int32)
+    n_sub = int(sub.max()) + 1
+    nu = base.nu
+    return Problem(name=name, n=n, xyz=xyz, tet2node=base.tet2node,
+                   tet_material=base.tet_material, nu=nu, kappa=base.kappa,
+                   tet_subdomain=sub, n_sub=n_sub, pec_pairs=base.pec_pairs, jq=base.jq,
+                   cuts=())
+
+
 def make_workload(name: str, seed: int = 0) -> Problem:
     c = WORKLOADS[name]
     return make_problem(c["n"], c["parts"], c["jitter"], c["materials"], c["loops"],

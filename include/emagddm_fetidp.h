@@ -50,7 +50,9 @@ typedef struct fetidp_config {
   int32_t precond_precision; /* Dirichlet blocks: 0 auto (FP64 when max nu / (min kappa
                                h_min^2) > 1e6, else FP32), 1 FP32, 2 FP64 */
   int32_t scaling;          /* interface scaling of the preconditioner: 0 auto (deluxe when
-                               kappa/nu varies by more than 1e4 between materials, else
+                               kappa/nu varies by more than 1e4 between materials or when
+                               neighbouring subdomains' bounding boxes overlap by more
+                               than one mean edge length, i.e. not a box partition; else
                                stiffness), 1 stiffness rho_t/(rho_s+rho_t), 2 deluxe
                                (S_s,FF + S_t,FF)^-1 S_t,FF per face (FP64) */
   int32_t side_stream;      /* 1: diagonal-tile factorizations overlap the column steps on a
@@ -87,8 +89,9 @@ void fetidp_config_default(fetidp_config* cfg);
 fetidp_handle fetidp_create(const fetidp_config* cfg);
 
 /* Change the configuration of an existing handle (as IterativeSolverConfig is
- * set on a solver, include/numeric/iterative_solvers.h:59-75).  A precision
- * change drops the current factor (call fetidp_factor again). */
+ * set on a solver, include/numeric/iterative_solvers.h:59-75).  A change of
+ * precision, precond_precision or scaling drops the current factor (call
+ * fetidp_factor again). */
 int32_t fetidp_set_config(fetidp_handle h, const fetidp_config* cfg);
 int32_t fetidp_get_config(fetidp_handle h, fetidp_config* cfg);
 
@@ -134,6 +137,15 @@ int32_t fetidp_solve(fetidp_handle h, const double* lambda0, double* x_free, fet
 /* Sizes: out[0..9] = n_edges, n_free, n_lambda, n_primal, n_sub, n_tets,
  * pool elements, operator elements (W+P), coarse padded order, max local order. */
 int32_t fetidp_get_sizes(fetidp_handle h, int64_t* out);
+
+/* Preconditioner choices the current configuration resolves to after setup (host only,
+ * no GPU work): *deluxe = 1 when deluxe scaling will be used, *precond_fp64 = 1 when the
+ * Dirichlet blocks will be stored in FP64, *box_overlap = largest overlap of neighbouring
+ * subdomains' bounding boxes in mean edge lengths (the partition-shape input of the
+ * automatic scaling rule).  Any pointer may be NULL.  No reference counterpart (the
+ * reference has no domain decomposition preconditioner). */
+int32_t fetidp_get_choices(fetidp_handle h, int32_t* deluxe, int32_t* precond_fp64,
+                           double* box_overlap);
 
 /* Topology readback: tet_free[6*n_tets] (free dof per local edge, -1 PEC, in
  * ElementGeometry order), edge_nodes[2*n_edges], lam_free[n_lambda],

@@ -52,6 +52,29 @@ def test_setup_topology_matches_oracle(n, parts, jitter):
     assert s.sizes["n_free"] == nf and s.sizes["n_sub"] == parts ** 3
 
 
+def test_setup_irregular_partition_topology_and_scaling_choice():
+    """Voronoi subdomains on a perturbed mesh: classification is the oracle's, and the
+    automatic rule picks deluxe scaling there and stiffness on box partitions."""
+    p = W.irregular_problem(10, 7, seed=4)
+    s = FETIDPSolver().setup_problem(p)
+    t = s.topology()
+    t2e, pairs = topology.edge_ids(p.tet2node)
+    free = topology.free_numbering(pairs, p.pec_pairs)
+    nf = int(free.max()) + 1
+    mult, _, _ = topology.classify(free[t2e], p.tet_subdomain, nf)
+    assert np.array_equal(t["lam_free"], np.flatnonzero(mult == 2))
+    assert np.array_equal(t["primal_free"], np.flatnonzero(mult >= 3))
+    c = s.choices()
+    assert c["deluxe"] and c["precond_fp64"] and c["box_overlap"] > 2.0
+    s.set_config_values(scaling=1)
+    assert not s.choices()["deluxe"]
+    b = FETIDPSolver().setup_problem(W.make_problem(12, 3, jitter=(3, 5), seed=1))
+    c = b.choices()
+    assert not c["deluxe"] and not c["precond_fp64"] and c["box_overlap"] == 0.0
+    b.set_config_values(scaling=2)
+    assert b.choices()["deluxe"]
+
+
 def test_setup_shuffled_tets_and_unsorted_nodes():
     # non-ascending local node order and shuffled tets: ids follow first appearance
     p = W.make_problem(5, 1, seed=0)

@@ -164,3 +164,29 @@ def test_deluxe_contrast_solve(gpu_available):
     xd = solvers.direct(F.A, F.f)
     assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-8
     assert np.linalg.norm(r.x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+def test_irregular_partition_and_perturbed_mesh(gpu_available):
+    """Perturbed node coordinates and Voronoi subdomains (not boxes).  Stiffness scaling
+    needs hundreds of PCPG iterations here, so the automatic rule picks deluxe scaling
+    (bounding boxes of neighbours overlap by several edge lengths); one PCPG stage takes
+    the oracle's deluxe iteration count, and the default stop rule reaches the direct
+    solution."""
+    p = W.irregular_problem(10, 7, seed=4)
+    print(f"[irregular] tets={p.n_tets} subdomains={p.n_sub}", flush=True)
+    F = ofetidp.DenseFETIDP(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.tet_subdomain,
+                            p.pec_pairs, p.jq, scaling="deluxe")
+    out = F.solve(tol=1e-10, max_iter=500)
+    assert out["status"] == solvers.SUCCESS and out["iterations"] < 40
+    xd = solvers.direct(F.A, F.f)
+    s = FETIDPSolver(tolerance=1.0, dual_tolerance=1e-10)
+    s.setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    r = s.solve()
+    assert r.status == SolverStatus.SUCCESS and r.deluxe
+    assert abs(r.iterations - out["iterations"]) <= 1
+    s.set_config_values(tolerance=1e-9)
+    r = s.solve()
+    assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-9
+    assert np.linalg.norm(r.x - xd) <= 1e-7 * np.linalg.norm(xd)
