@@ -57,6 +57,14 @@ std::string build_topology(const MeshInput& in, Topology& T) {
     max_sub = std::max(max_sub, s);
   }
   T.n_sub = max_sub + 1;
+  {
+    std::vector<char> used(size_t(T.n_sub), 0);
+    for (int64_t t = 0; t < nt; ++t) used[size_t(in.tet_subdomain[t])] = 1;
+    for (int64_t s = 0; s < T.n_sub; ++s)
+      if (!used[size_t(s)])
+        return "subdomain ids must be 0..n_sub-1 with no empty subdomain (id " + std::to_string(s) +
+               " has no tet)";
+  }
 
   // ---- global edge ids in first-appearance order --------------------------
   const size_t N = size_t(nt) * 6;

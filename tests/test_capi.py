@@ -75,6 +75,14 @@ def test_setup_irregular_partition_topology_and_scaling_choice():
     assert b.choices()["deluxe"]
 
 
+def test_setup_rejects_empty_subdomain_ids():
+    p = W.make_problem(6, 2, seed=1)
+    sub = p.tet_subdomain.copy()
+    sub[sub == 3] = 9   # ids 3..8 name no tet
+    with pytest.raises(RuntimeError, match="empty subdomain"):
+        FETIDPSolver().setup(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, sub, p.pec_pairs)
+
+
 def test_setup_shuffled_tets_and_unsorted_nodes():
     # non-ascending local node order and shuffled tets: ids follow first appearance
     p = W.make_problem(5, 1, seed=0)
