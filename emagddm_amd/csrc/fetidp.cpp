@@ -348,7 +348,7 @@ struct fetidp_solver {
     mp_graph_free(lam_graph);
     if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));  // sizes change with the plans
     pcpg_exec = nullptr;
-    MeshInput in{n_nodes, n_tets, h_t2n, h_sub, n_pec, h_pec};
+    MeshInput in{n_nodes, n_tets, h_t2n, h_sub, n_pec, h_pec, h_xyz};
     std::string e = build_topology(in, topo);
     if (!e.empty()) throw InputError("setup: " + e);
     build_plans(topo, plans);

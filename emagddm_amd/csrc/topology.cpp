@@ -2,6 +2,9 @@
 #include "topology.h"
 
 #include <algorithm>
+#include <array>
+#include <climits>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -36,6 +39,91 @@ void radix_sort_pairs(std::vector<uint64_t>& key, std::vector<uint32_t>& val, ui
     }
     key.swap(k2);
     val.swap(v2);
+  }
+}
+
+// Local elimination order inside each subdomain's class blocks [i | D | c].  The
+// tile-envelope Cholesky costs about (rows) x (front width)^2 in the interior block and,
+// for the boundary rows, their span back to the first interior neighbour.  So:
+//  * interior dofs in layers across the longest side of the bounding box of their edge
+//    midpoints (primary key: coordinate along that side, quantized to 1e-6 of the box
+//    diagonal so the layers of a structured mesh stay exact; then the middle side, then
+//    the shortest side, then the free id): the front is one cross-section, the smallest;
+//  * dual dofs by the position of their first interior neighbour, ties by free id, so the
+//    64 rows of a tile start their envelope at about one column.  Primal dofs keep the
+//    global primal order, which the coarse assembly relies on.
+// On the 48^3 / 6^3 jittered-box workload this removes about a third of the interior
+// phase's tile products against free-id order.
+static void order_subdomain_dofs(const double* xyz, Topology& T) {
+  const int64_t ns = T.n_sub, nf = T.n_free;
+  std::vector<int32_t> edge_of_free(size_t(nf), -1);
+  for (int64_t e = 0; e < T.n_edges; ++e)
+    if (T.edge_free[e] >= 0) edge_of_free[size_t(T.edge_free[e])] = int32_t(e);
+  std::vector<int32_t> lpos(size_t(nf), -1);
+  for (int64_t s = 0; s < ns; ++s) {
+    const SubLayout& L = T.sub[s];
+    int32_t* dofs = &T.sub_dof[T.sub_dof_ptr[s]];
+    const int n = L.ni + L.nd + L.nc;
+    // interior: layered order
+    if (L.ni > 1) {
+      std::vector<std::array<double, 3>> mid(size_t(L.ni));
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      for (int k = 0; k < L.ni; ++k) {
+        const int32_t e = edge_of_free[size_t(dofs[k])];
+        const int64_t a = T.edge_nodes[2 * e], b = T.edge_nodes[2 * e + 1];
+        for (int c = 0; c < 3; ++c) {
+          mid[k][c] = 0.5 * (xyz[3 * a + c] + xyz[3 * b + c]);
+          lo[c] = std::min(lo[c], mid[k][c]);
+          hi[c] = std::max(hi[c], mid[k][c]);
+        }
+      }
+      int ax[3] = {0, 1, 2};  // longest side first
+      std::stable_sort(ax, ax + 3, [&](int p, int q) { return hi[p] - lo[p] > hi[q] - lo[q]; });
+      double diag = 0.0;
+      for (int c = 0; c < 3; ++c) diag += (hi[c] - lo[c]) * (hi[c] - lo[c]);
+      const double q = 1e-6 * std::max(std::sqrt(diag), 1e-300);
+      std::vector<int64_t> layer(size_t(L.ni));
+      for (int k = 0; k < L.ni; ++k) layer[k] = int64_t(std::llround((mid[k][ax[0]] - lo[ax[0]]) / q));
+      std::vector<int32_t> perm(size_t(L.ni));
+      std::iota(perm.begin(), perm.end(), 0);
+      std::sort(perm.begin(), perm.end(), [&](int32_t u, int32_t v) {
+        if (layer[u] != layer[v]) return layer[u] < layer[v];
+        if (mid[u][ax[1]] != mid[v][ax[1]]) return mid[u][ax[1]] < mid[v][ax[1]];
+        if (mid[u][ax[2]] != mid[v][ax[2]]) return mid[u][ax[2]] < mid[v][ax[2]];
+        return dofs[u] < dofs[v];
+      });
+      std::vector<int32_t> tmp(dofs, dofs + L.ni);
+      for (int k = 0; k < L.ni; ++k) dofs[k] = tmp[perm[k]];
+    }
+    // dual rows: position of the first interior neighbour (INT32_MAX if none)
+    for (int k = 0; k < n; ++k) lpos[size_t(dofs[k])] = k;
+    std::vector<int32_t> fpos(size_t(n), INT32_MAX);
+    for (int64_t j = T.sub_tet_ptr[s]; j < T.sub_tet_ptr[s + 1]; ++j) {
+      const int64_t t = T.sub_tet[j];
+      for (int a = 0; a < 6; ++a) {
+        const int32_t da = T.tet_free[6 * t + a];
+        if (da < 0) continue;
+        const int32_t ka = lpos[size_t(da)];
+        for (int b = 0; b < 6; ++b) {
+          const int32_t db = T.tet_free[6 * t + b];
+          if (db < 0) continue;
+          const int32_t kb = lpos[size_t(db)];
+          if (kb < L.ni) fpos[size_t(ka)] = std::min(fpos[size_t(ka)], kb);
+        }
+      }
+    }
+    auto order_block = [&](int k0, int k1) {
+      std::vector<int32_t> perm(size_t(k1 - k0));
+      std::iota(perm.begin(), perm.end(), k0);
+      std::stable_sort(perm.begin(), perm.end(), [&](int32_t u, int32_t v) {
+        if (fpos[size_t(u)] != fpos[size_t(v)]) return fpos[size_t(u)] < fpos[size_t(v)];
+        return dofs[u] < dofs[v];
+      });
+      std::vector<int32_t> tmp(dofs + k0, dofs + k1);
+      for (int k = k0; k < k1; ++k) dofs[k] = tmp[size_t(perm[size_t(k - k0)] - k0)];
+    };
+    order_block(L.ni, L.ni + L.nd);  // primal rows stay in global primal order (coarse assembly)
+    for (int k = 0; k < n; ++k) lpos[size_t(dofs[k])] = -1;
   }
 }
 
@@ -199,6 +287,7 @@ std::string build_topology(const MeshInput& in, Topology& T) {
     std::vector<int64_t> pos(T.sub_tet_ptr.begin(), T.sub_tet_ptr.end() - 1);
     for (int64_t t = 0; t < nt; ++t) T.sub_tet[pos[in.tet_subdomain[t]]++] = int32_t(t);
   }
+  if (in.xyz) order_subdomain_dofs(in.xyz, T);
 
   // ---- padded layouts and offsets ------------------------------------------
   int64_t pool = 0, vec = 0, dvec = 0, wel = 0, pel = 0, yel = 0, cvec = 0, spel = 0;

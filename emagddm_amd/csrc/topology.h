@@ -118,6 +118,9 @@ struct MeshInput {
   const int32_t* tet_subdomain;
   int64_t n_pec;
   const int32_t* pec_pairs;
+  const double* xyz = nullptr;  // node coordinates [3 n_nodes]: when given, every subdomain's
+                                // interior dofs are ordered in layers across its longest
+                                // extent (see order_subdomain_dofs)
 };
 
 // Returns empty string on success, else an error message.
