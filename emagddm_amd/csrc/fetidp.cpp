@@ -128,6 +128,10 @@ struct fetidp_solver {
   DBuf<int> ffail;
   DBuf<int32_t> vec_dof;
   DBuf<double> vec_w;
+  DBuf<int64_t> local_vec_off;  // per subdomain: offset of its rows in vec / yvec
+  // vec / yvec hold the forward sweep of the current f_glob (done by the factorization
+  // itself, or by a condensation of f_glob since): condensing f_glob again skips it
+  bool fwd_valid = false;
   DBuf<int32_t> d_lam;
   DBuf<double> d_sgn, d_sgnD, d_half, rho;
   DBuf<int64_t> lam_pos, rho_pos;
@@ -423,6 +427,11 @@ struct fetidp_solver {
       rowabs.upload(plans.rowabs);
       coarse_rowabs.upload(plans.coarse_rowabs);
       local_inv_base.upload(plans.inv_off);
+      {
+        std::vector<int64_t> vo(size_t(topo.n_sub));
+        for (int64_t q = 0; q < topo.n_sub; ++q) vo[size_t(q)] = topo.sub[size_t(q)].vec_off;
+        local_vec_off.upload(vo);
+      }
       ftile.upload(plans.ftile);
       coarse_ftile.upload(plans.coarse_ftile);
       coarse_inv_base.upload(std::vector<int64_t>{0});
@@ -516,6 +525,7 @@ struct fetidp_solver {
     jq.upload(h_jq, size_t(12 * topo.n_tets));
     rhs_from_J = true;
     rhs_set = true;
+    fwd_valid = false;
     if (factored) {
       // same matrix, new source: re-integrate (A_e is rewritten with identical
       // values) and re-assemble f, so no refactorization is needed
@@ -532,6 +542,7 @@ struct fetidp_solver {
     HIPCHECK(hipMemcpy(f_glob.p, h_f, sizeof(double) * topo.n_free, hipMemcpyHostToDevice));
     rhs_from_J = false;
     rhs_set = true;
+    fwd_valid = false;
   }
 
   // ---------------------------------------------------------------- factor
@@ -544,14 +555,14 @@ struct fetidp_solver {
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& rowbase,
                     const DBuf<int64_t>& rabs, const DBuf<int64_t>& inv_base, T* pool, T* inv,
-                    int* failp, bool local) {
+                    int* failp, bool local, RhsArgs rhs = RhsArgs()) {
     const int slot0 = local ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     bool after_side = false;      // the previous launch was a mode-3 launch on st2
     for (const TileLaunch& L : S.launches) {
       if (L.mode == 3 && (profiling() || !cfg.side_stream)) {
         // profiled runs keep every launch on st, so event timings are per kernel
         timed(slot0 + 2, [&] {
-          launch_tile_mode<T>(3, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st);
+          launch_tile_mode<T>(3, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st, 4, rhs);
         });
         continue;
       }
@@ -560,7 +571,7 @@ struct fetidp_solver {
         HIPCHECK(hipStreamWaitEvent(st2, ev_main, 0));
         HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));  // before ev_side moves on to this launch
         timed_on(slot0 + 2, st2, [&] {
-          launch_tile_mode<T>(3, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st2);
+          launch_tile_mode<T>(3, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st2, 4, rhs);
         });
         HIPCHECK(hipEventRecord(ev_side, st2));
         after_side = true;
@@ -577,7 +588,7 @@ struct fetidp_solver {
       }
       timed(slot0 + L.mode, [&] {
         launch_tile_mode<T>(L.mode, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st,
-                            cfg.tile_waves == 0 ? 0 : 4);
+                            cfg.tile_waves == 0 ? 0 : 4, rhs);
       });
     }
     HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
@@ -683,10 +694,14 @@ struct fetidp_solver {
     launch_scaling(topo.dvec_elems, topo.n_lam, rho_pos.p, pool.p, sizeof(T) == 4, rho.p, lam_pos.p,
                    d_sgn.p, d_sgnD.p, d_half.p, st);
     fail.zero(st);
+    // the forward sweep of f_glob rides along with the column steps (RhsArgs)
+    launch_split_rhs(topo.vec_elems, vec_dof.p, vec_w.p, f_glob.p, vec.p, st);
+    fwd_valid = false;
+    const RhsArgs rhs{vec.p, yvec.p, local_vec_off.p};
     if constexpr (sizeof(T) == 8)
-      run_schedule<T>(plans.local_sched, local_items, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, true);
+      run_schedule<T>(plans.local_sched, local_items, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, true, rhs);
     else
-      run_schedule<T>(plans.local_sched_ll, local_items_ll, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, true);
+      run_schedule<T>(plans.local_sched_ll, local_items_ll, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, true, rhs);
     // W extraction (st3) and the coarse problem (st, st2) only read the local
     // factors, so they overlap; st waits for st3 before the end of the factor.
     if (profiling()) {
@@ -715,6 +730,7 @@ struct fetidp_solver {
     for (int s = 0; s < topo.n_sub; ++s)
       if (hf[s]) throw NumericError("subdomain " + std::to_string(s) + ": K_rr not positive definite");
     if (hc) throw NumericError("coarse Schur complement not positive definite");
+    fwd_valid = true;
     if (deluxe) {
       std::vector<int> ff(size_t(plans.faces.n));
       HIPCHECK(hipMemcpy(ff.data(), ffail.p, sizeof(int) * ff.size(), hipMemcpyDeviceToHost));
@@ -842,14 +858,19 @@ struct fetidp_solver {
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgnD.p, dbuf.p, zout, st);
   }
 
-  // condense the global free-dof RHS `fsrc` into the dual RHS dv (and fc, a_s)
+  // condense the global free-dof RHS `fsrc` into the dual RHS dv (and fc, a_s);
+  // is_f: fsrc holds f_glob, whose forward sweep may already be in vec / yvec
   template <typename T>
-  void condense(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, const double* fsrc) {
-    launch_split_rhs(topo.vec_elems, vec_dof.p, vec_w.p, fsrc, vec.p, st);
-    timed(12, [&] {
-      launch_forward<T>(topo.n_sub, plans.max_T, plans.max_Tr, devsub.p, pool.p, rowabs.p, inv.p,
-                        local_inv_base.p, ftile.p, vec.p, yvec.p, st);
-    });
+  void condense(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, const double* fsrc,
+                bool is_f) {
+    if (!(is_f && fwd_valid)) {
+      launch_split_rhs(topo.vec_elems, vec_dof.p, vec_w.p, fsrc, vec.p, st);
+      timed(12, [&] {
+        launch_forward<T>(topo.n_sub, plans.max_T, plans.max_Tr, devsub.p, pool.p, rowabs.p, inv.p,
+                          local_inv_base.p, ftile.p, vec.p, yvec.p, st);
+      });
+      fwd_valid = is_f;
+    }
     launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
     launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, fc.p, uc.p, st);
     // vec_D' holds -a_s: out = a_s - Phi u
@@ -993,7 +1014,7 @@ struct fetidp_solver {
     double dn = 0.0, res = 0.0;
     first_init = -1.0;
     if (!fp32()) {
-      condense<T>(pool, inv, W, f_glob.p);
+      condense<T>(pool, inv, W, f_glob.p, true);
       if (h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
       else if (lam_model) learned_guess();
       else lam.zero(st);
@@ -1017,7 +1038,7 @@ struct fetidp_solver {
       // rglob holds f - K xtot from the last residual evaluation
       int k = 0;
       while (status == FETIDP_SUCCESS && res > cfg.tolerance && k < cfg.max_refinements) {
-        condense<T>(pool, inv, W, rglob.p);
+        condense<T>(pool, inv, W, rglob.p, false);
         lam.zero(st);
         status = pcpg<T>(W, cfg.dual_tolerance, cfg.max_iterations - its, true, its, dn);
         if (status != FETIDP_SUCCESS) break;
@@ -1041,7 +1062,7 @@ struct fetidp_solver {
       res = 1.0;
       double pq_prev = 0.0;
       for (k = 0; k < cfg.max_refinements; ++k) {
-        condense<T>(pool, inv, W, f_cur.p);
+        condense<T>(pool, inv, W, f_cur.p, k == 0);
         if (k == 0 && h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
         else if (k == 0 && lam_model) learned_guess();
         else lam.zero(st);
@@ -1429,8 +1450,8 @@ int32_t fetidp_apply_preconditioner(fetidp_handle h, const double* r, double* z)
 int32_t fetidp_get_dual_rhs(fetidp_handle h, double* d) {
   return guard(h, [&] {
     if (!d) throw InputError("null vector");
-    h->with_ops([&] { h->condense<double>(h->pool64, h->inv64, h->W64, h->f_glob.p); },
-                [&] { h->condense<float>(h->pool32, h->inv32, h->W32, h->f_glob.p); });
+    h->with_ops([&] { h->condense<double>(h->pool64, h->inv64, h->W64, h->f_glob.p, true); },
+                [&] { h->condense<float>(h->pool32, h->inv32, h->W32, h->f_glob.p, true); });
     HIPCHECK(hipStreamSynchronize(h->st));
     HIPCHECK(hipMemcpy(d, h->dv.p, sizeof(double) * h->topo.n_lam, hipMemcpyDeviceToHost));
     return int32_t(FETIDP_SUCCESS);
@@ -1541,8 +1562,8 @@ int64_t fetidp_get_dual_graph(fetidp_handle h, int32_t* row_ptr, int32_t* col, f
 int32_t fetidp_get_lambda_features(fetidp_handle h, float* features, double* scales) {
   return guard(h, [&] {
     if (!features) throw InputError("get_lambda_features: null output");
-    h->with_ops([&] { h->condense<double>(h->pool64, h->inv64, h->W64, h->f_glob.p); },
-                [&] { h->condense<float>(h->pool32, h->inv32, h->W32, h->f_glob.p); });
+    h->with_ops([&] { h->condense<double>(h->pool64, h->inv64, h->W64, h->f_glob.p, true); },
+                [&] { h->condense<float>(h->pool32, h->inv32, h->W32, h->f_glob.p, true); });
     h->lambda_features();
     HIPCHECK(hipStreamSynchronize(h->st));
     HIPCHECK(hipMemcpy(features, h->lam_X.p, sizeof(float) * h->topo.n_lam * kMpF, hipMemcpyDeviceToHost));

@@ -30,10 +30,20 @@ void launch_gather_pool(int64_t n, const int64_t* target, const int64_t* ptr, co
 // the stored inverse diagonal tile + diagonal-tile share); 2: diagonal Cholesky + inverse.
 // Inverse diagonal tile of column J of matrix m: inv + inv_base[m] + J * 4096.
 // Matrix m's tile (I, J) lives at pool + rowabs[rowbase[m] + I] + J * 4096.
+// Optional forward substitution carried by the factorization: with b != nullptr,
+// every diagonal factorization (modes 2, 3) of column J of matrix m writes
+// y_J = L_JJ^-1 b_J to y + off[m] + 64 J, and every column-step tile (modes 1, 3)
+// applies b_I -= L_IJ y_J to b + off[m] + 64 I as soon as L_IJ is formed -- the
+// sweep of launch_forward without reading the factor again.
+struct RhsArgs {
+  double* b = nullptr;
+  double* y = nullptr;
+  const int64_t* off = nullptr;
+};
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
                       const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
-                      hipStream_t st, int waves = 4);
+                      hipStream_t st, int waves = 4, RhsArgs rhs = RhsArgs());
 
 // MFMA lane-layout probe (one wave, 512 doubles of output).
 void launch_mfma_probe(int is_f32, double* out);

@@ -409,6 +409,28 @@ __device__ __forceinline__ void factor_diag_tile(T* s, T* si, T* __restrict__ C,
   }
 }
 
+// Row i of a 64x64 column-major block M (element (i, k) at k*ld + i) times v, in the
+// one summation order every forward-substitution path uses: four 16-term fma chains
+// (k = 16q .. 16q+15), then ((p0 + p1) + p2) + p3.  The factorization's own sweep and
+// the separate launch_forward sweep therefore give bitwise the same vectors.
+template <typename T>
+__device__ __forceinline__ double part_dot16(const T* M, int ld, const double* v, int i, int q) {
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 16 * q; k < 16 * q + 16; ++k) acc = fma((double)M[k * ld + i], v[k], acc);
+  return acc;
+}
+
+// The same product by a whole work-group: wave q forms the chain of part q for all 64
+// rows, wave 0 adds the parts (result valid in wave 0, lane = row).  Call uniformly.
+template <typename T>
+__device__ __forceinline__ double block_row_dot64(const T* M, int ld, const double* v,
+                                                  double (*rpart)[64], int wave, int lane) {
+  rpart[wave][lane] = part_dot16(M, ld, v, lane, wave);
+  __syncthreads();
+  return ((rpart[0][lane] + rpart[1][lane]) + rpart[2][lane]) + rpart[3][lane];
+}
+
 // MODE 0: C_IJ -= sum_K L_IK L_JK^T                                (Schur updates)
 // MODE 1: same, then X = C L_JJ^-T with the stored inverse diagonal tile; with
 //         flags & 1 the diagonal tile of row I also takes its share C_II -= X X^T
@@ -426,7 +448,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
                                             const int64_t* __restrict__ rowabs,
                                             const int64_t* __restrict__ inv_base,
                                             T* __restrict__ pool, T* __restrict__ inv,
-                                            int* __restrict__ fail) {
+                                            int* __restrict__ fail, RhsArgs rhs) {
   using acc_t = typename Mfma<T>::acc_t;
   const TileWork w = work[blockIdx.x];
   // envelope-packed storage: tile (I, J) at ra[I] + J * 4096
@@ -495,6 +517,19 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
             C[j * 64 + i] = a2[x][y][r];
             s[j * kLd + i] = a2[x][y][r];
           }
+      __shared__ double bl[MODE == 3 ? 64 : 1];
+      __shared__ double rpart[4][64];
+      if (rhs.b) {  // uniform: b_I -= L_IJ y_J (fixed summation order, wave 0 writes)
+        __syncthreads();
+        const int64_t o = rhs.off[w.mat];
+        const double dot = block_row_dot64(s, kLd, rhs.y + o + (int64_t)w.J * 64, rpart, wave, lane);
+        if (wave == 0) {
+          double* bi = rhs.b + o + (int64_t)w.I * 64;
+          const double nb = bi[lane] - dot;
+          bi[lane] = nb;
+          if constexpr (MODE == 3) bl[lane] = nb;
+        }
+      }
       if (MODE == 1 && !(w.flags & 1)) return;  // uniform per workgroup
       __syncthreads();
       // C_II -= X X^T
@@ -530,10 +565,22 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
         __syncthreads();
         factor_diag_tile<T>(s, lds_panel, Cd, inv + inv_base[w.mat] + (int64_t)w.I * kTileElems,
                             fail + w.mat, tid, wave, lane);
+        if constexpr (MODE == 3)
+          if (rhs.b) {  // y_I = L_II^-1 b_I (b_I complete in bl)
+            __syncthreads();
+            const double yv = block_row_dot64(lds_panel, kLd, bl, rpart, wave, lane);
+            if (wave == 0) rhs.y[rhs.off[w.mat] + (int64_t)w.I * 64 + lane] = yv;
+          }
       }
     } else {
       factor_diag_tile<T>(s, lds_panel, C, inv + inv_base[w.mat] + (int64_t)w.J * kTileElems,
                           fail + w.mat, tid, wave, lane);
+      if (rhs.b) {  // y_J = L_JJ^-1 b_J (b_J complete in memory)
+        __shared__ double rpart2[4][64];
+        const int64_t o = rhs.off[w.mat] + (int64_t)w.J * 64;
+        const double yv = block_row_dot64(lds_panel, kLd, rhs.b + o, rpart2, wave, lane);
+        if (wave == 0) rhs.y[o + lane] = yv;
+      }
     }
   }
 }
@@ -546,8 +593,8 @@ __global__ __launch_bounds__(256, 4) void k_tile_update(const TileWork* __restri
                                                         const int64_t* __restrict__ rowabs,
                                                         const int64_t* __restrict__ inv_base,
                                                         T* __restrict__ pool, T* __restrict__ inv,
-                                                        int* __restrict__ fail) {
-  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail);
+                                                        int* __restrict__ fail, RhsArgs rhs) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
 }
 // Same code with the compiler's own register allocation (no occupancy floor): the
 // autotuner's alternative for the throughput modes.
@@ -557,8 +604,8 @@ __global__ __launch_bounds__(256) void k_tile_update_free(const TileWork* __rest
                                                            const int64_t* __restrict__ rowabs,
                                                            const int64_t* __restrict__ inv_base,
                                                            T* __restrict__ pool, T* __restrict__ inv,
-                                                           int* __restrict__ fail) {
-  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail);
+                                                           int* __restrict__ fail, RhsArgs rhs) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
 }
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_tile_factor(const TileWork* __restrict__ work,
@@ -566,32 +613,33 @@ __global__ __launch_bounds__(256) void k_tile_factor(const TileWork* __restrict_
                                                      const int64_t* __restrict__ rowabs,
                                                      const int64_t* __restrict__ inv_base,
                                                      T* __restrict__ pool, T* __restrict__ inv,
-                                                     int* __restrict__ fail) {
-  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail);
+                                                     int* __restrict__ fail, RhsArgs rhs) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
 }
 
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
                       const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
-                      hipStream_t st, int waves) {
+                      hipStream_t st, int waves, RhsArgs rhs) {
   if (n <= 0) return;
+  if (mode == 0) rhs = RhsArgs();  // Schur updates do not touch the right-hand side
   if (mode == 0 && waves == 0)
-    hipLaunchKernelGGL((k_tile_update_free<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update_free<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else if (mode == 1 && waves == 0)
-    hipLaunchKernelGGL((k_tile_update_free<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update_free<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else if (mode == 0)
-    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else if (mode == 1)
-    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_tile_factor<T, 2>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_factor<T, 2>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else
-    hipLaunchKernelGGL((k_tile_factor<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail);
+    hipLaunchKernelGGL((k_tile_factor<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
 }
 template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                       const int64_t*, double*, double*, int*, hipStream_t, int);
+                                       const int64_t*, double*, double*, int*, hipStream_t, int, RhsArgs);
 template void launch_tile_mode<float>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                      const int64_t*, float*, float*, int*, hipStream_t, int);
+                                      const int64_t*, float*, float*, int*, hipStream_t, int, RhsArgs);
 
 // ---------------------------------------------------------------------------
 // Operator extraction after the partial factorization
@@ -725,10 +773,7 @@ template <typename T>
 __device__ __forceinline__ void tile_matvec(const T* L, const double* b, double* ys,
                                             double (*part)[64], int tid) {
   const int i = tid & 63, q = tid >> 6;
-  double acc = 0.0;
-#pragma unroll 4
-  for (int k = q * 16; k < q * 16 + 16; ++k) acc += (double)L[k * 64 + i] * b[k];
-  part[q][i] = acc;
+  part[q][i] = part_dot16(L, 64, b, i, q);
   __syncthreads();
   if (tid < 64) ys[tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
   __syncthreads();
@@ -772,10 +817,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ sub
   }
   const T* L = pool + rowabs[S.ft_off + I] + (int64_t)J * kTileElems;
   const int i = tid & 63, q = tid >> 6;
-  double acc = 0.0;
-#pragma unroll 4
-  for (int c = q * 16; c < q * 16 + 16; ++c) acc += (double)L[c * 64 + i] * ys[c];
-  part[q][i] = acc;
+  part[q][i] = part_dot16(L, 64, ys, i, q);
   __syncthreads();
   if (tid < 64) v[I * 64 + tid] -= part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
 }

@@ -190,3 +190,29 @@ def test_irregular_partition_and_perturbed_mesh(gpu_available):
     r = s.solve()
     assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-9
     assert np.linalg.norm(r.x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("kind,precision", [("cube16", "fp64"), ("micro_contrast", "fp64"),
+                                            ("cube16", "fp32")])
+def test_forward_sweep_in_factorization_matches_trsv(gpu_available, kind, precision):
+    """The factorization carries the forward substitution of the right-hand side (each
+    column step applies b_I -= L_IJ y_J as it forms L_IJ).  Both paths sum every row in
+    the same fixed order, so the dual right-hand side and the whole solve are bitwise
+    those of the separate triangular sweeps, which run when the same f is handed over
+    again after the factorization."""
+    p = _problem(kind)
+    print(f"[fused-rhs] {kind} {precision}", flush=True)
+    s = FETIDPSolver(precision=precision)
+    s.setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    d_fused = s.dual_rhs()
+    r_fused = s.solve()
+    _, _, _, f = s.global_system()
+    s.set_rhs_free(f)   # same f: the solve now runs the separate sweeps
+    d_trsv = s.dual_rhs()
+    r_trsv = s.solve()
+    assert np.array_equal(d_fused, d_trsv)
+    assert r_fused.status == r_trsv.status == SolverStatus.SUCCESS
+    assert r_fused.iterations == r_trsv.iterations
+    assert np.array_equal(r_fused.x, r_trsv.x)
