@@ -197,6 +197,11 @@ def main():
                 ((k, v["profile"][k]) for k in v["solver"].PROFILE_SLOTS)}
         print(f"[bench] {prec} per-step event ms (ms, launches): {prof}", file=sys.stderr,
               flush=True)
+        fl = v["profile"].get("flops", {})
+        tfs = {k: round(fl[k] / (v["profile"][k][0] / PROFILE_STEPS * 1e-3) / 1e12, 1)
+               for k in fl if fl[k] > 0 and v["profile"][k][0] > 0}
+        print(f"[bench] {prec} tile classes, achieved TFLOP/s (one stream): {tfs}",
+              file=sys.stderr, flush=True)
         print(f"[bench] {prec} refinement residuals: {v['solver'].refinement_history()}",
               file=sys.stderr, flush=True)
     good = {k: v for k, v in variants.items() if v["ok"]}

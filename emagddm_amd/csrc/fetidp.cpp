@@ -1525,7 +1525,8 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
         if (L.mode == mode || (mode == 2 && L.mode == 3))  // mode 3 is timed with the diagonals
           for (int64_t i = 0; i < L.n; ++i) {
             const TileWork& w = sch[q]->items[L.off + i];
-            fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (L.mode == 1 || L.mode == 3 ? 1.0 + (w.flags & 1) : 0.0));
+            const bool trsm = (L.mode == 1 || L.mode == 3) && !(w.flags & 2);  // not a lookahead
+            fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (trsm ? 1.0 + (w.flags & 1) : 0.0));
           }
       if (k < cap) out[k++] = fl;
     }

@@ -434,7 +434,9 @@ __device__ __forceinline__ double block_row_dot64(const T* M, int ld, const doub
 // MODE 0: C_IJ -= sum_K L_IK L_JK^T                                (Schur updates)
 // MODE 1: same, then X = C L_JJ^-T with the stored inverse diagonal tile; with
 //         flags & 1 the diagonal tile of row I also takes its share C_II -= X X^T
-//         (column step, I > J; every WG of a launch owns a distinct row I)
+//         (column step, I > J; every WG of a launch owns a distinct row I, apart
+//         from a lookahead item, flags & 2, which only updates the next column's
+//         tile (J + 2, J + 1))
 // MODE 2: same (empty K range when the diagonal was updated incrementally), then
 //         Cholesky of the diagonal tile and its inverse
 // MODE 3: MODE 1 for the item I = J + 1, whose share completes C_II, then the
@@ -471,7 +473,8 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
                pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, lds_panel,
                lds_panel + kStage, tid, pb0, qb0, lr, lk);
 
-  if constexpr (MODE == 0) {
+  // MODE 0, and lookahead items (flags & 2) of a column-step launch: update only
+  if (MODE == 0 || (MODE == 1 && (w.flags & 2))) {  // uniform per workgroup
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -481,7 +484,9 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
           const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
           C[j * 64 + i] -= acc[x][y][r];
         }
-  } else {
+    return;
+  }
+  if constexpr (MODE != 0) {
     // staging tile: in mode 1 it reuses the (dead) panel buffers, so the kernel
     // needs 33 KB of LDS and three work-groups fit a CU; modes 2 and 3 keep the
     // panel space for the inverse tile and need their own

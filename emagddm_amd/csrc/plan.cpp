@@ -45,9 +45,10 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
   TileLaunch L{mode, int64_t(S.items.size()), int64_t(list.size())};
   S.items.insert(S.items.end(), list.begin(), list.end());
   S.launches.push_back(L);
-  // modes 1 and 3 add the TRSM product (and the diagonal-tile share when flagged)
+  // modes 1 and 3 add the TRSM product (and the diagonal-tile share when flagged);
+  // lookahead items (flags & 2) are update-only
   for (const TileWork& w : list) {
-    const double extra = (mode == 1 || mode == 3) ? 1.0 + (w.flags & 1) : 0.0;
+    const double extra = (mode == 1 || mode == 3) && !(w.flags & 2) ? 1.0 + (w.flags & 1) : 0.0;
     S.flops += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + extra);
   }
 }
@@ -63,6 +64,10 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
 // its last share, so it runs as a mode-3 launch that also factors that tile,
 // on a second stream beside the mode-1 launch of the other rows (the diagonal
 // launch of column j + 1 disappears).  Per column: [mode 2][mode 3][mode 1].
+// One matrix alone has one mode-3 item per column, on the critical path, whose K loop
+// grows with j; so the mode-1 launch of column j also carries a lookahead item
+// (flags 2: update only) that applies K in [.., j) to tile (j + 2, j + 1), and the
+// mode-3 item of column j + 1 keeps only its last product, K = j.
 TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc, const int32_t* ft) {
   TileSchedule S;
   auto f = [&](int I) { return ft ? ft[I] : 0; };
@@ -72,9 +77,17 @@ TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc, const int32_t* ft)
     std::vector<TileWork> next, below;
     for (int I = j + 1; I < Ta; ++I)
       if (f(I) <= j) {
-        const TileWork w{0, I, j, std::max(f(I), f(j)), j, inc ? 1 : 0};
-        (inc && I == j + 1 && I < Tp ? next : below).push_back(w);
+        TileWork w{0, I, j, std::max(f(I), f(j)), j, inc ? 1 : 0};
+        const bool is_next = inc && I == j + 1 && I < Tp;
+        // the lookahead of column j - 1 applied K < j - 1 to this tile already
+        if (is_next && j >= 1 && f(I) <= j - 1 && w.k0 < j - 1) w.k0 = j - 1;
+        (is_next ? next : below).push_back(w);
       }
+    // lookahead for the mode-3 item (j + 2, j + 1) of the next column
+    if (inc && j + 2 < Tp && f(j + 2) <= j + 1 && f(j + 2) <= j) {
+      const int k0 = std::max(f(j + 2), f(j + 1));
+      if (k0 < j) below.push_back(TileWork{0, j + 2, j + 1, k0, j, 2});
+    }
     add_tiles(S, 3, next);
     add_tiles(S, 1, below);
   }
@@ -550,6 +563,24 @@ void build_plans(const Topology& T, Plans& P) {
       check(w.k0 >= w.k1 || (w.k0 >= P.coarse_ftile[w.I] && w.k0 >= P.coarse_ftile[w.J]),
             "coarse tile panel");
     }
+    // every stored tile (I, J) below the diagonal, J an eliminated column, receives
+    // the K range [max(f_I, f_J), J) exactly once, split between a lookahead item and
+    // its own item at most (both listed in launch order)
+    const int Tp = int(P.coarse_ncp / kTile);
+    std::vector<int32_t> next_k(size_t(Ta) * size_t(Tp), -1);
+    for (const TileLaunch& L : P.coarse_sched.launches)
+      for (int64_t i = L.off; i < L.off + L.n; ++i) {
+        const TileWork& w = P.coarse_sched.items[size_t(i)];
+        if (w.J >= Tp || w.I == w.J || L.mode == 0) continue;
+        int32_t& nk = next_k[size_t(w.I) * Tp + w.J];
+        const int32_t start = std::max(P.coarse_ftile[w.I], P.coarse_ftile[w.J]);
+        if (nk < 0) nk = start;
+        check(w.k0 == nk || w.k0 >= w.k1, "coarse K ranges contiguous");
+        if (w.k1 > w.k0) nk = w.k1;
+      }
+    for (int I = 0; I < Ta; ++I)
+      for (int J = P.coarse_ftile[I]; J < std::min(I, Tp); ++J)
+        check(next_k[size_t(I) * Tp + J] == J, "coarse K ranges complete");
   }
   for (int64_t y : P.c_ypos) check(y >= 0 && y < T.y_elems, "coarse y position");
   {
