@@ -756,6 +756,7 @@ struct fetidp_solver {
         // FP32 factor lost positive definiteness: refactor in FP64 (solve follows)
         fell_back = true;
         fallback_reason = e.what();
+        if (cfg.verbose & 1) std::fprintf(stderr, "[fetidp] FP32 factor fell back to FP64: %s\n", e.what());
         pool32.release(); inv32.release(); W32.release();
         factor_impl<double>(pool64, inv64, W64);
       }
@@ -1517,7 +1518,8 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
     out[k++] = h->prof_ms[i];
     out[k++] = h->prof_cnt[i];
   }
-  const TileSchedule* sch[2] = {&h->plans.local_sched, &h->plans.coarse_sched};
+  const TileSchedule* sch[2] = {h->fp32() ? &h->plans.local_sched_ll : &h->plans.local_sched,
+                                &h->plans.coarse_sched};
   for (int q = 0; q < 2; ++q)
     for (int mode = 0; mode < 3; ++mode) {
       double fl = 0.0;

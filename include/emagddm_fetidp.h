@@ -43,7 +43,9 @@ typedef struct fetidp_config {
   double tolerance;         /* target ||f - K u|| / ||f|| on the assembled free-dof system */
   double dual_tolerance;    /* first PCPG stop ||r||/||d||; tightened x0.1 until tolerance holds */
   int32_t max_refinements;  /* FP32 mode: outer FP64 flexible-CG steps */
-  int32_t verbose;          /* bit 1 (value 2): record per-kernel-class event timings */
+  int32_t verbose;          /* bit 0 (value 1): report an FP32-to-FP64 factor fallback and
+                               its reason on stderr; bit 1 (value 2): record
+                               per-kernel-class event timings */
   double inner_tolerance;   /* FP32 mode: PCPG stop ||r||/||d|| of each refinement step */
   int32_t graphs;           /* 1: replay each PCPG iteration as one captured HIP graph
                                (ignored while event profiling is on) */
