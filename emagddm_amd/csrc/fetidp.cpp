@@ -152,6 +152,7 @@ struct fetidp_solver {
   double box_overlap = 0.0;
   DBuf<int> fail;
   DBuf<double> cpool, cinv, sinv;
+  int64_t sinv_ld() const { return (topo.n_c + 1) / 2 * 2; }  // even row stride of S~^-1
   DBuf<int> cfail;
   DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc, yvec, tbuf;
   DBuf<double> lam, r, z, p, q, dv, ubar, x, xtot, rglob, partial, pf, qf;
@@ -498,7 +499,7 @@ struct fetidp_solver {
     }
     cpool.alloc(plans.coarse_pool_elems);
     cinv.alloc(plans.coarse_inv_elems);
-    sinv.alloc(size_t(topo.n_c) * size_t(std::max<int64_t>(topo.n_c, 1)));
+    sinv.alloc(size_t(topo.n_c) * size_t(std::max<int64_t>(sinv_ld(), 2)));
     HIPCHECK(hipStreamSynchronize(st));
     device_ready = true;
     upload_ms = now_ms() - t0;
@@ -718,7 +719,7 @@ struct fetidp_solver {
                           pool.p, cpool.p, st);
     cfail.zero(st);
     run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, false);
-    launch_extract_sinv(topo.n_c, plans.coarse_ncp, cpool.p, coarse_rowabs.p, coarse_ftile.p, sinv.p, st);
+    launch_extract_sinv(topo.n_c, plans.coarse_ncp, sinv_ld(), cpool.p, coarse_rowabs.p, coarse_ftile.p, sinv.p, st);
     if (deluxe) deluxe_setup();
     if (!profiling()) HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
     HIPCHECK(hipStreamSynchronize(st));
@@ -790,7 +791,7 @@ struct fetidp_solver {
   template <typename T>
   void apply_F_rest(const DBuf<T>& W, double* qout) {
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
-    launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, gc.p, uc.p, st);
+    launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
@@ -873,7 +874,7 @@ struct fetidp_solver {
       fwd_valid = is_f;
     }
     launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
-    launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, fc.p, uc.p, st);
+    launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, fc.p, uc.p, st);
     // vec_D' holds -a_s: out = a_s - Phi u
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, dv.p, st);
@@ -884,7 +885,7 @@ struct fetidp_solver {
   void recover(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, double* xout) {
     apply_W<T>(W, lam.p);
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
-    launch_dense_gemv(topo.n_c, topo.n_c, sinv.p, gc.p, uc.p, st);
+    launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);

@@ -751,21 +751,23 @@ template void launch_extract_W<double>(int, int, int, const DevSub*, const doubl
 template void launch_extract_W<float>(int, int, int, const DevSub*, const float*, const int64_t*,
                                       const int32_t*, float*, hipStream_t);
 
-__global__ __launch_bounds__(256) void k_extract_sinv(int64_t nc, int64_t ncp,
+// S~^-1 as a dense row-major nc x nc matrix with an even row stride ld (16-byte rows);
+// columns nc .. ld-1 are zero.
+__global__ __launch_bounds__(256) void k_extract_sinv(int64_t nc, int64_t ncp, int64_t ld,
                                                       const double* __restrict__ cpool,
                                                       const int64_t* __restrict__ rowabs,
                                                       const int32_t* __restrict__ ftile,
                                                       double* __restrict__ sinv) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nc * nc) return;
-  const int a = int(e / nc), b = int(e % nc);
-  sinv[e] = -pool_sym(cpool, rowabs, ftile, int(ncp) + a, int(ncp) + b);
+  if (e >= nc * ld) return;
+  const int a = int(e / ld), b = int(e % ld);
+  sinv[e] = b < nc ? -pool_sym(cpool, rowabs, ftile, int(ncp) + a, int(ncp) + b) : 0.0;
 }
 
-void launch_extract_sinv(int64_t nc, int64_t ncp, const double* cpool, const int64_t* rowabs,
-                         const int32_t* ftile, double* sinv, hipStream_t st) {
+void launch_extract_sinv(int64_t nc, int64_t ncp, int64_t ld, const double* cpool,
+                         const int64_t* rowabs, const int32_t* ftile, double* sinv, hipStream_t st) {
   if (nc <= 0) return;
-  hipLaunchKernelGGL(k_extract_sinv, dim3((nc * nc + 255) / 256), dim3(256), 0, st, nc, ncp, cpool,
+  hipLaunchKernelGGL(k_extract_sinv, dim3((nc * ld + 255) / 256), dim3(256), 0, st, nc, ncp, ld, cpool,
                      rowabs, ftile, sinv);
 }
 
@@ -1061,6 +1063,11 @@ __global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ t
   const SymTile tl = tiles[blockIdx.x];
   const DevSub S = subs[tl.s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // the tile's loads go out first; the indirect gather of x (d_lam, then v) overlaps them
+  const T* A = M + (which == 0 ? S.w_off : S.p_off) + ((int64_t)tl.I * (tl.I + 1) / 2 + tl.J) * kTileElems;
+  T at[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) at[q] = A[(16 * wv + q) * 64 + lane];
   if (tid < 128) {
     const int blk = (tid >> 6) ? tl.I : tl.J;
     const int64_t slot = S.dvec_off + blk * 64 + lane;
@@ -1070,13 +1077,12 @@ __global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ t
     else xJ[lane] = xv;
   }
   __syncthreads();
-  const T* A = M + (which == 0 ? S.w_off : S.p_off) + ((int64_t)tl.I * (tl.I + 1) / 2 + tl.J) * kTileElems;
   const double xc = xJ[lane];
   double cs = 0.0;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int r = 16 * wv + q;
-    const double a = (double)A[r * 64 + lane];
+    const double a = (double)at[q];
     prod[r * 65 + lane] = a * xc;
     cs += a * xI[r];
   }
@@ -1201,24 +1207,56 @@ void launch_coarse_gather(int64_t nc, const int64_t* ptr, const int64_t* pos, co
                      add, out);
 }
 
+// y = A x for a dense row-major n x n matrix (ld = row stride, a multiple of 2): one wave
+// per two rows, 16-byte loads, two independent accumulators per row; the fixed order
+// of the lane partials and the shuffle tree keeps the result bitwise repeatable.
 __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, int64_t ld, const double* __restrict__ A,
                                                     const double* __restrict__ x,
                                                     double* __restrict__ y) {
-  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t row = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 2;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const double* a = A + row * ld;
-  double s = 0.0;
-  for (int64_t k = lane; k < n; k += 64) s += a[k] * x[k];
+  const bool two = row + 1 < n;
+  const double2* a0 = reinterpret_cast<const double2*>(A + row * ld);
+  const double2* a1 = reinterpret_cast<const double2*>(A + (two ? row + 1 : row) * ld);
+  const double2* x2 = reinterpret_cast<const double2*>(x);
+  const int64_t n2 = n / 2;
+  double s0 = 0.0, s1 = 0.0, t0 = 0.0, t1 = 0.0;
+  int64_t k = lane;
+  for (; k + 64 < n2; k += 128) {
+    const double2 u0 = a0[k], u1 = a1[k], v = x2[k];
+    const double2 w0 = a0[k + 64], w1 = a1[k + 64], z = x2[k + 64];
+    s0 = fma(u0.x, v.x, s0); s0 = fma(u0.y, v.y, s0);
+    s1 = fma(u1.x, v.x, s1); s1 = fma(u1.y, v.y, s1);
+    t0 = fma(w0.x, z.x, t0); t0 = fma(w0.y, z.y, t0);
+    t1 = fma(w1.x, z.x, t1); t1 = fma(w1.y, z.y, t1);
+  }
+  if (k < n2) {
+    const double2 u0 = a0[k], u1 = a1[k], v = x2[k];
+    s0 = fma(u0.x, v.x, s0); s0 = fma(u0.y, v.y, s0);
+    s1 = fma(u1.x, v.x, s1); s1 = fma(u1.y, v.y, s1);
+  }
+  if ((n & 1) && lane == 0) {  // odd order: the last column
+    s0 = fma(A[row * ld + n - 1], x[n - 1], s0);
+    s1 = fma(A[(two ? row + 1 : row) * ld + n - 1], x[n - 1], s1);
+  }
+  s0 += t0;
+  s1 += t1;
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) y[row] = s;
+  for (int o = 32; o >= 1; o >>= 1) {
+    s0 += __shfl_xor(s0, o);
+    s1 += __shfl_xor(s1, o);
+  }
+  if (lane == 0) {
+    y[row] = s0;
+    if (two) y[row + 1] = s1;
+  }
 }
 
 void launch_dense_gemv(int64_t n, int64_t ld, const double* A, const double* x, double* y,
                        hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_dense_gemv, dim3((n + 3) / 4), dim3(256), 0, st, n, ld, A, x, y);
+  hipLaunchKernelGGL(k_dense_gemv, dim3((n + 7) / 8), dim3(256), 0, st, n, ld, A, x, y);
 }
 
 __global__ __launch_bounds__(256) void k_fill_U(const DevSub* __restrict__ subs,
