@@ -582,8 +582,9 @@ struct fetidp_solver {
       after_side = false;
       if (L.mode < 0) {
         timed(5, [&] {
-          if (p64) launch_extract_P<T, double>(topo.n_sub, plans.max_ndp, devsub.p, pool, rowabs.p, ftile.p, Pd.p, st);
-          else launch_extract_P<T, float>(topo.n_sub, plans.max_ndp, devsub.p, pool, rowabs.p, ftile.p, Pf.p, st);
+          const int64_t nt = int64_t(plans.sym_tiles.size());
+          if (p64) launch_extract_P<T, double>(nt, sym_tiles.p, devsub.p, pool, rowabs.p, ftile.p, Pd.p, st);
+          else launch_extract_P<T, float>(nt, sym_tiles.p, devsub.p, pool, rowabs.p, ftile.p, Pf.p, st);
         });
         continue;
       }
@@ -706,11 +707,11 @@ struct fetidp_solver {
     // W extraction (st3) and the coarse problem (st, st2) only read the local
     // factors, so they overlap; st waits for st3 before the end of the factor.
     if (profiling()) {
-      timed(5, [&] { launch_extract_W<T>(topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st); });
+      timed(5, [&] { launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st); });
     } else {
       HIPCHECK(hipEventRecord(ev_main, st));
       HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
-      launch_extract_W<T>(topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st3);
+      launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st3);
       HIPCHECK(hipEventRecord(ev_extract, st3));
     }
     // coarse problem in FP64

@@ -48,12 +48,16 @@ void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* row
 // MFMA lane-layout probe (one wave, 512 doubles of output).
 void launch_mfma_probe(int is_f32, double* out);
 
+// P tiles (the D block of the pool) and W = [Z tiles, Phi^T rows] (the D' block and
+// the (D', c) block, negated) from the partially factored pools; `tiles` lists every lower
+// tile of every subdomain's D x D store.
 template <typename T, typename TP>
-void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, const int64_t* rowabs,
-                      const int32_t* ftile, TP* P, hipStream_t st);
+void launch_extract_P(int64_t ntiles, const SymTile* tiles, const DevSub* subs, const T* pool,
+                      const int64_t* rowabs, const int32_t* ftile, TP* P, hipStream_t st);
 template <typename T>
-void launch_extract_W(int ns, int max_ncp, int max_ndp, const DevSub* subs, const T* pool,
-                      const int64_t* rowabs, const int32_t* ftile, T* W, hipStream_t st);
+void launch_extract_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp, int max_ndp,
+                      const DevSub* subs, const T* pool, const int64_t* rowabs, const int32_t* ftile,
+                      T* W, hipStream_t st);
 // S~^-1 (dense, row stride ld, even) from the trailing block of the coarse pool.
 void launch_extract_sinv(int64_t nc, int64_t ncp, int64_t ld, const double* cpool,
                          const int64_t* rowabs, const int32_t* ftile, double* sinv, hipStream_t st);

@@ -678,81 +678,98 @@ __device__ __forceinline__ void sym_tile_rc(int64_t e, int& row, int& col) {
   col = J * 64 + (w & 63);
 }
 
+// Lower 64x64 tiles of a symmetric block of the factor pool (the D block for P, the D'
+// block for Z), copied tile by tile into a lower-tile store (row-major inside a tile):
+// one work-group per stored tile, coalesced reads of the column-major pool tile, the
+// transpose through LDS; diagonal tiles take their lower half; tiles outside the
+// envelope are zero.  out tile (I, J) = sign * pool(o + 64 I .., o + 64 J ..).
 template <typename T, typename TP>
-__global__ __launch_bounds__(256) void k_extract_P(const DevSub* __restrict__ subs,
-                                                   const T* __restrict__ pool,
-                                                   const int64_t* __restrict__ rowabs,
-                                                   const int32_t* __restrict__ ftile,
-                                                   TP* __restrict__ P) {
-  const DevSub S = subs[blockIdx.y];
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= sym_elems(S.ndp)) return;
-  int a, b;
-  sym_tile_rc(e, a, b);
-  P[S.p_off + e] = (TP)pool_sym(pool, rowabs + S.ft_off, ftile + S.ft_off, S.o_d + a, S.o_d + b);
-}
-
-// The Dirichlet preconditioner blocks are stored in FP32 (half the bytes per
-// PCPG iteration; they only steer PCPG) unless the problem is so ill-conditioned
-// that FP32 rounding could make them indefinite; then in FP64.
-template <typename T, typename TP>
-void launch_extract_P(int ns, int max_ndp, const DevSub* subs, const T* pool, const int64_t* rowabs,
-                      const int32_t* ftile, TP* P, hipStream_t st) {
-  const int64_t n = sym_tile_elems(max_ndp);
-  if (n <= 0 || ns <= 0) return;
-  hipLaunchKernelGGL((k_extract_P<T, TP>), dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool,
-                     rowabs, ftile, P);
-}
-template void launch_extract_P<double, float>(int, int, const DevSub*, const double*, const int64_t*,
-                                              const int32_t*, float*, hipStream_t);
-template void launch_extract_P<float, float>(int, int, const DevSub*, const float*, const int64_t*,
-                                             const int32_t*, float*, hipStream_t);
-template void launch_extract_P<double, double>(int, int, const DevSub*, const double*, const int64_t*,
-                                               const int32_t*, double*, hipStream_t);
-template void launch_extract_P<float, double>(int, int, const DevSub*, const float*, const int64_t*,
-                                              const int32_t*, double*, hipStream_t);
-
-// W = Z (lower tiles) then Phi^T from the trailing block: Z = -A(D',D'), Phi = -A(D',c)
-template <typename T>
-__global__ __launch_bounds__(256) void k_extract_W(const DevSub* __restrict__ subs,
-                                                   const T* __restrict__ pool,
-                                                   const int64_t* __restrict__ rowabs,
-                                                   const int32_t* __restrict__ ftile,
-                                                   T* __restrict__ W) {
-  const DevSub S = subs[blockIdx.y];
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t zt = sym_elems(S.ndp);
-  if (e >= zt + (int64_t)S.ncp * S.ndp) return;
-  const int64_t* ra = rowabs + S.ft_off;
-  const int32_t* ft = ftile + S.ft_off;
-  T v;
-  if (e < zt) {  // Z = -A(D', D') as lower tiles
-    int a, b;
-    sym_tile_rc(e, a, b);
-    v = -pool_sym(pool, ra, ft, S.o_a + a, S.o_a + b);
-  } else {       // Phi^T = -A(c, D') rows, row-major ncp x ndp
-    const int64_t f = e - zt;
-    const int row = int(f / S.ndp), k = int(f % S.ndp);
-    v = -pool_sym(pool, ra, ft, S.o_a + k, S.o_c + row);
+__global__ __launch_bounds__(256) void k_extract_sym(const SymTile* __restrict__ tiles,
+                                                     const DevSub* __restrict__ subs,
+                                                     const T* __restrict__ pool,
+                                                     const int64_t* __restrict__ rowabs,
+                                                     const int32_t* __restrict__ ftile, int block,
+                                                     double sign, TP* __restrict__ out) {
+  const SymTile tl = tiles[blockIdx.x];
+  const DevSub S = subs[tl.s];
+  const int o = (block == 0 ? S.o_d : S.o_a) >> 6;
+  const int I = o + tl.I, J = o + tl.J;
+  TP* dst = out + (block == 0 ? S.p_off : S.w_off) + ((int64_t)tl.I * (tl.I + 1) / 2 + tl.J) * kTileElems;
+  const int tid = threadIdx.x;
+  if (J < ftile[S.ft_off + I]) {  // outside the envelope (uniform per work-group)
+    for (int e = tid; e < kTileElems; e += 256) dst[e] = TP(0);
+    return;
   }
-  W[S.w_off + e] = v;
+  __shared__ T t[64 * 65];
+  const T* src = pool + rowabs[S.ft_off + I] + (int64_t)J * kTileElems;
+  for (int e = tid; e < kTileElems; e += 256) t[(e >> 6) * 65 + (e & 63)] = src[e];  // t[c][r] = (r, c)
+  __syncthreads();
+  const bool diag = I == J;
+  for (int e = tid; e < kTileElems; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const T v = (!diag || r >= c) ? t[c * 65 + r] : t[r * 65 + c];
+    dst[e] = (TP)(sign * (double)v);
+  }
 }
+
+// Phi^T = -A(c, D') as rows of n_cp x n_Dp (row-major) after Z in W: one work-group per
+// 64 x 64 block; the pool tile (D' tile row, c tile column) is read coalesced and written
+// in the same order (Phi^T row = pool column).
+template <typename T>
+__global__ __launch_bounds__(256) void k_extract_phi(const DevSub* __restrict__ subs,
+                                                     const T* __restrict__ pool,
+                                                     const int64_t* __restrict__ rowabs,
+                                                     const int32_t* __restrict__ ftile,
+                                                     T* __restrict__ W) {
+  const DevSub S = subs[blockIdx.y];
+  const int tc = S.ncp >> 6, td = S.ndp >> 6;
+  const int b = blockIdx.x;
+  if (b >= tc * td) return;
+  const int cI = b / td, kJ = b % td;
+  const int I = (S.o_a >> 6) + kJ, J = (S.o_c >> 6) + cI;  // pool tile (D' row, c column)
+  T* dst = W + S.w_off + sym_elems(S.ndp) + (int64_t)(cI * 64) * S.ndp + kJ * 64;
+  const int tid = threadIdx.x;
+  const bool in = J >= ftile[S.ft_off + I];
+  const T* src = pool + rowabs[S.ft_off + I] + (int64_t)J * kTileElems;
+  for (int e = tid; e < kTileElems; e += 256) {
+    const int c = e >> 6, r = e & 63;  // pool element (r, c) of the tile at c*64 + r
+    dst[(int64_t)c * S.ndp + r] = in ? -src[e] : T(0);
+  }
+}
+
+template <typename T, typename TP>
+void launch_extract_P(int64_t ntiles, const SymTile* tiles, const DevSub* subs, const T* pool,
+                      const int64_t* rowabs, const int32_t* ftile, TP* P, hipStream_t st) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL((k_extract_sym<T, TP>), dim3(ntiles), dim3(256), 0, st, tiles, subs, pool, rowabs,
+                     ftile, 0, 1.0, P);
+}
+template void launch_extract_P<double, float>(int64_t, const SymTile*, const DevSub*, const double*,
+                                              const int64_t*, const int32_t*, float*, hipStream_t);
+template void launch_extract_P<float, float>(int64_t, const SymTile*, const DevSub*, const float*,
+                                             const int64_t*, const int32_t*, float*, hipStream_t);
+template void launch_extract_P<double, double>(int64_t, const SymTile*, const DevSub*, const double*,
+                                               const int64_t*, const int32_t*, double*, hipStream_t);
+template void launch_extract_P<float, double>(int64_t, const SymTile*, const DevSub*, const float*,
+                                              const int64_t*, const int32_t*, double*, hipStream_t);
 
 template <typename T>
-void launch_extract_W(int ns, int max_ncp, int max_ndp, const DevSub* subs, const T* pool,
-                      const int64_t* rowabs, const int32_t* ftile, T* W, hipStream_t st) {
-  const int64_t n = sym_tile_elems(max_ndp) + (int64_t)max_ncp * max_ndp;
-  if (n <= 0 || ns <= 0) return;
-  hipLaunchKernelGGL(k_extract_W<T>, dim3((n + 255) / 256, ns), dim3(256), 0, st, subs, pool, rowabs,
-                     ftile, W);
+void launch_extract_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp, int max_ndp,
+                      const DevSub* subs, const T* pool, const int64_t* rowabs, const int32_t* ftile,
+                      T* W, hipStream_t st) {
+  if (ntiles > 0)
+    hipLaunchKernelGGL((k_extract_sym<T, T>), dim3(ntiles), dim3(256), 0, st, tiles, subs, pool, rowabs,
+                       ftile, 1, -1.0, W);
+  const int nb = (max_ncp >> 6) * (max_ndp >> 6);
+  if (nb > 0 && ns > 0)
+    hipLaunchKernelGGL(k_extract_phi<T>, dim3(nb, ns), dim3(256), 0, st, subs, pool, rowabs, ftile, W);
 }
-template void launch_extract_W<double>(int, int, int, const DevSub*, const double*, const int64_t*,
-                                       const int32_t*, double*, hipStream_t);
-template void launch_extract_W<float>(int, int, int, const DevSub*, const float*, const int64_t*,
-                                      const int32_t*, float*, hipStream_t);
+template void launch_extract_W<double>(int64_t, const SymTile*, int, int, int, const DevSub*,
+                                       const double*, const int64_t*, const int32_t*, double*,
+                                       hipStream_t);
+template void launch_extract_W<float>(int64_t, const SymTile*, int, int, int, const DevSub*,
+                                      const float*, const int64_t*, const int32_t*, float*, hipStream_t);
 
-// S~^-1 as a dense row-major nc x nc matrix with an even row stride ld (16-byte rows);
-// columns nc .. ld-1 are zero.
 __global__ __launch_bounds__(256) void k_extract_sinv(int64_t nc, int64_t ncp, int64_t ld,
                                                       const double* __restrict__ cpool,
                                                       const int64_t* __restrict__ rowabs,
