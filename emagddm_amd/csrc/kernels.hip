@@ -1149,6 +1149,9 @@ template void launch_symv<float>(int64_t, const SymTile*, int, int, const DevSub
                                  const int32_t*, const double*, const double*, double*, double*,
                                  hipStream_t);
 
+// out_D = alpha y_D + beta Phi u_c(s) + gamma vec_D' per subdomain.  One work-group per
+// 64 D slots: wave q sums the primal rows c = q, q+4, ... (Phi^T rows, coalesced over the
+// slots), then the four partials are added in fixed order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__ subs,
                                                       const T* __restrict__ W,
@@ -1159,18 +1162,30 @@ __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__
                                                       double beta, double gamma,
                                                       double* __restrict__ out) {
   const DevSub S = subs[blockIdx.y];
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= S.ndp) return;
-  double r = 0.0;
-  if (k < S.nd) {
-    double s = 0.0;
-    const T* ph = W + S.w_off + sym_elems(S.ndp);  // Phi^T rows
-    for (int c = 0; c < S.nc; ++c) s += (double)ph[(int64_t)c * S.ndp + k] * uc[c_loc[S.cvec_off + c]];
-    r = beta * s;
-    if (alpha != 0.0) r += alpha * y[S.y_off + k];
-    if (gamma != 0.0) r += gamma * vec[S.vec_off + S.o_a + k];
+  const int k0 = blockIdx.x * 64;
+  if (k0 >= S.ndp) return;
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int k = k0 + lane;
+  __shared__ double part[4][64];
+  const T* ph = W + S.w_off + sym_elems(S.ndp);  // Phi^T rows
+  double s0 = 0.0, s1 = 0.0;
+  int c = q;
+  for (; c + 4 < S.nc; c += 8) {
+    s0 = fma((double)ph[(int64_t)c * S.ndp + k], uc[c_loc[S.cvec_off + c]], s0);
+    s1 = fma((double)ph[(int64_t)(c + 4) * S.ndp + k], uc[c_loc[S.cvec_off + c + 4]], s1);
   }
-  out[S.dvec_off + k] = r;
+  if (c < S.nc) s0 = fma((double)ph[(int64_t)c * S.ndp + k], uc[c_loc[S.cvec_off + c]], s0);
+  part[q][lane] = s0 + s1;
+  __syncthreads();
+  if (q == 0) {
+    double r = 0.0;
+    if (k < S.nd) {
+      r = beta * ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+      if (alpha != 0.0) r += alpha * y[S.y_off + k];
+      if (gamma != 0.0) r += gamma * vec[S.vec_off + S.o_a + k];
+    }
+    out[S.dvec_off + k] = r;
+  }
 }
 
 template <typename T>
@@ -1178,7 +1193,7 @@ void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, co
                          const int32_t* c_loc, const double* uc, const double* vec, double alpha,
                          double beta, double gamma, double* out, hipStream_t st) {
   if (ns <= 0 || max_ndp <= 0) return;
-  hipLaunchKernelGGL(k_dual_correct<T>, dim3((max_ndp + 255) / 256, ns), dim3(256), 0, st, subs, W,
+  hipLaunchKernelGGL(k_dual_correct<T>, dim3((max_ndp + 63) / 64, ns), dim3(256), 0, st, subs, W,
                      y, c_loc, uc, vec, alpha, beta, gamma, out);
 }
 template void launch_dual_correct<double>(int, int, const DevSub*, const double*, const double*,
