@@ -460,7 +460,10 @@ void build_plans(const Topology& T, Plans& P) {
       maxTi = std::max(maxTi, L.Ti);
       maxTd = std::max(maxTd, L.Tr - L.Ti);
     }
-    // phase 1: interior columns; rows of D' tiles are structurally zero here
+    // phase 1: interior columns; rows of D' tiles are structurally zero here.
+    // Lookahead (incremental variant): the mode-1 launch of column j also applies
+    // K in [.., j) to the next column's mode-3 tile (j + 2, j + 1) (flags 2, update
+    // only), so the side-stream item keeps one product and its diagonal factorization.
     for (int j = 0; j < maxTi; ++j) {
       std::vector<TileWork> diag, next, below;
       for (int64_t s = 0; s < ns; ++s) {
@@ -471,9 +474,15 @@ void build_plans(const Topology& T, Plans& P) {
           diag.push_back(TileWork{int32_t(s), j, j, inc ? 0 : ft[j], inc ? 0 : j, 0});
         for (int I = j + 1; I < L.Tc; ++I)
           if (ft[I] <= j) {
-            const TileWork w{int32_t(s), I, j, std::max(ft[I], ft[j]), j, fl};
-            (inc && I == j + 1 && I < L.Ti ? next : below).push_back(w);
+            TileWork w{int32_t(s), I, j, std::max(ft[I], ft[j]), j, fl};
+            const bool is_next = inc && I == j + 1 && I < L.Ti;
+            if (is_next && j >= 1 && w.k0 < j - 1) w.k0 = j - 1;  // rest applied by lookahead
+            (is_next ? next : below).push_back(w);
           }
+        if (inc && j + 2 < L.Ti) {
+          const int k0 = std::max(ft[j + 2], ft[j + 1]);
+          if (k0 < j) below.push_back(TileWork{int32_t(s), j + 2, j + 1, k0, j, 2});
+        }
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
@@ -506,9 +515,15 @@ void build_plans(const Topology& T, Plans& P) {
           diag.push_back(TileWork{int32_t(s), col, col, inc ? 0 : std::max(L.Ti, ft[col]), inc ? 0 : col, 0});
         for (int I = col + 1; I < L.T; ++I)
           if (ft[I] <= col) {
-            const TileWork w{int32_t(s), I, col, std::max(L.Ti, std::max(ft[I], ft[col])), col, fl};
-            (inc && I == col + 1 && I < L.Tr ? next : below).push_back(w);
+            TileWork w{int32_t(s), I, col, std::max(L.Ti, std::max(ft[I], ft[col])), col, fl};
+            const bool is_next = inc && I == col + 1 && I < L.Tr;
+            if (is_next && col >= 1 && w.k0 < col - 1) w.k0 = col - 1;  // rest applied by lookahead
+            (is_next ? next : below).push_back(w);
           }
+        if (inc && col + 2 < L.Tr) {
+          const int k0 = std::max(L.Ti, std::max(ft[col + 2], ft[col + 1]));
+          if (k0 < col) below.push_back(TileWork{int32_t(s), col + 2, col + 1, k0, col, 2});
+        }
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
@@ -549,6 +564,39 @@ void build_plans(const Topology& T, Plans& P) {
       check(w.J <= w.I && w.I < L.T && w.k1 <= w.J && w.J >= ft[w.I], "local tile");
       check(w.k0 >= w.k1 || (w.k0 >= ft[w.I] && w.k0 >= ft[w.J]), "local tile panel");
     }
+  // every stored tile (I > J) of an eliminated column J receives its column-step K range
+  // ([max(f_I, f_J), J) for interior columns, [max(Ti, f_I, f_J), J) for dual columns)
+  // exactly once, split between a lookahead item and its own item at most
+  {
+    const TileSchedule& S = P.local_sched;
+    std::vector<int64_t> toff(size_t(ns) + 1, 0);
+    for (int64_t s = 0; s < ns; ++s) toff[size_t(s) + 1] = toff[size_t(s)] + int64_t(T.sub[s].T) * T.sub[s].Tr;
+    std::vector<int32_t> next_k(size_t(toff[size_t(ns)]), -1);
+    for (const TileLaunch& L : S.launches) {
+      if (L.mode != 1 && L.mode != 3) continue;
+      for (int64_t i = L.off; i < L.off + L.n; ++i) {
+        const TileWork& w = S.items[size_t(i)];
+        const SubLayout& Ls = T.sub[w.mat];
+        check(w.J < Ls.Tr && w.I > w.J, "local column-step item");
+        const int32_t* ft = &P.ftile[P.ftile_off[w.mat]];
+        const int32_t start = std::max(w.J >= Ls.Ti ? Ls.Ti : 0, std::max(ft[w.I], ft[w.J]));
+        int32_t& nk = next_k[size_t(toff[w.mat] + int64_t(w.I) * Ls.Tr + w.J)];
+        if (nk < 0) nk = start;
+        check(w.k0 == nk || w.k0 >= w.k1, "local K ranges contiguous");
+        if (w.k1 > w.k0) nk = w.k1;
+      }
+    }
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& Ls = T.sub[s];
+      const int32_t* ft = &P.ftile[P.ftile_off[s]];
+      for (int I = 0; I < Ls.T; ++I)
+        for (int J = ft[I]; J < std::min(I, Ls.Tr); ++J) {
+          if (J < Ls.Ti && I >= Ls.Tc) continue;  // D' rows are zero in interior columns
+          const int32_t nk = next_k[size_t(toff[s] + int64_t(I) * Ls.Tr + J)];
+          check(nk == J, "local K ranges complete (and every tile has its column step)");
+        }
+    }
+  }
   // a mode-3 item completes and factors the diagonal tile of the next column
   for (const TileSchedule* S : {&P.local_sched, &P.local_sched_ll, &P.coarse_sched})
     for (const TileLaunch& L : S->launches)
