@@ -216,3 +216,28 @@ def test_forward_sweep_in_factorization_matches_trsv(gpu_available, kind, precis
     assert r_fused.status == r_trsv.status == SolverStatus.SUCCESS
     assert r_fused.iterations == r_trsv.iterations
     assert np.array_equal(r_fused.x, r_trsv.x)
+
+
+@pytest.mark.parametrize("layout", ["one", "halves", "slabs"])
+def test_degenerate_partitions(gpu_available, layout):
+    """One subdomain (no multipliers, no primal dofs), two halves and four slabs (no edge
+    shared by three subdomains, so no primal dofs and an empty coarse problem): the local
+    problems stay SPD through the kappa mass term and the solve matches the direct one."""
+    p = W.make_problem(6, 1, seed=1, name="degenerate")
+    cent = p.xyz[p.tet2node].mean(axis=1)
+    sub = {"one": np.zeros(p.n_tets, np.int32),
+           "halves": (cent[:, 0] > 0.5).astype(np.int32),
+           "slabs": np.minimum((cent[:, 0] * 4).astype(np.int32), 3)}[layout]
+    print(f"[degenerate] {layout}", flush=True)
+    s = FETIDPSolver(tolerance=1e-9)
+    s.setup(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, sub, p.pec_pairs)
+    assert s.sizes["n_primal"] == 0
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    r = s.solve()
+    asm = assembly.assemble(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.pec_pairs, p.jq)
+    xd = solvers.direct(asm["A"], asm["f"])
+    assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-9
+    assert np.linalg.norm(r.x - xd) <= 1e-8 * np.linalg.norm(xd)
+    if layout == "one":
+        assert r.iterations == 0
