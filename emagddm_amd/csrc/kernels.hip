@@ -1065,7 +1065,9 @@ template void launch_gemv_phi<float>(int, int, const DevSub*, const float*, cons
 // 64-vectors of partial sums, M_IJ x_J at slot (I, J) and (for I > J) M_IJ^T x_I
 // at slot (J, I) of the subdomain's (ndp/64)^2 partial array; k_symv_reduce then
 // sums slots (I, 0 .. Td-1) in order.  Wave w holds tile rows 16w .. 16w+15, lane
-// = column, so each tile row is one 512-byte (FP64) coalesced load.
+// = column, so each tile row is one coalesced load (512 B in FP64).  The 16 row sums
+// of a wave are formed by a halving butterfly over the lanes (8 + 4 + 2 + 1 + 2
+// shuffles, no LDS), the column sums by one 4-way LDS reduction across the waves.
 template <typename T>
 __global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ tiles,
                                                    const DevSub* __restrict__ subs,
@@ -1075,7 +1077,6 @@ __global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ t
                                                    const double* __restrict__ v,
                                                    double* __restrict__ part) {
   __shared__ double xJ[64], xI[64];
-  __shared__ double prod[64 * 65];
   __shared__ double colp[4][64];
   const SymTile tl = tiles[blockIdx.x];
   const DevSub S = subs[tl.s];
@@ -1095,28 +1096,55 @@ __global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ t
   }
   __syncthreads();
   const double xc = xJ[lane];
+  double rv[16];
   double cs = 0.0;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
-    const int r = 16 * wv + q;
     const double a = (double)at[q];
-    prod[r * 65 + lane] = a * xc;
-    cs += a * xI[r];
+    rv[q] = a * xc;
+    cs += a * xI[16 * wv + q];
   }
-  const bool off = tl.I != tl.J;  // uniform per work-group
-  if (off) colp[wv][lane] = cs;
-  __syncthreads();
-  const int Td = S.ndp >> 6;
-  const int r = tid >> 2, qq = tid & 3;
-  double rs = 0.0;
+  // halving butterfly: after the steps over lane bits 5, 4, 3, 2 lane l holds the partial
+  // of row 8 b5 + 4 b4 + 2 b3 + b2 (b = bits of l) over the lanes that differ in bits 0-1
 #pragma unroll
-  for (int c = 0; c < 16; ++c) rs += prod[r * 65 + 16 * qq + c];
-  rs += __shfl_xor(rs, 1);
+  for (int i = 0; i < 8; ++i) {
+    const bool hi = lane & 32;
+    const double keep = hi ? rv[i + 8] : rv[i], send = hi ? rv[i] : rv[i + 8];
+    rv[i] = keep + __shfl_xor(send, 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool hi = lane & 16;
+    const double keep = hi ? rv[i + 4] : rv[i], send = hi ? rv[i] : rv[i + 4];
+    rv[i] = keep + __shfl_xor(send, 16);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const bool hi = lane & 8;
+    const double keep = hi ? rv[i + 2] : rv[i], send = hi ? rv[i] : rv[i + 2];
+    rv[i] = keep + __shfl_xor(send, 8);
+  }
+  {
+    const bool hi = lane & 4;
+    const double keep = hi ? rv[1] : rv[0], send = hi ? rv[0] : rv[1];
+    rv[0] = keep + __shfl_xor(send, 4);
+  }
+  double rs = rv[0];
   rs += __shfl_xor(rs, 2);
-  if (qq == 0) part[S.sp_off + ((int64_t)tl.I * Td + tl.J) * 64 + r] = rs;
-  if (off && tid < 64)
-    part[S.sp_off + ((int64_t)tl.J * Td + tl.I) * 64 + tid] =
-        (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
+  rs += __shfl_xor(rs, 1);
+  const int Td = S.ndp >> 6;
+  if ((lane & 3) == 0) {
+    const int row = 16 * wv + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                    ((lane >> 2) & 1);
+    part[S.sp_off + ((int64_t)tl.I * Td + tl.J) * 64 + row] = rs;
+  }
+  if (tl.I != tl.J) {  // uniform per work-group
+    colp[wv][lane] = cs;
+    __syncthreads();
+    if (tid < 64)
+      part[S.sp_off + ((int64_t)tl.J * Td + tl.I) * 64 + tid] =
+          (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
+  }
 }
 
 // out[I*64 + r] = sum_{J < Td} part[(I, J)][r], at y_off (which 0) or dvec_off (which 1)
