@@ -129,6 +129,11 @@ struct fetidp_solver {
   DBuf<int32_t> vec_dof;
   DBuf<double> vec_w;
   DBuf<int64_t> local_vec_off;  // per subdomain: offset of its rows in vec / yvec
+  // per pool tile: 1 if the factorization (FP64 schedule / FP32 schedule) ever writes it;
+  // the others stay stale and read as zero in the extractions and the coarse gather
+  DBuf<uint8_t> tw64, tw32, ts64, ts32;  // ts: written before the S_DD snapshot
+  DBuf<int32_t> asm_tiles;  // the tiles that receive assembled entries (cleared per factor)
+  const uint8_t* written() const { return fp32() ? tw32.p : tw64.p; }
   // vec / yvec hold the forward sweep of the current f_glob (done by the factorization
   // itself, or by a condensation of f_glob since): condensing f_glob again skips it
   bool fwd_valid = false;
@@ -428,6 +433,11 @@ struct fetidp_solver {
       rowabs.upload(plans.rowabs);
       coarse_rowabs.upload(plans.coarse_rowabs);
       local_inv_base.upload(plans.inv_off);
+      tw64.upload(plans.tile_written);
+      tw32.upload(plans.tile_written_ll);
+      ts64.upload(plans.tile_written_snap);
+      ts32.upload(plans.tile_written_snap_ll);
+      asm_tiles.upload(plans.assembled_tiles);
       {
         std::vector<int64_t> vo(size_t(topo.n_sub));
         for (int64_t q = 0; q < topo.n_sub; ++q) vo[size_t(q)] = topo.sub[size_t(q)].vec_off;
@@ -583,8 +593,9 @@ struct fetidp_solver {
       if (L.mode < 0) {
         timed(5, [&] {
           const int64_t nt = int64_t(plans.sym_tiles.size());
-          if (p64) launch_extract_P<T, double>(nt, sym_tiles.p, devsub.p, pool, rowabs.p, ftile.p, Pd.p, st);
-          else launch_extract_P<T, float>(nt, sym_tiles.p, devsub.p, pool, rowabs.p, ftile.p, Pf.p, st);
+          const uint8_t* snap = sizeof(T) == 8 ? ts64.p : ts32.p;
+          if (p64) launch_extract_P<T, double>(nt, sym_tiles.p, devsub.p, pool, rowabs.p, ftile.p, snap, Pd.p, st);
+          else launch_extract_P<T, float>(nt, sym_tiles.p, devsub.p, pool, rowabs.p, ftile.p, snap, Pf.p, st);
         });
         continue;
       }
@@ -686,7 +697,9 @@ struct fetidp_solver {
       Pd.release();
       Pf.alloc(size_t(topo.p_elems));
     }
-    pool.zero(st);
+    // only tiles with assembled entries are cleared: the schedule marks each item whose
+    // tile (or diagonal tile) nothing has written yet, and the kernels read those as zero
+    launch_zero_tiles<T>(int64_t(plans.assembled_tiles.size()), asm_tiles.p, pool.p, st);
     timed(1, [&] {
       if constexpr (sizeof(T) == 8)
         launch_assemble_f64(plans.pool.size(), pool_target.p, pool_ptr.p, pool_src.p, Ae.p, pool.p, st);
@@ -707,17 +720,17 @@ struct fetidp_solver {
     // W extraction (st3) and the coarse problem (st, st2) only read the local
     // factors, so they overlap; st waits for st3 before the end of the factor.
     if (profiling()) {
-      timed(5, [&] { launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st); });
+      timed(5, [&] { launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st); });
     } else {
       HIPCHECK(hipEventRecord(ev_main, st));
       HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
-      launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, W.p, st3);
+      launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st3);
       HIPCHECK(hipEventRecord(ev_extract, st3));
     }
     // coarse problem in FP64
     cpool.zero(st);
     launch_gather_pool<T>(plans.coarse.size(), coarse_target.p, coarse_ptr.p, coarse_srcpos.p,
-                          pool.p, cpool.p, st);
+                          pool.p, (sizeof(T) == 8 ? tw64.p : tw32.p), cpool.p, st);
     cfail.zero(st);
     run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, false);
     launch_extract_sinv(topo.n_c, plans.coarse_ncp, sinv_ld(), cpool.p, coarse_rowabs.p, coarse_ftile.p, sinv.p, st);
@@ -1493,10 +1506,10 @@ int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S) {
     tmp.zero(h->st);
     if (h->fp32())
       launch_gather_pool<float>(h->plans.coarse.size(), h->coarse_target.p, h->coarse_ptr.p,
-                                h->coarse_srcpos.p, h->pool32.p, tmp.p, h->st);
+                                h->coarse_srcpos.p, h->pool32.p, h->written(), tmp.p, h->st);
     else
       launch_gather_pool<double>(h->plans.coarse.size(), h->coarse_target.p, h->coarse_ptr.p,
-                                 h->coarse_srcpos.p, h->pool64.p, tmp.p, h->st);
+                                 h->coarse_srcpos.p, h->pool64.p, h->written(), tmp.p, h->st);
     HIPCHECK(hipStreamSynchronize(h->st));
     std::vector<double> pool(h->plans.coarse_pool_elems);
     HIPCHECK(hipMemcpy(pool.data(), tmp.p, sizeof(double) * pool.size(), hipMemcpyDeviceToHost));

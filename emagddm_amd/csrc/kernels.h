@@ -24,7 +24,10 @@ void launch_assemble_f32(int64_t n, const int64_t* target, const int64_t* ptr, c
 // out[target[i]] = sum_{c} pool[srcpos[c]]  (coarse assembly from factor pools)
 template <typename T>
 void launch_gather_pool(int64_t n, const int64_t* target, const int64_t* ptr, const int64_t* srcpos,
-                        const T* pool, double* out, hipStream_t st);
+                        const T* pool, const uint8_t* tflag, double* out, hipStream_t st);
+// Clear the pool tiles listed by index (elements 4096 t .. 4096 t + 4095 of tile t).
+template <typename T>
+void launch_zero_tiles(int64_t n, const int32_t* list, T* pool, hipStream_t st);
 
 // mode 0: Schur update list; 1: column step below the diagonal (update + trsm with
 // the stored inverse diagonal tile + diagonal-tile share); 2: diagonal Cholesky + inverse.
@@ -53,11 +56,12 @@ void launch_mfma_probe(int is_f32, double* out);
 // tile of every subdomain's D x D store.
 template <typename T, typename TP>
 void launch_extract_P(int64_t ntiles, const SymTile* tiles, const DevSub* subs, const T* pool,
-                      const int64_t* rowabs, const int32_t* ftile, TP* P, hipStream_t st);
+                      const int64_t* rowabs, const int32_t* ftile, const uint8_t* tflag, TP* P,
+                      hipStream_t st);
 template <typename T>
 void launch_extract_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp, int max_ndp,
                       const DevSub* subs, const T* pool, const int64_t* rowabs, const int32_t* ftile,
-                      T* W, hipStream_t st);
+                      const uint8_t* tflag, T* W, hipStream_t st);
 // S~^-1 (dense, row stride ld, even) from the trailing block of the coarse pool.
 void launch_extract_sinv(int64_t nc, int64_t ncp, int64_t ld, const double* cpool,
                          const int64_t* rowabs, const int32_t* ftile, double* sinv, hipStream_t st);

@@ -170,28 +170,44 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gather_pool(int64_t n, const int64_t* __restrict__ target,
                                                      const int64_t* __restrict__ ptr,
                                                      const int64_t* __restrict__ srcpos,
-                                                     const T* __restrict__ pool, double* __restrict__ out) {
+                                                     const T* __restrict__ pool,
+                                                     const uint8_t* __restrict__ tflag,
+                                                     double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
   for (int64_t c = ptr[i]; c < ptr[i + 1]; ++c) {
     const int64_t p = srcpos[c];
-    s += p < 0 ? 1.0 : (double)pool[p];
+    s += p < 0 ? 1.0 : ((tflag && !tflag[p / kTileElems]) ? 0.0 : (double)pool[p]);
   }
   out[target[i]] = s;
 }
 
 template <typename T>
 void launch_gather_pool(int64_t n, const int64_t* target, const int64_t* ptr, const int64_t* srcpos,
-                        const T* pool, double* out, hipStream_t st) {
+                        const T* pool, const uint8_t* tflag, double* out, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_gather_pool<T>, dim3((n + 255) / 256), dim3(256), 0, st, n, target, ptr,
-                     srcpos, pool, out);
+                     srcpos, pool, tflag, out);
 }
 template void launch_gather_pool<double>(int64_t, const int64_t*, const int64_t*, const int64_t*,
-                                         const double*, double*, hipStream_t);
+                                         const double*, const uint8_t*, double*, hipStream_t);
 template void launch_gather_pool<float>(int64_t, const int64_t*, const int64_t*, const int64_t*,
-                                        const float*, double*, hipStream_t);
+                                        const float*, const uint8_t*, double*, hipStream_t);
+
+// Clear the listed 64x64 tiles of a pool (tile t = elements 4096 t .. 4096 t + 4095).
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_tiles(const int32_t* __restrict__ list, T* __restrict__ pool) {
+  T* t = pool + (int64_t)list[blockIdx.x] * kTileElems;
+  for (int e = threadIdx.x; e < kTileElems; e += 256) t[e] = T(0);
+}
+template <typename T>
+void launch_zero_tiles(int64_t n, const int32_t* list, T* pool, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_tiles<T>, dim3(n), dim3(256), 0, st, list, pool);
+}
+template void launch_zero_tiles<double>(int64_t, const int32_t*, double*, hipStream_t);
+template void launch_zero_tiles<float>(int64_t, const int32_t*, float*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // Batched tile Cholesky on MFMA
@@ -456,6 +472,8 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
   // envelope-packed storage: tile (I, J) at ra[I] + J * 4096
   const int64_t* ra = rowabs + rowbase[w.mat];
   T* C = pool + ra[w.I] + (int64_t)w.J * kTileElems;
+  // flags & 4: nothing has written tile C yet (no assembled entries): it reads as zero
+  const bool fresh = w.flags & 4;  // uniform per work-group
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   // D = C^T: rows p of D are columns j of C, columns q of D are rows i of C.
@@ -482,7 +500,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-          C[j * 64 + i] -= acc[x][y][r];
+          C[j * 64 + i] = (fresh ? T(0) : C[j * 64 + i]) - acc[x][y][r];
         }
     return;
   }
@@ -499,7 +517,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-          s[j * kLd + i] = C[j * 64 + i] - acc[x][y][r];
+          s[j * kLd + i] = (fresh ? T(0) : C[j * 64 + i]) - acc[x][y][r];
         }
     __syncthreads();
     if constexpr (MODE == 1 || MODE == 3) {
@@ -545,6 +563,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
         for (int y = 0; y < 2; ++y) a3[x][y] = acc_t{0, 0, 0, 0};
       tile_mma<T, kLd, kLd>(a3, s, s, pb0, qb0, lr, lk);
       T* Cd = pool + ra[w.I] + (int64_t)w.I * kTileElems;
+      const bool dfresh = w.flags & 8;  // flags & 8: the diagonal tile is still unwritten
       if constexpr (MODE == 1) {
 #pragma unroll
         for (int x = 0; x < 2; ++x)
@@ -553,7 +572,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-              Cd[j * 64 + i] -= a3[x][y][r];
+              Cd[j * 64 + i] = (dfresh ? T(0) : Cd[j * 64 + i]) - a3[x][y][r];
             }
       } else {
         // C_II is complete: stage it in s once every wave has finished reading X
@@ -565,7 +584,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-              s[j * kLd + i] = Cd[j * 64 + i] - a3[x][y][r];
+              s[j * kLd + i] = (dfresh ? T(0) : Cd[j * 64 + i]) - a3[x][y][r];
             }
         __syncthreads();
         factor_diag_tile<T>(s, lds_panel, Cd, inv + inv_base[w.mat] + (int64_t)w.I * kTileElems,
@@ -688,7 +707,8 @@ __global__ __launch_bounds__(256) void k_extract_sym(const SymTile* __restrict__
                                                      const DevSub* __restrict__ subs,
                                                      const T* __restrict__ pool,
                                                      const int64_t* __restrict__ rowabs,
-                                                     const int32_t* __restrict__ ftile, int block,
+                                                     const int32_t* __restrict__ ftile,
+                                                     const uint8_t* __restrict__ tflag, int block,
                                                      double sign, TP* __restrict__ out) {
   const SymTile tl = tiles[blockIdx.x];
   const DevSub S = subs[tl.s];
@@ -696,12 +716,13 @@ __global__ __launch_bounds__(256) void k_extract_sym(const SymTile* __restrict__
   const int I = o + tl.I, J = o + tl.J;
   TP* dst = out + (block == 0 ? S.p_off : S.w_off) + ((int64_t)tl.I * (tl.I + 1) / 2 + tl.J) * kTileElems;
   const int tid = threadIdx.x;
-  if (J < ftile[S.ft_off + I]) {  // outside the envelope (uniform per work-group)
-    for (int e = tid; e < kTileElems; e += 256) dst[e] = TP(0);
+  const int64_t soff = rowabs[S.ft_off + I] + (int64_t)J * kTileElems;
+  if (J < ftile[S.ft_off + I] || (tflag && !tflag[soff / kTileElems])) {  // outside the envelope
+    for (int e = tid; e < kTileElems; e += 256) dst[e] = TP(0);         // or never written
     return;
   }
   __shared__ T t[64 * 65];
-  const T* src = pool + rowabs[S.ft_off + I] + (int64_t)J * kTileElems;
+  const T* src = pool + soff;
   for (int e = tid; e < kTileElems; e += 256) t[(e >> 6) * 65 + (e & 63)] = src[e];  // t[c][r] = (r, c)
   __syncthreads();
   const bool diag = I == J;
@@ -720,6 +741,7 @@ __global__ __launch_bounds__(256) void k_extract_phi(const DevSub* __restrict__ 
                                                      const T* __restrict__ pool,
                                                      const int64_t* __restrict__ rowabs,
                                                      const int32_t* __restrict__ ftile,
+                                                     const uint8_t* __restrict__ tflag,
                                                      T* __restrict__ W) {
   const DevSub S = subs[blockIdx.y];
   const int tc = S.ncp >> 6, td = S.ndp >> 6;
@@ -729,8 +751,9 @@ __global__ __launch_bounds__(256) void k_extract_phi(const DevSub* __restrict__ 
   const int I = (S.o_a >> 6) + kJ, J = (S.o_c >> 6) + cI;  // pool tile (D' row, c column)
   T* dst = W + S.w_off + sym_elems(S.ndp) + (int64_t)(cI * 64) * S.ndp + kJ * 64;
   const int tid = threadIdx.x;
-  const bool in = J >= ftile[S.ft_off + I];
-  const T* src = pool + rowabs[S.ft_off + I] + (int64_t)J * kTileElems;
+  const int64_t soff = rowabs[S.ft_off + I] + (int64_t)J * kTileElems;
+  const bool in = J >= ftile[S.ft_off + I] && (!tflag || tflag[soff / kTileElems]);
+  const T* src = pool + soff;
   for (int e = tid; e < kTileElems; e += 256) {
     const int c = e >> 6, r = e & 63;  // pool element (r, c) of the tile at c*64 + r
     dst[(int64_t)c * S.ndp + r] = in ? -src[e] : T(0);
@@ -739,36 +762,37 @@ __global__ __launch_bounds__(256) void k_extract_phi(const DevSub* __restrict__ 
 
 template <typename T, typename TP>
 void launch_extract_P(int64_t ntiles, const SymTile* tiles, const DevSub* subs, const T* pool,
-                      const int64_t* rowabs, const int32_t* ftile, TP* P, hipStream_t st) {
+                      const int64_t* rowabs, const int32_t* ftile, const uint8_t* tflag, TP* P,
+                      hipStream_t st) {
   if (ntiles <= 0) return;
   hipLaunchKernelGGL((k_extract_sym<T, TP>), dim3(ntiles), dim3(256), 0, st, tiles, subs, pool, rowabs,
-                     ftile, 0, 1.0, P);
+                     ftile, tflag, 0, 1.0, P);
 }
 template void launch_extract_P<double, float>(int64_t, const SymTile*, const DevSub*, const double*,
-                                              const int64_t*, const int32_t*, float*, hipStream_t);
+                                              const int64_t*, const int32_t*, const uint8_t*, float*, hipStream_t);
 template void launch_extract_P<float, float>(int64_t, const SymTile*, const DevSub*, const float*,
-                                             const int64_t*, const int32_t*, float*, hipStream_t);
+                                             const int64_t*, const int32_t*, const uint8_t*, float*, hipStream_t);
 template void launch_extract_P<double, double>(int64_t, const SymTile*, const DevSub*, const double*,
-                                               const int64_t*, const int32_t*, double*, hipStream_t);
+                                               const int64_t*, const int32_t*, const uint8_t*, double*, hipStream_t);
 template void launch_extract_P<float, double>(int64_t, const SymTile*, const DevSub*, const float*,
-                                              const int64_t*, const int32_t*, double*, hipStream_t);
+                                              const int64_t*, const int32_t*, const uint8_t*, double*, hipStream_t);
 
 template <typename T>
 void launch_extract_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp, int max_ndp,
                       const DevSub* subs, const T* pool, const int64_t* rowabs, const int32_t* ftile,
-                      T* W, hipStream_t st) {
+                      const uint8_t* tflag, T* W, hipStream_t st) {
   if (ntiles > 0)
     hipLaunchKernelGGL((k_extract_sym<T, T>), dim3(ntiles), dim3(256), 0, st, tiles, subs, pool, rowabs,
-                       ftile, 1, -1.0, W);
+                       ftile, tflag, 1, -1.0, W);
   const int nb = (max_ncp >> 6) * (max_ndp >> 6);
   if (nb > 0 && ns > 0)
-    hipLaunchKernelGGL(k_extract_phi<T>, dim3(nb, ns), dim3(256), 0, st, subs, pool, rowabs, ftile, W);
+    hipLaunchKernelGGL(k_extract_phi<T>, dim3(nb, ns), dim3(256), 0, st, subs, pool, rowabs, ftile, tflag, W);
 }
 template void launch_extract_W<double>(int64_t, const SymTile*, int, int, int, const DevSub*,
-                                       const double*, const int64_t*, const int32_t*, double*,
-                                       hipStream_t);
+                                       const double*, const int64_t*, const int32_t*, const uint8_t*,
+                                       double*, hipStream_t);
 template void launch_extract_W<float>(int64_t, const SymTile*, int, int, int, const DevSub*,
-                                      const float*, const int64_t*, const int32_t*, float*, hipStream_t);
+                                      const float*, const int64_t*, const int32_t*, const uint8_t*, float*, hipStream_t);
 
 __global__ __launch_bounds__(256) void k_extract_sinv(int64_t nc, int64_t ncp, int64_t ld,
                                                       const double* __restrict__ cpool,

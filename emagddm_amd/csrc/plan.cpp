@@ -316,6 +316,10 @@ void build_plans(const Topology& T, Plans& P) {
       for (int k = 0; k < n; ++k) loc[dofs[k]] = -1;
     }
     group_emissions(tgt, code, uint64_t(P.pool_elems), P.pool.target, P.pool.ptr, P.pool.src);
+    P.tile_assembled.assign(size_t(P.pool_elems / kTileElems), 0);
+    for (int64_t t : P.pool.target) P.tile_assembled[size_t(t / kTileElems)] = 1;
+    for (size_t t = 0; t < P.tile_assembled.size(); ++t)
+      if (P.tile_assembled[t]) P.assembled_tiles.push_back(int32_t(t));
   }
 
   // ---- global CSR and RHS ---------------------------------------------------
@@ -541,6 +545,38 @@ void build_plans(const Topology& T, Plans& P) {
         }
     }
     add_tiles(S, 0, schur2);
+  }
+
+  // ---- fresh tiles: mark items whose tile (flags 4) or diagonal tile (flags 8) nothing
+  // has written yet (only assembled tiles are cleared before a factorization) ----------
+  for (int variant = 0; variant < 2; ++variant) {
+    TileSchedule& S = variant == 0 ? P.local_sched : P.local_sched_ll;
+    std::vector<uint8_t> w = P.tile_assembled;
+    std::vector<uint8_t>& snap = variant == 0 ? P.tile_written_snap : P.tile_written_snap_ll;
+    for (const TileLaunch& L : S.launches) {
+      if (L.mode < 0) {
+        snap = w;
+        continue;
+      }
+      for (int64_t i = L.off; i < L.off + L.n; ++i) {
+        TileWork& it = S.items[size_t(i)];
+        const int64_t* ra = &P.rowabs[P.ftile_off[it.mat]];
+        const size_t c = size_t((ra[it.I] + int64_t(it.J) * kTileElems) / kTileElems);
+        if (!w[c]) it.flags |= 4;
+        const bool share = (L.mode == 1 && (it.flags & 1) && !(it.flags & 2)) || L.mode == 3;
+        if (share) {
+          const size_t d = size_t((ra[it.I] + int64_t(it.I) * kTileElems) / kTileElems);
+          if (!w[d]) it.flags |= 8;
+          w[d] = 1;
+        }
+      }
+      for (int64_t i = L.off; i < L.off + L.n; ++i) {  // this launch's writes
+        const TileWork& it = S.items[size_t(i)];
+        const int64_t* ra = &P.rowabs[P.ftile_off[it.mat]];
+        w[size_t((ra[it.I] + int64_t(it.J) * kTileElems) / kTileElems)] = 1;
+      }
+    }
+    (variant == 0 ? P.tile_written : P.tile_written_ll) = w;
   }
 
   P.d_self.assign(T.dvec_elems, -1);

@@ -86,6 +86,14 @@ struct Plans {
                                       // in the packed pool; tile (I, J >= f_I) at rowabs + J*4096
   std::vector<int64_t> pool_base;     // per subdomain: first element of its packed tiles
   int64_t pool_elems = 0;
+  // Pool tiles (index = element offset / 4096) that receive assembled entries: only they
+  // are cleared before assembly; every other stored tile starts as "fresh" (its flag is 0)
+  // and the tile kernels read it as zero until they first write it.
+  std::vector<uint8_t> tile_assembled;
+  std::vector<int32_t> assembled_tiles;
+  // per pool tile: written by the end of the FP64 / FP32 schedule, and before its S_DD
+  // snapshot; tiles never written keep stale data and are read as zero
+  std::vector<uint8_t> tile_written, tile_written_ll, tile_written_snap, tile_written_snap_ll;
   std::vector<int32_t> coarse_ftile;
   std::vector<int64_t> coarse_rowabs;
   int64_t inv_elems = 0, coarse_inv_elems = 0;
