@@ -569,7 +569,21 @@ struct fetidp_solver {
                     int* failp, bool local, RhsArgs rhs = RhsArgs()) {
     const int slot0 = local ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     bool after_side = false;      // the previous launch was a mode-3 launch on st2
-    for (const TileLaunch& L : S.launches) {
+    for (size_t li = 0; li < S.launches.size(); ++li) {
+      const TileLaunch& L = S.launches[li];
+      if (L.mode == 3 && cfg.side_stream == 2 && !profiling()) {
+        // the factoring items (flags & 16) go at the front of the column-step launch
+        // that follows them: dispatched first, no second stream
+        int64_t n = L.n;
+        if (li + 1 < S.launches.size() && S.launches[li + 1].mode == 1 &&
+            S.launches[li + 1].off == L.off + L.n)
+          n += S.launches[++li].n;
+        timed(slot0 + 1, [&] {
+          launch_tile_mode<T>(1, items.p + L.off, n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st,
+                              cfg.tile_waves == 0 ? 0 : 4, rhs);
+        });
+        continue;
+      }
       if (L.mode == 3 && (profiling() || !cfg.side_stream)) {
         // profiled runs keep every launch on st, so event timings are per kernel
         timed(slot0 + 2, [&] {
@@ -1146,10 +1160,10 @@ struct fetidp_solver {
   }
 
   // Autotuner: time factor + solve for every combination of the launch choices that
-  // leave the arithmetic unchanged (side stream for the diagonal factorizations, PCPG
-  // graph replay, throughput tile kernels capped at 128 registers or allocated freely),
-  // check that
-  // every combination returns bitwise the same solution, keep the fastest.
+  // leave the arithmetic unchanged (where the diagonal factorizations run: same
+  // stream, second stream, or at the front of the column-step launch; PCPG graph
+  // replay; throughput tile kernels capped at 128 registers or allocated freely),
+  // check that every combination returns bitwise the same solution, keep the fastest.
   // out[3k..3k+2] = (side_stream, graphs, tile_waves), out[... ] times in milliseconds:
   // per variant 4 values {side_stream, graphs, tile_waves, ms}.
   int autotune(int reps, double* out, int cap) {
@@ -1159,10 +1173,13 @@ struct fetidp_solver {
     std::vector<double> ref;
     double best = 1e300;
     int best_v = -1, k = 0, nv = 0;
-    for (int v = 0; v < 8; ++v) {
-      cfg.side_stream = (v >> 0) & 1;
-      cfg.graphs = (v >> 1) & 1;
-      cfg.tile_waves = (v >> 2) & 1 ? 0 : 4;
+    auto pick = [&](int v) {
+      cfg.side_stream = v % 3;
+      cfg.graphs = (v / 3) & 1;
+      cfg.tile_waves = (v / 6) & 1 ? 0 : 4;
+    };
+    for (int v = 0; v < 12; ++v) {
+      pick(v);
       fetidp_stats S{};
       factor();
       solve(nullptr, nullptr, S);  // warm-up (graph capture, allocations)
@@ -1193,9 +1210,7 @@ struct fetidp_solver {
       }
     }
     cfg = base;
-    cfg.side_stream = (best_v >> 0) & 1;
-    cfg.graphs = (best_v >> 1) & 1;
-    cfg.tile_waves = (best_v >> 2) & 1 ? 0 : 4;
+    pick(best_v);
     factor();
     return nv;
   }
@@ -1275,7 +1290,7 @@ void fetidp_config_default(fetidp_config* cfg) {
   cfg->verbose = 0;
   cfg->inner_tolerance = 1e-2;
   cfg->graphs = 1;
-  cfg->side_stream = 1;
+  cfg->side_stream = 2;
   cfg->tile_waves = 4;
 }
 
@@ -1290,6 +1305,7 @@ static bool normalized_config(const fetidp_config* in, fetidp_config& out) {
   if (out.max_refinements <= 0) out.max_refinements = 12;
   if (!(out.inner_tolerance > 0)) out.inner_tolerance = 1e-2;
   if (out.tile_waves != 0) out.tile_waves = 4;
+  if (out.side_stream < 0 || out.side_stream > 2) out.side_stream = 2;
   return true;
 }
 

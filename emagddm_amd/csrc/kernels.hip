@@ -308,50 +308,138 @@ __device__ __forceinline__ float readlane(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Cholesky of the 64x64 tile held in LDS (column-major, ld kLd) by one wave and
-// the inverse of the factor: lane i keeps row i in registers, L[k][j] reaches
-// every lane through s_readlane (fully unrolled, so k and j are constants);
-// then lane c builds column c of L^-1 from broadcast LDS reads of L.
-// On return s holds L (strict upper part zeroed), si holds L^-1, rdg 1/diag.
-template <typename T>
-__device__ __forceinline__ int wave_potrf_inv(T* s, T* si, T* rdg, int lane) {
-  T r[64];
-#pragma unroll
-  for (int j = 0; j < 64; ++j) r[j] = s[j * kLd + lane];
-  int bad = 0;
-#pragma unroll
-  for (int k = 0; k < 64; ++k) {
-    T v = T(0);
-#pragma unroll
-    for (int j = 0; j < k; ++j) v += r[j] * readlane(r[j], k);
-    const T piv = readlane(r[k] - v, k);
-    T d;
-    if (!(piv > T(0)) || !(piv < T(3.0e38))) {
-      bad = 1;
-      d = T(1);
-    } else {
-      d = sqrt(piv);
-    }
-    const T rd = T(1) / d;
-    r[k] = lane == k ? d : (lane > k ? (r[k] - v) * rd : T(0));
-    if (lane == k) rdg[k] = rd;
-  }
-#pragma unroll
-  for (int j = 0; j < 64; ++j) s[j * kLd + lane] = r[j];
+__device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  T y[64];
+}
+
+// Blocked Cholesky of the 64x64 tile in s (LDS, column-major, ld kLd) by the whole
+// work-group (call uniformly).  Panel b (columns 16b .. 16b+15): wave 0 factors it
+// over all 64 rows (lane = row; each pivot and multiplier reaches every lane by
+// readlane, and the panel's later columns take their update right after each pivot,
+// so only the pivot chain is serial); the four waves then subtract the panel's
+// rank-16 product from the trailing 16x16 blocks on MFMA.  On return s holds L in
+// its lower triangle (zeros above the diagonal inside the diagonal blocks; the blocks
+// above them are untouched) and rdg 1/diag.  Returns 1 (wave 0) when a pivot was not
+// positive and finite; that column then uses pivot 1.
+template <typename T>
+__device__ __forceinline__ int tile_potrf(T* s, T* rdg, int tid) {
+  using acc_t = typename Mfma<T>::acc_t;
+  const int lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  int bad = 0;
 #pragma unroll
-  for (int i = 0; i < 64; ++i) {
-    T acc = T(0);
+  for (int b = 0; b < 4; ++b) {
+    const int c0 = 16 * b;
+    if (wave == 0) {
+      T r[16];
 #pragma unroll
-    for (int k = 0; k < i; ++k) acc += s[k * kLd + i] * y[k];
-    y[i] = i < lane ? T(0) : (i == lane ? rdg[i] : -acc * rdg[i]);
+      for (int u = 0; u < 16; ++u) r[u] = s[(c0 + u) * kLd + lane];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const T piv = readlane(r[k], c0 + k);
+        const bool ok = piv > T(0) && piv < T(3.0e38);
+        bad |= !ok;
+        const T rd = ok ? rsqrt(piv) : T(1);
+        const T d = ok ? piv * rd : T(1);
+        r[k] = lane == c0 + k ? d : (lane > c0 + k ? r[k] * rd : T(0));
+        if (lane == c0 + k) rdg[c0 + k] = rd;
+#pragma unroll
+        for (int j = k + 1; j < 16; ++j) r[j] -= r[k] * readlane(r[k], c0 + j);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) s[(c0 + u) * kLd + lane] = r[u];
+    }
+    __syncthreads();
+    if (b < 3) {
+      // trailing blocks (I, J), b < J <= I <= 3, column by column; wave w takes
+      // blocks w and w + 4.  D[p][q] = sum_k L(16J+p, k) L(16I+q, k) is the update
+      // of C(16I+q, 16J+p).
+      constexpr int nblk_tab[3] = {6, 3, 1};
+      for (int m = wave; m < nblk_tab[b]; m += 4) {
+        int J = b + 1, mm = m;
+        while (mm >= 4 - J) {
+          mm -= 4 - J;
+          ++J;
+        }
+        const int I = J + mm;
+        acc_t acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = s[(16 * J + Mfma<T>::row(lk, r)) * kLd + 16 * I + lr];
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 4) {
+          const int kc = c0 + kk + lk;
+          acc = Mfma<T>::op(-s[kc * kLd + 16 * J + lr], s[kc * kLd + 16 * I + lr], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s[(16 * J + Mfma<T>::row(lk, r)) * kLd + 16 * I + lr] = acc[r];
+      }
+      __syncthreads();
+    }
   }
-#pragma unroll
-  for (int i = 0; i < 64; ++i) si[lane * kLd + i] = y[i];
   return bad;
+}
+
+// In-place inverse of the lower-triangular factor in s (as left by tile_potrf; rdg
+// holds 1/diag) by the whole work-group (call uniformly).  Diagonal blocks: wave w
+// inverts block w by forward substitution (lane = column).  Block columns then go
+// right to left, b = 2, 1, 0: with the trailing inverse Y already in place, block
+// (I, b) of the inverse is -(sum_{K=b+1}^{I} Y_IK L_Kb) Y_bb, wave I-b-1 forming the
+// sum on MFMA, staging it in tmp (3 x 16 x 17 elements of LDS) and multiplying by
+// Y_bb; the blocks are written once every wave has read column b of L.  On return
+// the lower triangle of s holds L^-1 (blocks above the diagonal blocks untouched).
+template <typename T>
+__device__ __forceinline__ void tile_trtri(T* s, const T* rdg, T* tmp, int tid) {
+  using acc_t = typename Mfma<T>::acc_t;
+  const int lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  {
+    const int c0 = 16 * wave;
+    T y[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < i; ++k) acc += s[(c0 + k) * kLd + c0 + i] * y[k];
+      const T rdi = rdg[c0 + i];
+      y[i] = i < lr ? T(0) : (i == lr ? rdi : -acc * rdi);
+    }
+    wave_sync_lds();  // every lane has read block w of L
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if ((i >> 2) == lk) s[(c0 + lr) * kLd + c0 + i] = y[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 2; b >= 0; --b) {
+    acc_t col = acc_t{0, 0, 0, 0};
+    const int I = b + 1 + wave;
+    if (I <= 3) {
+      // W(i, c) = sum_K sum_k Y(16I+i, 16K+k) L(16K+k, 16b+c): p = i, q = c
+      acc_t w = acc_t{0, 0, 0, 0};
+      for (int K = b + 1; K <= I; ++K)
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 4) {
+          const int kc = kk + lk;
+          w = Mfma<T>::op(s[(16 * K + kc) * kLd + 16 * I + lr], s[(16 * b + lr) * kLd + 16 * K + kc], w);
+        }
+      T* tw = tmp + wave * (16 * 17);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tw[Mfma<T>::row(lk, r) * 17 + lr] = w[r];  // tw[i*17 + c]
+      wave_sync_lds();
+      // block (I, b) = -W Y_bb: A(p=i, k) = -W(i, k), B(q=c, k) = Y(16b+k, 16b+c)
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 4) {
+        const int kc = kk + lk;
+        col = Mfma<T>::op(-tw[lr * 17 + kc], s[(16 * b + lr) * kLd + 16 * b + kc], col);
+      }
+    }
+    __syncthreads();  // column b of L has been read by every wave
+    if (I <= 3) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s[(16 * b + lr) * kLd + 16 * I + Mfma<T>::row(lk, r)] = col[r];
+    }
+    __syncthreads();
+  }
 }
 
 // acc += A_panel * B_panel^T over ktot (multiple of 16) columns.  A row panel of
@@ -406,23 +494,27 @@ __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
   __syncthreads();
 }
 
-// Cholesky of the 64x64 tile staged in s (LDS, ld kLd) by wave 0 (registers +
-// readlane) and its inverse into si (LDS); the workgroup then stores L (upper
-// part zeroed) to C and L^-1 to Li, and flags a non-positive pivot in *fail.
+// Cholesky of the 64x64 tile staged in s (LDS, ld kLd) and the inverse of the
+// factor, in place, by the whole work-group (call uniformly); L (upper part zeroed)
+// goes to C, L^-1 to Li, and a non-positive pivot is flagged in *fail.  On return s
+// holds L^-1 with zeros above the diagonal.  rdg: 64 elements of LDS, tmp: 816.
 template <typename T>
-__device__ __forceinline__ void factor_diag_tile(T* s, T* si, T* __restrict__ C, T* __restrict__ Li,
-                                                 int* fail, int tid, int wave, int lane) {
-  __shared__ T rdg[64];
-  if (wave == 0) {
-    const int bad = wave_potrf_inv<T>(s, si, rdg, lane);
-    if (bad && lane == 0) atomicOr(fail, 1);
-  }
-  __syncthreads();
+__device__ __forceinline__ void factor_diag_tile(T* s, T* rdg, T* tmp, T* __restrict__ C,
+                                                 T* __restrict__ Li, int* fail, int tid) {
+  const int bad = tile_potrf<T>(s, rdg, tid);
+  if (bad && tid == 0) atomicOr(fail, 1);
   for (int e = tid; e < kTileElems; e += 256) {
     const int i = e & 63, j = e >> 6;
     C[e] = (i >= j) ? s[j * kLd + i] : T(0);
-    Li[e] = si[j * kLd + i];
   }
+  __syncthreads();  // L is stored before the inverse overwrites it
+  tile_trtri<T>(s, rdg, tmp, tid);
+  for (int e = tid; e < kTileElems; e += 256) {
+    const int i = e & 63, j = e >> 6;
+    if ((i >> 4) < (j >> 4)) s[j * kLd + i] = T(0);  // blocks above the diagonal blocks
+  }
+  __syncthreads();
+  for (int e = tid; e < kTileElems; e += 256) Li[e] = s[(e >> 6) * kLd + (e & 63)];
 }
 
 // Row i of a 64x64 column-major block M (element (i, k) at k*ld + i) times v, in the
@@ -452,13 +544,13 @@ __device__ __forceinline__ double block_row_dot64(const T* M, int ld, const doub
 //         flags & 1 the diagonal tile of row I also takes its share C_II -= X X^T
 //         (column step, I > J; every WG of a launch owns a distinct row I, apart
 //         from a lookahead item, flags & 2, which only updates the next column's
-//         tile (J + 2, J + 1))
+//         tile (J + 2, J + 1)).  Items with flags & 16 do the MODE 3 work; the
+//         schedule puts them at the front of the launch so they are dispatched first,
+//         and their waves run at raised priority (they are on the critical path).
 // MODE 2: same (empty K range when the diagonal was updated incrementally), then
 //         Cholesky of the diagonal tile and its inverse
 // MODE 3: MODE 1 for the item I = J + 1, whose share completes C_II, then the
-//         Cholesky and inverse of C_II for the next column step (run on a second
-//         stream beside the MODE 1 launch of the same column; ~256 VGPRs, so it
-//         is kept out of MODE 1)
+//         Cholesky and inverse of C_II for the next column step
 // The inverse diagonal tile of column J of matrix m lives at inv + inv_base[m] + J*4096.
 template <typename T, int MODE>
 __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
@@ -469,6 +561,9 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
                                             int* __restrict__ fail, RhsArgs rhs) {
   using acc_t = typename Mfma<T>::acc_t;
   const TileWork w = work[blockIdx.x];
+  // uniform per work-group: this item completes and factors the next diagonal tile
+  const bool fac = MODE == 3 || (MODE == 1 && (w.flags & 16));
+  if (MODE == 1 && fac) __builtin_amdgcn_s_setprio(3);
   // envelope-packed storage: tile (I, J) at ra[I] + J * 4096
   const int64_t* ra = rowabs + rowbase[w.mat];
   T* C = pool + ra[w.I] + (int64_t)w.J * kTileElems;
@@ -484,9 +579,14 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
     for (int y = 0; y < 2; ++y) acc[x][y] = acc_t{0, 0, 0, 0};
   // LDS: the panel staging buffers are dead after panel_mma; the same space then
-  // holds the inverse tile of a diagonal factorization.
+  // holds the staged tile (and, in place, its factor and inverse).  aux holds the
+  // partial dot products of the right-hand side or the inverse's staging blocks,
+  // never both at once.
   constexpr int kStage = 2 * 64 * 16, kPanelLds = 2 * kStage > 64 * kLd ? 2 * kStage : 64 * kLd;
+  constexpr int kTmp = 3 * 16 * 17;
+  constexpr int kAux = (kTmp * (int)sizeof(T) + 7) / 8 > 256 ? (kTmp * (int)sizeof(T) + 7) / 8 : 256;
   __shared__ T lds_panel[kPanelLds];
+  __shared__ double aux[kAux];
   panel_mma<T>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
                pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, lds_panel,
                lds_panel + kStage, tid, pb0, qb0, lr, lk);
@@ -505,11 +605,10 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
     return;
   }
   if constexpr (MODE != 0) {
-    // staging tile: in mode 1 it reuses the (dead) panel buffers, so the kernel
-    // needs 33 KB of LDS and three work-groups fit a CU; modes 2 and 3 keep the
-    // panel space for the inverse tile and need their own
-    __shared__ T s_own[MODE >= 2 ? 64 * kLd : 1];
-    T* s = MODE >= 2 ? s_own : lds_panel;
+    T* s = lds_panel;
+    __shared__ T rdg[MODE >= 1 ? 64 : 1];
+    T* tmp = reinterpret_cast<T*>(aux);
+    double(*rpart)[64] = reinterpret_cast<double(*)[64]>(aux);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -540,8 +639,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
             C[j * 64 + i] = a2[x][y][r];
             s[j * kLd + i] = a2[x][y][r];
           }
-      __shared__ double bl[MODE == 3 ? 64 : 1];
-      __shared__ double rpart[4][64];
+      __shared__ double bl[64];
       if (rhs.b) {  // uniform: b_I -= L_IJ y_J (fixed summation order, wave 0 writes)
         __syncthreads();
         const int64_t o = rhs.off[w.mat];
@@ -550,10 +648,10 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
           double* bi = rhs.b + o + (int64_t)w.I * 64;
           const double nb = bi[lane] - dot;
           bi[lane] = nb;
-          if constexpr (MODE == 3) bl[lane] = nb;
+          if (fac) bl[lane] = nb;
         }
       }
-      if (MODE == 1 && !(w.flags & 1)) return;  // uniform per workgroup
+      if (!fac && !(w.flags & 1)) return;  // uniform per workgroup
       __syncthreads();
       // C_II -= X X^T
       acc_t a3[2][2];
@@ -564,7 +662,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       tile_mma<T, kLd, kLd>(a3, s, s, pb0, qb0, lr, lk);
       T* Cd = pool + ra[w.I] + (int64_t)w.I * kTileElems;
       const bool dfresh = w.flags & 8;  // flags & 8: the diagonal tile is still unwritten
-      if constexpr (MODE == 1) {
+      if (!fac) {
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -587,22 +685,19 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
               s[j * kLd + i] = (dfresh ? T(0) : Cd[j * 64 + i]) - a3[x][y][r];
             }
         __syncthreads();
-        factor_diag_tile<T>(s, lds_panel, Cd, inv + inv_base[w.mat] + (int64_t)w.I * kTileElems,
-                            fail + w.mat, tid, wave, lane);
-        if constexpr (MODE == 3)
-          if (rhs.b) {  // y_I = L_II^-1 b_I (b_I complete in bl)
-            __syncthreads();
-            const double yv = block_row_dot64(lds_panel, kLd, bl, rpart, wave, lane);
-            if (wave == 0) rhs.y[rhs.off[w.mat] + (int64_t)w.I * 64 + lane] = yv;
-          }
+        factor_diag_tile<T>(s, rdg, tmp, Cd, inv + inv_base[w.mat] + (int64_t)w.I * kTileElems,
+                            fail + w.mat, tid);
+        if (rhs.b) {  // y_I = L_II^-1 b_I (b_I complete in bl; s holds L_II^-1)
+          const double yv = block_row_dot64(s, kLd, bl, rpart, wave, lane);
+          if (wave == 0) rhs.y[rhs.off[w.mat] + (int64_t)w.I * 64 + lane] = yv;
+        }
       }
     } else {
-      factor_diag_tile<T>(s, lds_panel, C, inv + inv_base[w.mat] + (int64_t)w.J * kTileElems,
-                          fail + w.mat, tid, wave, lane);
+      factor_diag_tile<T>(s, rdg, tmp, C, inv + inv_base[w.mat] + (int64_t)w.J * kTileElems,
+                          fail + w.mat, tid);
       if (rhs.b) {  // y_J = L_JJ^-1 b_J (b_J complete in memory)
-        __shared__ double rpart2[4][64];
         const int64_t o = rhs.off[w.mat] + (int64_t)w.J * 64;
-        const double yv = block_row_dot64(lds_panel, kLd, rhs.b + o, rpart2, wave, lane);
+        const double yv = block_row_dot64(s, kLd, rhs.b + o, rpart, wave, lane);
         if (wave == 0) rhs.y[o + lane] = yv;
       }
     }

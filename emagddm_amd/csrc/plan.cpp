@@ -45,6 +45,10 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
   TileLaunch L{mode, int64_t(S.items.size()), int64_t(list.size())};
   S.items.insert(S.items.end(), list.begin(), list.end());
   S.launches.push_back(L);
+  // flags & 16 marks a factoring item, so that it can also ride at the front of the
+  // column-step launch that follows it
+  if (mode == 3)
+    for (size_t i = size_t(L.off); i < S.items.size(); ++i) S.items[i].flags |= 16;
   // modes 1 and 3 add the TRSM product (and the diagonal-tile share when flagged);
   // lookahead items (flags & 2) are update-only
   for (const TileWork& w : list) {

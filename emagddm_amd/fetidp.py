@@ -95,8 +95,9 @@ class FETIDPSolver:
         return cfg
 
     def autotune(self, repetitions=2):
-        """Time the launch variants (side stream, graph replay, tile-kernel waves) on the
-        current problem, keep the fastest; returns [(side_stream, graphs, waves, ms)]."""
+        """Time the launch variants (where the diagonal factorizations run, graph replay,
+        tile-kernel waves) on the current problem, keep the fastest; returns
+        [(side_stream, graphs, waves, ms)]."""
         buf = np.zeros(64)
         n = self._lib.fetidp_autotune(self._h, repetitions, _dp(buf), 64)
         if n < 0:

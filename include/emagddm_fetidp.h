@@ -57,8 +57,10 @@ typedef struct fetidp_config {
                                than one mean edge length, i.e. not a box partition; else
                                stiffness), 1 stiffness rho_t/(rho_s+rho_t), 2 deluxe
                                (S_s,FF + S_t,FF)^-1 S_t,FF per face (FP64) */
-  int32_t side_stream;      /* 1: diagonal-tile factorizations overlap the column steps on a
-                               second stream (default); 0: one stream */
+  int32_t side_stream;      /* where the diagonal-tile factorizations of the column steps
+                               run: 2 at the front of the column-step launch, dispatched
+                               first at raised wave priority (default); 1 on a second
+                               stream beside it; 0 as a launch of their own, one stream */
   int32_t tile_waves;       /* throughput tile kernels: 4 = capped at 128 registers for four
                                waves per SIMD (default), 0 = the compiler's own allocation */
   int32_t reserved;

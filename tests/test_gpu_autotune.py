@@ -16,8 +16,8 @@ def test_autotune_keeps_results(gpu_available):
     s.setup_problem(p)
     s.set_rhs_tet_J(p.jq)
     rows = s.autotune(repetitions=1)
-    assert len(rows) == 8
-    assert {(a, b, c) for a, b, c, _ in rows} == {(a, b, c) for a in (0, 1) for b in (0, 1)
+    assert len(rows) == 12
+    assert {(a, b, c) for a, b, c, _ in rows} == {(a, b, c) for a in (0, 1, 2) for b in (0, 1)
                                                    for c in (0, 4)}
     best = min(rows, key=lambda r: r[3])
     cfg = s.get_config()
