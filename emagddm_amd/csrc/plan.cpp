@@ -65,40 +65,58 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
 // Left-looking diagonal (inc = false): one rounding per diagonal entry and Schur
 // step, used for FP32 factors where repeated rounding of the diagonal matters.
 // With incremental diagonals, the item (j + 1, j) of column j gives C_{j+1,j+1}
-// its last share, so it runs as a mode-3 launch that also factors that tile,
-// on a second stream beside the mode-1 launch of the other rows (the diagonal
-// launch of column j + 1 disappears).  Per column: [mode 2][mode 3][mode 1].
-// One matrix alone has one mode-3 item per column, on the critical path, whose K loop
-// grows with j; so the mode-1 launch of column j also carries a lookahead item
-// (flags 2: update only) that applies K in [.., j) to tile (j + 2, j + 1), and the
-// mode-3 item of column j + 1 keeps only its last product, K = j.
+// its last share, so it runs as a mode-3 item that also factors that tile for the
+// next column (the diagonal launch of column j + 1 disappears).
+// Per column: [mode 2][mode 3][mode 1].
+// One matrix alone has few items per column, and the column launches are a chain
+// whose links are the longest item of each.  So with incremental diagonals every
+// tile's K range [k0, E) (E = J for a column-step tile, E = Tp for a Schur tile) is
+// cut into pieces: its own item applies only the last p products (p = 1 for
+// column-step tiles, 1 + (I + J) % kChunk for Schur tiles, which spreads their
+// pieces over the launches), and update-only items (flags 2) of earlier column
+// launches apply the rest in chunks of at most kChunk products, the chunk that ends
+// at K = e riding in the launch of column e (columns < e are final there).  No item
+// of a column launch loops over more than kChunk products.
 TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc, const int32_t* ft) {
+  constexpr int kChunk = 8;
   TileSchedule S;
   auto f = [&](int I) { return ft ? ft[I] : 0; };
+  auto k0_of = [&](int I, int J) { return std::max(f(I), f(J)); };
+  // start of the part of tile (I, J)'s K range its own item applies
+  auto tail = [&](int I, int J) {
+    const int E = std::min(J, Tp), p = J < Tp ? 1 : 1 + (I + J) % kChunk;
+    return std::max(k0_of(I, J), E - p);
+  };
+  // chunks of tile (I, J) end at tail, tail - kChunk, ...: the chunk ending at e
+  // (e > k0) is [max(k0, e - kChunk), e) and rides in the launch of column e
+  auto chunk_in = [&](int I, int J, int e, TileWork& w) {
+    const int k0 = k0_of(I, J), t = tail(I, J);
+    if (e <= k0 || e > t || (t - e) % kChunk) return false;
+    w = TileWork{0, I, J, std::max(k0, e - kChunk), e, 2};
+    return true;
+  };
   for (int j = 0; j < Tp; ++j) {
     if (!(inc && j > 0 && f(j) <= j - 1))  // else factored by column j-1's mode-3 item
       add_tiles(S, 2, {TileWork{0, j, j, inc ? 0 : f(j), inc ? 0 : j, 0}});
     std::vector<TileWork> next, below;
     for (int I = j + 1; I < Ta; ++I)
       if (f(I) <= j) {
-        TileWork w{0, I, j, std::max(f(I), f(j)), j, inc ? 1 : 0};
-        const bool is_next = inc && I == j + 1 && I < Tp;
-        // the lookahead of column j - 1 applied K < j - 1 to this tile already
-        if (is_next && j >= 1 && f(I) <= j - 1 && w.k0 < j - 1) w.k0 = j - 1;
-        (is_next ? next : below).push_back(w);
+        TileWork w{0, I, j, inc ? tail(I, j) : k0_of(I, j), j, inc ? 1 : 0};
+        (inc && I == j + 1 && I < Tp ? next : below).push_back(w);
       }
-    // lookahead for the mode-3 item (j + 2, j + 1) of the next column
-    if (inc && j + 2 < Tp && f(j + 2) <= j + 1 && f(j + 2) <= j) {
-      const int k0 = std::max(f(j + 2), f(j + 1));
-      if (k0 < j) below.push_back(TileWork{0, j + 2, j + 1, k0, j, 2});
-    }
+    if (inc)
+      for (int J = j + 1; J < Ta; ++J)
+        for (int I = J + 1; I < Ta; ++I) {
+          TileWork w;
+          if (f(I) < j && f(J) < j && chunk_in(I, J, j, w)) below.push_back(w);
+        }
     add_tiles(S, 3, next);
     add_tiles(S, 1, below);
   }
   std::vector<TileWork> schur;
   for (int J = Tp; J < Ta; ++J)
     for (int I = inc ? J + 1 : J; I < Ta; ++I) {
-      const int k0 = std::max(f(I), f(J));
+      const int k0 = inc ? tail(I, J) : k0_of(I, J);
       if (k0 < Tp) schur.push_back(TileWork{0, I, J, k0, Tp, 0});
     }
   add_tiles(S, 0, schur);
@@ -651,24 +669,30 @@ void build_plans(const Topology& T, Plans& P) {
       check(w.k0 >= w.k1 || (w.k0 >= P.coarse_ftile[w.I] && w.k0 >= P.coarse_ftile[w.J]),
             "coarse tile panel");
     }
-    // every stored tile (I, J) below the diagonal, J an eliminated column, receives
-    // the K range [max(f_I, f_J), J) exactly once, split between a lookahead item and
-    // its own item at most (both listed in launch order)
+    // every stored tile (I, J) below the diagonal receives the K range
+    // [max(f_I, f_J), min(J, Tp)) exactly once, in pieces listed in launch order
+    // (update-only chunks, then its own item); no column-launch item other than a
+    // first diagonal loops over more than 8 products
     const int Tp = int(P.coarse_ncp / kTile);
-    std::vector<int32_t> next_k(size_t(Ta) * size_t(Tp), -1);
+    std::vector<int32_t> next_k(size_t(Ta) * size_t(Ta), -1);
     for (const TileLaunch& L : P.coarse_sched.launches)
       for (int64_t i = L.off; i < L.off + L.n; ++i) {
         const TileWork& w = P.coarse_sched.items[size_t(i)];
-        if (w.J >= Tp || w.I == w.J || L.mode == 0) continue;
-        int32_t& nk = next_k[size_t(w.I) * Tp + w.J];
+        if (L.mode == 1 || L.mode == 3) check(w.k1 - w.k0 <= 8, "coarse chunk length");
+        if (w.I == w.J) continue;
+        int32_t& nk = next_k[size_t(w.I) * Ta + w.J];
         const int32_t start = std::max(P.coarse_ftile[w.I], P.coarse_ftile[w.J]);
         if (nk < 0) nk = start;
         check(w.k0 == nk || w.k0 >= w.k1, "coarse K ranges contiguous");
         if (w.k1 > w.k0) nk = w.k1;
       }
     for (int I = 0; I < Ta; ++I)
-      for (int J = P.coarse_ftile[I]; J < std::min(I, Tp); ++J)
-        check(next_k[size_t(I) * Tp + J] == J, "coarse K ranges complete");
+      for (int J = P.coarse_ftile[I]; J < I; ++J) {
+        const int32_t nk = next_k[size_t(I) * Ta + J];
+        const int32_t start = std::max(P.coarse_ftile[I], P.coarse_ftile[J]);
+        if (J < Tp) check(nk == J, "coarse K ranges complete");
+        else if (start < Tp) check(nk == Tp, "coarse Schur K ranges complete");
+      }
   }
   for (int64_t y : P.c_ypos) check(y >= 0 && y < T.y_elems, "coarse y position");
   {
