@@ -278,17 +278,17 @@ void launch_mfma_probe(int is_f32, double* out) {
     hipLaunchKernelGGL(k_mfma_probe<double>, dim3(1), dim3(64), 0, nullptr, out);
 }
 
-// acc (32x32 per wave, 2x2 MFMA blocks) += A_p,k * B_k,q over one 64-deep tile,
-// A operand read from Ap (column-major 64x64, element (p,k) at k*lda + p),
+// acc (32x32 per wave, 2x2 MFMA blocks) += A_p,k * B_k,q over one 64-deep tile (-=
+// with NEG), A operand read from Ap (column-major 64x64, element (p,k) at k*lda + p),
 // B operand from Bq (element (q,k) at k*ldb + q).
-template <typename T, int LDA, int LDB>
+template <typename T, int LDA, int LDB, bool NEG = false>
 __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[2][2], const T* Ap,
                                          const T* Bq, int pb0, int qb0, int lr, int lk) {
 #pragma unroll 4
   for (int kk = 0; kk < 64; kk += 4) {
     const int kc = kk + lk;
-    const T a0 = Ap[kc * LDA + pb0 + lr];
-    const T a1 = Ap[kc * LDA + pb0 + 16 + lr];
+    const T a0 = NEG ? -Ap[kc * LDA + pb0 + lr] : Ap[kc * LDA + pb0 + lr];
+    const T a1 = NEG ? -Ap[kc * LDA + pb0 + 16 + lr] : Ap[kc * LDA + pb0 + 16 + lr];
     const T b0 = Bq[kc * LDB + qb0 + lr];
     const T b1 = Bq[kc * LDB + qb0 + 16 + lr];
     acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
@@ -442,7 +442,7 @@ __device__ __forceinline__ void tile_trtri(T* s, const T* rdg, T* tmp, int tid) 
   }
 }
 
-// acc += A_panel * B_panel^T over ktot (multiple of 16) columns.  A row panel of
+// acc -= A_panel * B_panel^T over ktot (multiple of 16) columns.  A row panel of
 // tiles (J, k0 .. k1) is contiguous: element (p, k) at k*64 + p, so the K loop is
 // one panel product.  64x16 chunks of both panels are staged through LDS (two
 // stages, next chunk prefetched into registers), so each element crosses L2 once
@@ -466,7 +466,7 @@ __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
     T* bb = sB + (c & 1) * CH;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      a[tid + 256 * u] = ra[u];
+      a[tid + 256 * u] = -ra[u];
       bb[tid + 256 * u] = rb[u];
     }
     __syncthreads();
@@ -573,11 +573,23 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
   const int lr = lane & 15, lk = lane >> 4;
   // D = C^T: rows p of D are columns j of C, columns q of D are rows i of C.
   const int pb0 = (wave >> 1) * 32, qb0 = (wave & 1) * 32;
+  // FP64: the accumulator starts from C (loaded before the panel loads are issued, so
+  // its latency overlaps them) and takes -sum_K L_IK L_JK^T.  FP32 sums the products
+  // from zero and adds C once at the end (one rounding against C; accumulating into C
+  // costs the FP32 factors accuracy they do not have to spare).
+  constexpr bool from_c = sizeof(T) == 8;
   acc_t acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) acc[x][y] = acc_t{0, 0, 0, 0};
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        acc[x][y][r] = (!from_c || fresh) ? T(0) : C[(pb0 + 16 * x + Mfma<T>::row(lk, r)) * 64 + qb0 + 16 * y + lr];
+  // C + acc: the updated tile (element (i, j) at j * 64 + i)
+  auto updated = [&](int x, int y, int r, int j, int i) {
+    return (from_c || fresh) ? acc[x][y][r] : C[j * 64 + i] + acc[x][y][r];
+  };
   // LDS: the panel staging buffers are dead after panel_mma; the same space then
   // holds the staged tile (and, in place, its factor and inverse).  aux holds the
   // partial dot products of the right-hand side or the inverse's staging blocks,
@@ -600,7 +612,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-          C[j * 64 + i] = (fresh ? T(0) : C[j * 64 + i]) - acc[x][y][r];
+          C[j * 64 + i] = updated(x, y, r, j, i);
         }
     return;
   }
@@ -616,10 +628,27 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-          s[j * kLd + i] = (fresh ? T(0) : C[j * 64 + i]) - acc[x][y][r];
+          s[j * kLd + i] = updated(x, y, r, j, i);
         }
     __syncthreads();
     if constexpr (MODE == 1 || MODE == 3) {
+      // the diagonal share's accumulator starts from C_II, loaded ahead of the TRSM
+      const bool share = fac || (w.flags & 1);  // uniform per work-group
+      const bool dfresh = w.flags & 8;  // flags & 8: the diagonal tile is still unwritten
+      T* Cd = pool + ra[w.I] + (int64_t)w.I * kTileElems;
+      acc_t a3[2][2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            a3[x][y][r] = (!from_c || !share || dfresh)
+                              ? T(0)
+                              : Cd[(pb0 + 16 * x + Mfma<T>::row(lk, r)) * 64 + qb0 + 16 * y + lr];
+      auto shared_d = [&](int x, int y, int r, int j, int i) {
+        return (from_c || dfresh) ? a3[x][y][r] : Cd[j * 64 + i] + a3[x][y][r];
+      };
       // X[i][j] = sum_k C'[i][k] Linv[j][k]:  D2[p=j][q=i], A op = Linv (p,k), B op = C' (q,k)
       const T* Li = inv + inv_base[w.mat] + (int64_t)w.J * kTileElems;
       acc_t a2[2][2];
@@ -654,14 +683,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       if (!fac && !(w.flags & 1)) return;  // uniform per workgroup
       __syncthreads();
       // C_II -= X X^T
-      acc_t a3[2][2];
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) a3[x][y] = acc_t{0, 0, 0, 0};
-      tile_mma<T, kLd, kLd>(a3, s, s, pb0, qb0, lr, lk);
-      T* Cd = pool + ra[w.I] + (int64_t)w.I * kTileElems;
-      const bool dfresh = w.flags & 8;  // flags & 8: the diagonal tile is still unwritten
+      tile_mma<T, kLd, kLd, true>(a3, s, s, pb0, qb0, lr, lk);
       if (!fac) {
 #pragma unroll
         for (int x = 0; x < 2; ++x)
@@ -670,7 +692,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-              Cd[j * 64 + i] = (dfresh ? T(0) : Cd[j * 64 + i]) - a3[x][y][r];
+              Cd[j * 64 + i] = shared_d(x, y, r, j, i);
             }
       } else {
         // C_II is complete: stage it in s once every wave has finished reading X
@@ -682,7 +704,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-              s[j * kLd + i] = (dfresh ? T(0) : Cd[j * 64 + i]) - a3[x][y][r];
+              s[j * kLd + i] = shared_d(x, y, r, j, i);
             }
         __syncthreads();
         factor_diag_tile<T>(s, rdg, tmp, Cd, inv + inv_base[w.mat] + (int64_t)w.I * kTileElems,
@@ -981,29 +1003,52 @@ template void launch_forward<float>(int, int, int, const DevSub*, const float*, 
                                     const float*, const int64_t*, const int32_t*, double*, double*,
                                     hipStream_t);
 
-// t_K = y_K - sum_{I in [Ti, Tc)} L_IK^T U_I for every interior tile column K
+// Interface part of the interior solve, for every interior tile column K:
+// t_K = y_K - sum_{I in [Ti, Tc)} L_IK^T U_I.  1024 threads per (K, subdomain): quarter q
+// sums the rows I = Ti+q, Ti+q+4, ..., quarter 0 adds the partials in order.  The
+// work-group of the last interior column also forms x_K = L_KK^-T t_K, which starts
+// the backward steps.
 template <typename T>
-__global__ __launch_bounds__(256) void k_int_prep(const DevSub* __restrict__ subs,
-                                                  const T* __restrict__ pool,
-                                                  const int64_t* __restrict__ rowabs,
-                                                  const int32_t* __restrict__ ftile,
-                                                  const double* __restrict__ yv,
-                                                  const double* __restrict__ U,
-                                                  double* __restrict__ tbuf) {
+__global__ __launch_bounds__(1024) void k_int_prep(const DevSub* __restrict__ subs,
+                                                   const T* __restrict__ pool,
+                                                   const int64_t* __restrict__ rowabs,
+                                                   const T* __restrict__ inv,
+                                                   const int64_t* __restrict__ inv_base,
+                                                   const int32_t* __restrict__ ftile,
+                                                   const double* __restrict__ yv,
+                                                   const double* __restrict__ U,
+                                                   double* __restrict__ tbuf,
+                                                   double* __restrict__ xout) {
   const DevSub S = subs[blockIdx.y];
   const int K = blockIdx.x;
-  if (K >= S.Ti) return;
-  const int tid = threadIdx.x;
+  if (K >= S.Ti) return;  // uniform per work-group
+  const int q = threadIdx.x >> 8, tid = threadIdx.x & 255;
   const double* u = U + S.vec_off;
-  double s = 0.0;
-  for (int I = S.Ti; I < S.Tc; ++I)
+  double sum = 0.0;
+  for (int I = S.Ti + q; I < S.Tc; I += 4)
     if (ftile[S.ft_off + I] <= K)
-      s += colsum_part(pool + rowabs[S.ft_off + I] + (int64_t)K * kTileElems, u + I * 64, tid);
-  if ((tid & 3) == 0) tbuf[S.vec_off + K * 64 + (tid >> 2)] = yv[S.vec_off + K * 64 + (tid >> 2)] - s;
+      sum += colsum_part(pool + rowabs[S.ft_off + I] + (int64_t)K * kTileElems, u + (int64_t)I * 64, tid);
+  __shared__ double part[4][64];
+  __shared__ double ts[64];
+  if ((tid & 3) == 0) part[q][tid >> 2] = sum;
+  __syncthreads();
+  if (q == 0 && tid < 64) {
+    const int64_t o = S.vec_off + (int64_t)K * 64 + tid;
+    const double t = yv[o] - (((part[0][tid] + part[1][tid]) + part[2][tid]) + part[3][tid]);
+    tbuf[o] = t;
+    ts[tid] = t;
+  }
+  if (K != S.Ti - 1) return;  // uniform per work-group
+  __syncthreads();
+  if (q == 0) {
+    const double x = colsum_part(inv + inv_base[blockIdx.y] + (int64_t)K * kTileElems, ts, tid);
+    if ((tid & 3) == 0) xout[S.vec_off + (int64_t)K * 64 + (tid >> 2)] = x;
+  }
 }
 
-// Backward step J: x_J = L_JJ^-T t_J (row-J workgroup writes x);
-// rows I < J: t_I -= L_JI^T x_J.
+// Backward step J >= 1 (x_J known): rows I < J of the envelope take t_I -= L_JI^T x_J
+// (work-group I reads tile (J, I) of the contiguous tile row J); the work-group of row
+// J - 1, whose t is then complete, also forms x_{J-1} = L_{J-1,J-1}^-T t_{J-1}.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
@@ -1016,19 +1061,23 @@ __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ sub
   const DevSub S = subs[blockIdx.y];
   if (J >= S.Ti) return;
   const int I = blockIdx.x;
-  if (I > J || I < ftile[S.ft_off + J]) return;  // tile (J, I) outside the envelope
-  __shared__ double xs[64];
+  const bool in_env = I >= ftile[S.ft_off + J];  // tile (J, I) is stored
+  if (I >= J || (!in_env && I != J - 1)) return;
   const int tid = threadIdx.x;
   double* t = tbuf + S.vec_off;
-  const double x = colsum_part(inv + inv_base[blockIdx.y] + (int64_t)J * kTileElems, t + J * 64, tid);
-  if ((tid & 3) == 0) xs[tid >> 2] = x;
-  __syncthreads();
-  if (I == J) {
-    if (tid < 64) xout[S.vec_off + J * 64 + tid] = xs[tid];
-    return;
+  const double s = in_env ? colsum_part(pool + rowabs[S.ft_off + J] + (int64_t)I * kTileElems,
+                                        xout + S.vec_off + (int64_t)J * 64, tid)
+                          : 0.0;
+  __shared__ double ts[64];
+  if ((tid & 3) == 0) {
+    const double nt = t[I * 64 + (tid >> 2)] - s;
+    t[I * 64 + (tid >> 2)] = nt;
+    ts[tid >> 2] = nt;
   }
-  const double s = colsum_part(pool + rowabs[S.ft_off + J] + (int64_t)I * kTileElems, xs, tid);
-  if ((tid & 3) == 0) t[I * 64 + (tid >> 2)] -= s;
+  if (I != J - 1) return;  // uniform per work-group
+  __syncthreads();
+  const double x = colsum_part(inv + inv_base[blockIdx.y] + (int64_t)I * kTileElems, ts, tid);
+  if ((tid & 3) == 0) xout[S.vec_off + (int64_t)I * 64 + (tid >> 2)] = x;
 }
 
 template <typename T>
@@ -1036,10 +1085,10 @@ void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, cons
                      const T* inv, const int64_t* inv_base, const int32_t* ftile, const double* yv,
                      const double* U, double* tbuf, double* xout, hipStream_t st) {
   if (ns <= 0 || max_Ti <= 0) return;
-  hipLaunchKernelGGL(k_int_prep<T>, dim3(max_Ti, ns), dim3(256), 0, st, subs, pool, rowabs, ftile, yv,
-                     U, tbuf);
-  for (int J = max_Ti - 1; J >= 0; --J)
-    hipLaunchKernelGGL(k_bwd_step<T>, dim3(J + 1, ns), dim3(256), 0, st, subs, pool, rowabs, inv,
+  hipLaunchKernelGGL(k_int_prep<T>, dim3(max_Ti, ns), dim3(1024), 0, st, subs, pool, rowabs, inv,
+                     inv_base, ftile, yv, U, tbuf, xout);
+  for (int J = max_Ti - 1; J >= 1; --J)
+    hipLaunchKernelGGL(k_bwd_step<T>, dim3(J, ns), dim3(256), 0, st, subs, pool, rowabs, inv,
                        inv_base, ftile, J, tbuf, xout);
 }
 template void launch_interior<double>(int, int, const DevSub*, const double*, const int64_t*,
