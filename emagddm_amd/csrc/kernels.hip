@@ -596,9 +596,20 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
   // never both at once.
   constexpr int kStage = 2 * 64 * 16, kPanelLds = 2 * kStage > 64 * kLd ? 2 * kStage : 64 * kLd;
   constexpr int kTmp = 3 * 16 * 17;
-  constexpr int kAux = (kTmp * (int)sizeof(T) + 7) / 8 > 256 ? (kTmp * (int)sizeof(T) + 7) / 8 : 256;
+  // aux: partial dot products [4][64] + y_J [64], or the inverse's staging blocks
+  constexpr int kAux = (kTmp * (int)sizeof(T) + 7) / 8 > 320 ? (kTmp * (int)sizeof(T) + 7) / 8 : 320;
   __shared__ T lds_panel[kPanelLds];
   __shared__ double aux[kAux];
+  // right-hand side operands of a column-step item, fetched ahead: y_J into aux[256..320)
+  // (read after later barriers), b_I into a register of wave 0
+  double bpre = 0.0;
+  if constexpr (MODE == 1 || MODE == 3) {
+    if (rhs.b && !(MODE == 1 && (w.flags & 2))) {  // uniform per work-group
+      const int64_t o = rhs.off[w.mat];
+      if (tid < 64) aux[256 + tid] = rhs.y[o + (int64_t)w.J * 64 + tid];
+      if (wave == 0) bpre = rhs.b[o + (int64_t)w.I * 64 + lane];
+    }
+  }
   panel_mma<T>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
                pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, lds_panel,
                lds_panel + kStage, tid, pb0, qb0, lr, lk);
@@ -671,12 +682,10 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       __shared__ double bl[64];
       if (rhs.b) {  // uniform: b_I -= L_IJ y_J (fixed summation order, wave 0 writes)
         __syncthreads();
-        const int64_t o = rhs.off[w.mat];
-        const double dot = block_row_dot64(s, kLd, rhs.y + o + (int64_t)w.J * 64, rpart, wave, lane);
+        const double dot = block_row_dot64(s, kLd, aux + 256, rpart, wave, lane);
         if (wave == 0) {
-          double* bi = rhs.b + o + (int64_t)w.I * 64;
-          const double nb = bi[lane] - dot;
-          bi[lane] = nb;
+          const double nb = bpre - dot;
+          rhs.b[rhs.off[w.mat] + (int64_t)w.I * 64 + lane] = nb;
           if (fac) bl[lane] = nb;
         }
       }
