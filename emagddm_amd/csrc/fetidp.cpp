@@ -820,7 +820,7 @@ struct fetidp_solver {
   void apply_F_rest(const DBuf<T>& W, double* qout) {
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
     launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
-    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
   // Per face: [[A_F, I], [I, 0]] from the FP64 Dirichlet blocks, eliminate A_F with the
@@ -865,12 +865,12 @@ struct fetidp_solver {
     launch_deluxe_out(F.vec_elems, face_map_lam.p, face_slot1.p, fv1.p, dbuf.p, zout, st);
   }
 
-  // ybuf = W x with x = B^T p gathered per subdomain: Z from its lower tiles, then Phi^T rows
+  // W x with x = B^T p gathered per subdomain: Z x as per-tile partials in sympart (summed
+  // by launch_dual_correct), Phi^T x into ybuf
   template <typename T>
   void apply_W(const DBuf<T>& W, const double* pin) {
-    launch_symv<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, W.p, 0,
-                   d_lam.p, d_sgn.p, pin, sympart.p, ybuf.p, st);
-    launch_gemv_phi<T>(topo.n_sub, plans.max_ncp, devsub.p, W.p, d_lam.p, d_sgn.p, pin, ybuf.p, st, plans.max_ndp);
+    launch_apply_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp,
+                      devsub.p, W.p, d_lam.p, d_sgn.p, pin, sympart.p, ybuf.p, st);
   }
   void apply_M(const double* rin, double* zout) {
     if (deluxe) {
@@ -904,7 +904,7 @@ struct fetidp_solver {
     launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
     launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, fc.p, uc.p, st);
     // vec_D' holds -a_s: out = a_s - Phi u
-    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, -1.0, dbuf.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, dv.p, st);
   }
 
@@ -914,7 +914,7 @@ struct fetidp_solver {
     apply_W<T>(W, lam.p);
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
     launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
-    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, ybuf.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
     timed(13, [&] {
@@ -934,22 +934,41 @@ struct fetidp_solver {
   }
 
   // One PCPG iteration as a fixed kernel chain: the scalars alpha and beta stay
-  // on the device (k_pcpg_alpha / k_pcpg_beta), so the chain can be captured.
+  // on the device, so the chain can be captured.  The lambda scatters carry the dot
+  // products; alpha and beta are formed by the launches that use them.  Bitwise the
+  // same arithmetic as apply_F / apply_M, k_dots, k_pcpg_alpha / beta and the updates.
   template <typename T>
   void pcpg_iteration_kernels(const DBuf<T>& W) {
     const int64_t nl = topo.n_lam;
-    apply_F<T>(W, p.p, q.p);
-    const double* a1[1] = {p.p};
-    const double* b1[1] = {q.p};
-    launch_dots(nl, 1, a1, b1, partial.p, st);
-    launch_pcpg_alpha(partial.p, dsc.p, hsc_dev, st);
-    launch_axpy2_dev(nl, dsc.p, p.p, q.p, lam.p, r.p, st);
-    apply_M(r.p, z.p);
+    timed(9, [&] { apply_W<T>(W, p.p); });
+    timed(10, [&] {
+      launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
+      launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
+      launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0,
+                             dbuf.p, st);
+      const double* a1[1] = {p.p};
+      const double* b1[1] = {nullptr};
+      launch_scatter_dot(nl, lam_pos.p, d_sgn.p, dbuf.p, q.p, 1, a1, b1, partial.p, st);
+      launch_alpha_axpy2(nl, partial.p, dsc.p, hsc_dev, p.p, q.p, lam.p, r.p, st);
+    });
     const double* a2[2] = {r.p, r.p};
-    const double* b2[2] = {r.p, z.p};
-    launch_dots(nl, 2, a2, b2, partial.p, st);
-    launch_pcpg_beta(partial.p, dsc.p, hsc_dev, 0, st);
-    launch_xpby_dev(nl, z.p, dsc.p, p.p, st);
+    const double* b2[2] = {r.p, nullptr};
+    if (deluxe) {
+      apply_M(r.p, z.p);
+      const double* b3[2] = {r.p, z.p};
+      launch_dots(nl, 2, a2, b3, partial.p, st);
+    } else {
+      timed(11, [&] {
+        if (p64)
+          launch_symv<double>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p,
+                              Pd.p, 1, d_lam.p, d_sgnD.p, r.p, sympart.p, dbuf.p, st);
+        else
+          launch_symv<float>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p,
+                             Pf.p, 1, d_lam.p, d_sgnD.p, r.p, sympart.p, dbuf.p, st);
+        launch_scatter_dot(nl, lam_pos.p, d_sgnD.p, dbuf.p, z.p, 2, a2, b2, partial.p, st);
+      });
+    }
+    launch_beta_xpby(nl, partial.p, dsc.p, hsc_dev, z.p, p.p, st);
   }
 
   // Launch one iteration: replay of the captured graph, or the same kernels eagerly.

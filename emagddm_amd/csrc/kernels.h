@@ -87,14 +87,22 @@ void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void*
 template <typename T>
 void launch_gemv_phi(int ns, int max_ncp, const DevSub* subs, const T* W, const int32_t* d_lam,
                      const double* d_scale, const double* v, double* y, hipStream_t st, int max_ndp);
-// symmetric operator (which 0: Z of W, out at y_off; which 1: P, out at dvec_off)
+// symmetric operator (which 0: Z of W, out at y_off; which 1: P, out at dvec_off); the
+// per-tile partials stay in part, and with out null their sum is left to the consumer
 template <typename T>
 void launch_symv(int64_t ntiles, const SymTile* tiles, int ns, int max_Td, const DevSub* subs,
                  const T* M, int which, const int32_t* d_lam, const double* d_scale,
                  const double* v, double* part, double* out, hipStream_t st);
-// out[slot] = alpha*y[...] + beta*(Phi u)[k] + gamma*vec[o_a + k]
+// W x of every subdomain in one launch: Phi^T x into y (at y_off + ndp), Z x as the
+// symmetric-product partials in part (summed by launch_dual_correct)
 template <typename T>
-void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* y,
+void launch_apply_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp, int max_ndp,
+                    const DevSub* subs, const T* W, const int32_t* d_lam, const double* d_scale,
+                    const double* v, double* part, double* y, hipStream_t st);
+// out[slot] = alpha*(Z x)[k] + beta*(Phi u)[k] + gamma*vec[o_a + k], (Z x) summed from the
+// symmetric-product partials zpart of launch_symv(which 0)
+template <typename T>
+void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* zpart,
                          const int32_t* c_loc, const double* uc, const double* vec, double alpha,
                          double beta, double gamma, double* out, hipStream_t st);
 // out[l] = sum over the two sides of scale[slot] * w[slot]
@@ -138,7 +146,17 @@ void launch_deluxe_sum(int64_t nv, const int32_t* map_lam, const int64_t* slot0,
 void launch_deluxe_out(int64_t nv, const int32_t* map_lam, const int64_t* slot1, const double* h,
                        const double* y, double* z, hipStream_t st);
 // PCPG device scalars (slots of the scalar block)
-enum { kScRZ = 0, kScPQ = 1, kScAlpha = 2, kScBeta = 3, kScRR = 4, kScCount = 8 };
+enum { kScRZ = 0, kScPQ = 1, kScAlpha = 2, kScBeta = 3, kScRR = 4, kScRZold = 5, kScCount = 8 };
+// q = lambda scatter of w fused with the k_dots partials of npairs pairs (null b[k]: the
+// pair's second vector is q)
+void launch_scatter_dot(int64_t nl, const int64_t* lam_pos, const double* scale, const double* w,
+                        double* q, int npairs, const double* const* a, const double* const* b,
+                        double* partial, hipStream_t st);
+// k_pcpg_alpha + k_axpy2_dev and k_pcpg_beta + k_xpby_dev as single launches
+void launch_alpha_axpy2(int64_t n, const double* partial, double* sc, double* host, const double* p,
+                        const double* q, double* lam, double* r, hipStream_t st);
+void launch_beta_xpby(int64_t n, const double* partial, double* sc, double* host, const double* z,
+                      double* p, hipStream_t st);
 void launch_pcpg_alpha(const double* partial, double* sc, double* host, hipStream_t st);
 void launch_pcpg_beta(const double* partial, double* sc, double* host, int init, hipStream_t st);
 void launch_axpy2_dev(int64_t n, const double* sc, const double* p, const double* q, double* lam,

@@ -1172,16 +1172,16 @@ template <> struct Seg<float> {
 // lane reads whole 16-byte segments; columns in [nd, ndp) meet zeros in x.  Each
 // wave walks two rows at once so four segment loads are in flight per lane.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemv_phi(const DevSub* __restrict__ subs,
-                                                  const T* __restrict__ W,
-                                                  const int32_t* __restrict__ d_lam,
-                                                  const double* __restrict__ d_scale,
-                                                  const double* __restrict__ v, double* __restrict__ y) {
+__device__ __forceinline__ void gemv_phi_block(int bx, int by, const DevSub* __restrict__ subs,
+                                               const T* __restrict__ W,
+                                               const int32_t* __restrict__ d_lam,
+                                               const double* __restrict__ d_scale,
+                                               const double* __restrict__ v, double* __restrict__ y,
+                                               double* xs) {
   constexpr int V = Seg<T>::V;
-  extern __shared__ double xs[];
-  const DevSub S = subs[blockIdx.y];
+  const DevSub S = subs[by];
   const int rows = S.ncp;
-  const int r0 = blockIdx.x * kGemvRows;
+  const int r0 = bx * kGemvRows;
   if (r0 >= rows) return;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int k = tid; k < S.ndp; k += 256) {
@@ -1224,6 +1224,15 @@ __global__ __launch_bounds__(256) void k_gemv_phi(const DevSub* __restrict__ sub
     }
   }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_phi(const DevSub* __restrict__ subs,
+                                                  const T* __restrict__ W,
+                                                  const int32_t* __restrict__ d_lam,
+                                                  const double* __restrict__ d_scale,
+                                                  const double* __restrict__ v, double* __restrict__ y) {
+  extern __shared__ double xs[];
+  gemv_phi_block<T>(blockIdx.x, blockIdx.y, subs, W, d_lam, d_scale, v, y, xs);
+}
 
 template <typename T>
 void launch_gemv_phi(int ns, int max_ncp, const DevSub* subs, const T* W, const int32_t* d_lam,
@@ -1246,16 +1255,16 @@ template void launch_gemv_phi<float>(int, int, const DevSub*, const float*, cons
 // of a wave are formed by a halving butterfly over the lanes (8 + 4 + 2 + 1 + 2
 // shuffles, no LDS), the column sums by one 4-way LDS reduction across the waves.
 template <typename T>
-__global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ tiles,
-                                                   const DevSub* __restrict__ subs,
-                                                   const T* __restrict__ M, int which,
-                                                   const int32_t* __restrict__ d_lam,
-                                                   const double* __restrict__ d_scale,
-                                                   const double* __restrict__ v,
-                                                   double* __restrict__ part) {
+__device__ __forceinline__ void symv_tile_block(int b, const SymTile* __restrict__ tiles,
+                                                const DevSub* __restrict__ subs,
+                                                const T* __restrict__ M, int which,
+                                                const int32_t* __restrict__ d_lam,
+                                                const double* __restrict__ d_scale,
+                                                const double* __restrict__ v,
+                                                double* __restrict__ part) {
   __shared__ double xJ[64], xI[64];
   __shared__ double colp[4][64];
-  const SymTile tl = tiles[blockIdx.x];
+  const SymTile tl = tiles[b];
   const DevSub S = subs[tl.s];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // the tile's loads go out first; the indirect gather of x (d_lam, then v) overlaps them
@@ -1323,6 +1332,49 @@ __global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ t
           (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
   }
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_symv_tile(const SymTile* __restrict__ tiles,
+                                                   const DevSub* __restrict__ subs,
+                                                   const T* __restrict__ M, int which,
+                                                   const int32_t* __restrict__ d_lam,
+                                                   const double* __restrict__ d_scale,
+                                                   const double* __restrict__ v,
+                                                   double* __restrict__ part) {
+  symv_tile_block<T>(blockIdx.x, tiles, subs, M, which, d_lam, d_scale, v, part);
+}
+// W x in one launch: the Phi^T row blocks of every subdomain first (nphi per subdomain,
+// the longer work-groups), then the Z tiles; Z x is left as partials.
+template <typename T>
+__global__ __launch_bounds__(256) void k_symv_phi(int nphi, int ns, const SymTile* __restrict__ tiles,
+                                                  const DevSub* __restrict__ subs,
+                                                  const T* __restrict__ W,
+                                                  const int32_t* __restrict__ d_lam,
+                                                  const double* __restrict__ d_scale,
+                                                  const double* __restrict__ v,
+                                                  double* __restrict__ part,
+                                                  double* __restrict__ y) {
+  extern __shared__ double xs[];
+  const int b = blockIdx.x, nb = nphi * ns;
+  if (b < nb) gemv_phi_block<T>(b % nphi, b / nphi, subs, W, d_lam, d_scale, v, y, xs);
+  else symv_tile_block<T>(b - nb, tiles, subs, W, 0, d_lam, d_scale, v, part);
+}
+template <typename T>
+void launch_apply_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp, int max_ndp,
+                    const DevSub* subs, const T* W, const int32_t* d_lam, const double* d_scale,
+                    const double* v, double* part, double* y, hipStream_t st) {
+  if (ns <= 0) return;
+  const int nphi = (max_ncp + kGemvRows - 1) / kGemvRows;
+  const int64_t nb = int64_t(nphi) * ns + ntiles;
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_symv_phi<T>, dim3(nb), dim3(256), size_t(max_ndp) * sizeof(double), st, nphi,
+                     ns, tiles, subs, W, d_lam, d_scale, v, part, y);
+}
+template void launch_apply_W<double>(int64_t, const SymTile*, int, int, int, const DevSub*,
+                                     const double*, const int32_t*, const double*, const double*,
+                                     double*, double*, hipStream_t);
+template void launch_apply_W<float>(int64_t, const SymTile*, int, int, int, const DevSub*,
+                                    const float*, const int32_t*, const double*, const double*,
+                                    double*, double*, hipStream_t);
 
 // out[I*64 + r] = sum_{J < Td} part[(I, J)][r], at y_off (which 0) or dvec_off (which 1)
 __global__ __launch_bounds__(64) void k_symv_reduce(const DevSub* __restrict__ subs,
@@ -1345,7 +1397,8 @@ void launch_symv(int64_t ntiles, const SymTile* tiles, int ns, int max_Td, const
   if (ntiles <= 0 || ns <= 0) return;
   hipLaunchKernelGGL(k_symv_tile<T>, dim3(ntiles), dim3(256), 0, st, tiles, subs, M, which, d_lam,
                      d_scale, v, part);
-  hipLaunchKernelGGL(k_symv_reduce, dim3(max_Td, ns), dim3(64), 0, st, subs, part, which, out);
+  if (out)  // else the consumer sums the partials itself
+    hipLaunchKernelGGL(k_symv_reduce, dim3(max_Td, ns), dim3(64), 0, st, subs, part, which, out);
 }
 template void launch_symv<double>(int64_t, const SymTile*, int, int, const DevSub*, const double*,
                                   int, const int32_t*, const double*, const double*, double*,
@@ -1360,7 +1413,7 @@ template void launch_symv<float>(int64_t, const SymTile*, int, int, const DevSub
 template <typename T>
 __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__ subs,
                                                       const T* __restrict__ W,
-                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ zpart,
                                                       const int32_t* __restrict__ c_loc,
                                                       const double* __restrict__ uc,
                                                       const double* __restrict__ vec, double alpha,
@@ -1386,7 +1439,13 @@ __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__
     double r = 0.0;
     if (k < S.nd) {
       r = beta * ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
-      if (alpha != 0.0) r += alpha * y[S.y_off + k];
+      if (alpha != 0.0) {  // (Z x)_k from the symmetric-product partials, in k_symv_reduce's order
+        const int Td = S.ndp >> 6;
+        const double* zp = zpart + S.sp_off + (int64_t)blockIdx.x * Td * 64 + lane;
+        double zx = 0.0;
+        for (int J = 0; J < Td; ++J) zx += zp[(int64_t)J * 64];
+        r += alpha * zx;
+      }
       if (gamma != 0.0) r += gamma * vec[S.vec_off + S.o_a + k];
     }
     out[S.dvec_off + k] = r;
@@ -1394,12 +1453,12 @@ __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__
 }
 
 template <typename T>
-void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* y,
+void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* zpart,
                          const int32_t* c_loc, const double* uc, const double* vec, double alpha,
                          double beta, double gamma, double* out, hipStream_t st) {
   if (ns <= 0 || max_ndp <= 0) return;
   hipLaunchKernelGGL(k_dual_correct<T>, dim3((max_ndp + 63) / 64, ns), dim3(256), 0, st, subs, W,
-                     y, c_loc, uc, vec, alpha, beta, gamma, out);
+                     zpart, c_loc, uc, vec, alpha, beta, gamma, out);
 }
 template void launch_dual_correct<double>(int, int, const DevSub*, const double*, const double*,
                                           const int32_t*, const double*, const double*, double,
@@ -1729,6 +1788,112 @@ __global__ __launch_bounds__(256) void k_xpby_dev(int64_t n, const double* __res
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   p[i] = z[i] + sc[kScBeta] * p[i];
+}
+
+// q[l] = scale[p0] w[p0] + scale[p1] w[p1] over the two D slots of multiplier l (as
+// k_lambda_scatter), plus the block partial sums of a_k . y_k for np <= 2 pairs, y_k = b_k
+// or, for a null b_k, q itself.  Blocks, threads and the reduction tree are those of
+// k_dots, so the partials equal a separate k_dots' bitwise.
+__global__ __launch_bounds__(256) void k_scatter_dot(int64_t nl, const int64_t* __restrict__ pos,
+                                                     const double* __restrict__ scale,
+                                                     const double* __restrict__ w,
+                                                     double* __restrict__ q, int np, DotArgs args,
+                                                     double* __restrict__ partial) {
+  __shared__ double red[2][256];
+  double s[2] = {0.0, 0.0};
+  for (int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x; l < nl; l += (int64_t)gridDim.x * 256) {
+    const int64_t p0 = pos[2 * l], p1 = pos[2 * l + 1];
+    const double ql = scale[p0] * w[p0] + scale[p1] * w[p1];
+    q[l] = ql;
+    for (int p = 0; p < np; ++p) s[p] += args.a[p][l] * (args.b[p] ? args.b[p][l] : ql);
+  }
+  for (int p = 0; p < np; ++p) red[p][threadIdx.x] = s[p];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int p = 0; p < np; ++p) red[p][threadIdx.x] += red[p][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int p = 0; p < np; ++p) partial[p * kDotBlocks + blockIdx.x] = red[p][0];
+}
+
+void launch_scatter_dot(int64_t nl, const int64_t* lam_pos, const double* scale, const double* w,
+                        double* q, int npairs, const double* const* a, const double* const* b,
+                        double* partial, hipStream_t st) {
+  DotArgs args{};
+  for (int p = 0; p < npairs; ++p) {
+    args.a[p] = a[p];
+    args.b[p] = b[p];
+  }
+  hipLaunchKernelGGL(k_scatter_dot, dim3(kDotBlocks), dim3(256), 0, st, nl, lam_pos, scale, w, q,
+                     npairs, args, partial);
+}
+
+// Every block forms pq and alpha = rz / pq from the k_dots partials (as k_pcpg_alpha) and
+// applies lam += alpha p, r -= alpha q to its slice; block 0 records the scalars and keeps
+// rz for the next beta (slot kScRZold, which no block of this launch reads).
+__global__ __launch_bounds__(kDotBlocks) void k_alpha_axpy2(int64_t n, const double* __restrict__ partial,
+                                                            double* __restrict__ sc, double* host,
+                                                            const double* __restrict__ p,
+                                                            const double* __restrict__ q,
+                                                            double* __restrict__ lam,
+                                                            double* __restrict__ r) {
+  __shared__ double red[kDotBlocks];
+  const double pq = sum_partials(partial, red);
+  const double alpha = fabs(pq) >= 1e-300 ? sc[kScRZ] / pq : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[kScRZold] = sc[kScRZ];
+    sc[kScPQ] = pq;
+    sc[kScAlpha] = alpha;
+    if (host) {
+      host[kScPQ] = pq;
+      host[kScAlpha] = alpha;
+    }
+  }
+  const int64_t i = (int64_t)blockIdx.x * kDotBlocks + threadIdx.x;
+  if (i >= n) return;
+  lam[i] += alpha * p[i];
+  r[i] -= alpha * q[i];
+}
+
+// Every block forms rr, rz and beta = rz / rz_old (as k_pcpg_beta) and applies
+// p = z + beta p to its slice; block 0 records the scalars.
+__global__ __launch_bounds__(kDotBlocks) void k_beta_xpby(int64_t n, const double* __restrict__ partial,
+                                                          double* __restrict__ sc, double* host,
+                                                          const double* __restrict__ z,
+                                                          double* __restrict__ p) {
+  __shared__ double red[kDotBlocks];
+  const double rr = sum_partials(partial, red);
+  __syncthreads();
+  const double rz = sum_partials(partial + kDotBlocks, red);
+  const double beta = rz / sc[kScRZold];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[kScBeta] = beta;
+    sc[kScRZ] = rz;
+    sc[kScRR] = rr;
+    if (host) {
+      host[kScBeta] = beta;
+      host[kScRZ] = rz;
+      host[kScRR] = rr;
+    }
+  }
+  const int64_t i = (int64_t)blockIdx.x * kDotBlocks + threadIdx.x;
+  if (i >= n) return;
+  p[i] = z[i] + beta * p[i];
+}
+
+void launch_alpha_axpy2(int64_t n, const double* partial, double* sc, double* host, const double* p,
+                        const double* q, double* lam, double* r, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_alpha_axpy2, dim3((n + kDotBlocks - 1) / kDotBlocks), dim3(kDotBlocks), 0, st,
+                     n, partial, sc, host, p, q, lam, r);
+}
+void launch_beta_xpby(int64_t n, const double* partial, double* sc, double* host, const double* z,
+                      double* p, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_beta_xpby, dim3((n + kDotBlocks - 1) / kDotBlocks), dim3(kDotBlocks), 0, st, n,
+                     partial, sc, host, z, p);
 }
 
 void launch_pcpg_alpha(const double* partial, double* sc, double* host, hipStream_t st) {
