@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -507,6 +508,10 @@ struct fetidp_solver {
       HIPCHECK(hipHostMalloc((void**)&hsc, sizeof(double) * kScCount, hipHostMallocMapped));
       HIPCHECK(hipHostGetDevicePointer((void**)&hsc_dev, hsc, 0));
     }
+    // device and host sequence slots start equal (wait_scalars relies on it)
+    dsc.zero(st);
+    HIPCHECK(hipStreamSynchronize(st));
+    std::memset(hsc, 0, sizeof(double) * kScCount);
     cpool.alloc(plans.coarse_pool_elems);
     cinv.alloc(plans.coarse_inv_elems);
     sinv.alloc(size_t(topo.n_c) * size_t(std::max<int64_t>(sinv_ld(), 2)));
@@ -820,7 +825,7 @@ struct fetidp_solver {
   void apply_F_rest(const DBuf<T>& W, double* qout) {
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
     launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
-    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
   // Per face: [[A_F, I], [I, 0]] from the FP64 Dirichlet blocks, eliminate A_F with the
@@ -904,7 +909,7 @@ struct fetidp_solver {
     launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
     launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, fc.p, uc.p, st);
     // vec_D' holds -a_s: out = a_s - Phi u
-    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, -1.0, dbuf.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, dv.p, st);
   }
 
@@ -914,7 +919,7 @@ struct fetidp_solver {
     apply_W<T>(W, lam.p);
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
     launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
-    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
+    launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
     timed(13, [&] {
@@ -944,7 +949,7 @@ struct fetidp_solver {
     timed(10, [&] {
       launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
       launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
-      launch_dual_correct<T>(topo.n_sub, plans.max_ndp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0,
+      launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0,
                              dbuf.p, st);
       const double* a1[1] = {p.p};
       const double* b1[1] = {nullptr};
@@ -1037,8 +1042,9 @@ struct fetidp_solver {
     }
     if (last_dual < tol) return FETIDP_SUCCESS;
     for (int k = 0; k < max_it; ++k) {
+      const double seq = hs[kScSeq] + 1.0;
       pcpg_iteration<T>(W);
-      HIPCHECK(hipStreamSynchronize(st));
+      wait_scalars(seq);
       if (!(std::fabs(hs[kScPQ]) >= 1e-300)) return FETIDP_NUMERICAL_ERROR;
       ++its;
       last_dual = std::sqrt(hs[kScRR]) / dn;
@@ -1052,6 +1058,23 @@ struct fetidp_solver {
   }
   double prev_ = 0.0;
   double first_init = -1.0;  // initial dual residual of the first PCPG stage of a solve
+
+  // Wait until the iteration's scalars are on the host: its last launch writes them to the
+  // mapped host block and then bumps the sequence slot, so the host polls that slot (the
+  // next iteration can be queued before the rest of the stream drains) and falls back on
+  // the stream's completion.
+  void wait_scalars(double seq) {
+    volatile const double* hs = hsc;
+    for (uint32_t spin = 1;; ++spin) {
+      if (hs[kScSeq] == seq) break;
+      if ((spin & 255) == 0) {
+        const hipError_t e = hipStreamQuery(st);
+        if (e == hipSuccess) break;
+        if (e != hipErrorNotReady) HIPCHECK(e);
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+  }
 
   template <typename T>
   int solve_impl(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W,

@@ -102,7 +102,7 @@ void launch_apply_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp, i
 // out[slot] = alpha*(Z x)[k] + beta*(Phi u)[k] + gamma*vec[o_a + k], (Z x) summed from the
 // symmetric-product partials zpart of launch_symv(which 0)
 template <typename T>
-void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* zpart,
+void launch_dual_correct(int ns, int max_ndp, int max_ncp, const DevSub* subs, const T* W, const double* zpart,
                          const int32_t* c_loc, const double* uc, const double* vec, double alpha,
                          double beta, double gamma, double* out, hipStream_t st);
 // out[l] = sum over the two sides of scale[slot] * w[slot]
@@ -146,13 +146,15 @@ void launch_deluxe_sum(int64_t nv, const int32_t* map_lam, const int64_t* slot0,
 void launch_deluxe_out(int64_t nv, const int32_t* map_lam, const int64_t* slot1, const double* h,
                        const double* y, double* z, hipStream_t st);
 // PCPG device scalars (slots of the scalar block)
-enum { kScRZ = 0, kScPQ = 1, kScAlpha = 2, kScBeta = 3, kScRR = 4, kScRZold = 5, kScCount = 8 };
+enum { kScRZ = 0, kScPQ = 1, kScAlpha = 2, kScBeta = 3, kScRR = 4, kScRZold = 5, kScSeq = 6,
+       kScCount = 8 };
 // q = lambda scatter of w fused with the k_dots partials of npairs pairs (null b[k]: the
 // pair's second vector is q)
 void launch_scatter_dot(int64_t nl, const int64_t* lam_pos, const double* scale, const double* w,
                         double* q, int npairs, const double* const* a, const double* const* b,
                         double* partial, hipStream_t st);
-// k_pcpg_alpha + k_axpy2_dev and k_pcpg_beta + k_xpby_dev as single launches
+// k_pcpg_alpha + k_axpy2_dev and k_pcpg_beta + k_xpby_dev as single launches; the
+// latter bumps the sequence slot kScSeq after publishing the iteration's scalars to host
 void launch_alpha_axpy2(int64_t n, const double* partial, double* sc, double* host, const double* p,
                         const double* q, double* lam, double* r, hipStream_t st);
 void launch_beta_xpby(int64_t n, const double* partial, double* sc, double* host, const double* z,

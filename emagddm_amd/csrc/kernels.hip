@@ -1425,27 +1425,33 @@ __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int k = k0 + lane;
   __shared__ double part[4][64];
+  extern __shared__ double ul[];  // the subdomain's primal values u_c(s), staged once
+  for (int c = threadIdx.x; c < S.nc; c += 256) ul[c] = uc[c_loc[S.cvec_off + c]];
+  // (Z x)_k from the symmetric-product partials, in k_symv_reduce's order (wave 0; its
+  // loads overlap the staging)
+  double zx = 0.0;
+  if (q == 0 && alpha != 0.0) {
+    const int Td = S.ndp >> 6;
+    const double* zp = zpart + S.sp_off + (int64_t)blockIdx.x * Td * 64 + lane;
+#pragma unroll 4
+    for (int J = 0; J < Td; ++J) zx += zp[(int64_t)J * 64];
+  }
+  __syncthreads();
   const T* ph = W + S.w_off + sym_elems(S.ndp);  // Phi^T rows
   double s0 = 0.0, s1 = 0.0;
   int c = q;
   for (; c + 4 < S.nc; c += 8) {
-    s0 = fma((double)ph[(int64_t)c * S.ndp + k], uc[c_loc[S.cvec_off + c]], s0);
-    s1 = fma((double)ph[(int64_t)(c + 4) * S.ndp + k], uc[c_loc[S.cvec_off + c + 4]], s1);
+    s0 = fma((double)ph[(int64_t)c * S.ndp + k], ul[c], s0);
+    s1 = fma((double)ph[(int64_t)(c + 4) * S.ndp + k], ul[c + 4], s1);
   }
-  if (c < S.nc) s0 = fma((double)ph[(int64_t)c * S.ndp + k], uc[c_loc[S.cvec_off + c]], s0);
+  if (c < S.nc) s0 = fma((double)ph[(int64_t)c * S.ndp + k], ul[c], s0);
   part[q][lane] = s0 + s1;
   __syncthreads();
   if (q == 0) {
     double r = 0.0;
     if (k < S.nd) {
       r = beta * ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
-      if (alpha != 0.0) {  // (Z x)_k from the symmetric-product partials, in k_symv_reduce's order
-        const int Td = S.ndp >> 6;
-        const double* zp = zpart + S.sp_off + (int64_t)blockIdx.x * Td * 64 + lane;
-        double zx = 0.0;
-        for (int J = 0; J < Td; ++J) zx += zp[(int64_t)J * 64];
-        r += alpha * zx;
-      }
+      if (alpha != 0.0) r += alpha * zx;
       if (gamma != 0.0) r += gamma * vec[S.vec_off + S.o_a + k];
     }
     out[S.dvec_off + k] = r;
@@ -1453,17 +1459,18 @@ __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__
 }
 
 template <typename T>
-void launch_dual_correct(int ns, int max_ndp, const DevSub* subs, const T* W, const double* zpart,
+void launch_dual_correct(int ns, int max_ndp, int max_ncp, const DevSub* subs, const T* W, const double* zpart,
                          const int32_t* c_loc, const double* uc, const double* vec, double alpha,
                          double beta, double gamma, double* out, hipStream_t st) {
   if (ns <= 0 || max_ndp <= 0) return;
-  hipLaunchKernelGGL(k_dual_correct<T>, dim3((max_ndp + 63) / 64, ns), dim3(256), 0, st, subs, W,
-                     zpart, c_loc, uc, vec, alpha, beta, gamma, out);
+  hipLaunchKernelGGL(k_dual_correct<T>, dim3((max_ndp + 63) / 64, ns), dim3(256),
+                     size_t(std::max(max_ncp, 1)) * sizeof(double), st, subs, W, zpart, c_loc, uc, vec,
+                     alpha, beta, gamma, out);
 }
-template void launch_dual_correct<double>(int, int, const DevSub*, const double*, const double*,
+template void launch_dual_correct<double>(int, int, int, const DevSub*, const double*, const double*,
                                           const int32_t*, const double*, const double*, double,
                                           double, double, double*, hipStream_t);
-template void launch_dual_correct<float>(int, int, const DevSub*, const float*, const double*,
+template void launch_dual_correct<float>(int, int, int, const DevSub*, const float*, const double*,
                                          const int32_t*, const double*, const double*, double, double,
                                          double, double*, hipStream_t);
 
@@ -1849,6 +1856,7 @@ __global__ __launch_bounds__(kDotBlocks) void k_alpha_axpy2(int64_t n, const dou
     if (host) {
       host[kScPQ] = pq;
       host[kScAlpha] = alpha;
+      __threadfence_system();
     }
   }
   const int64_t i = (int64_t)blockIdx.x * kDotBlocks + threadIdx.x;
@@ -1872,10 +1880,14 @@ __global__ __launch_bounds__(kDotBlocks) void k_beta_xpby(int64_t n, const doubl
     sc[kScBeta] = beta;
     sc[kScRZ] = rz;
     sc[kScRR] = rr;
+    const double seq = sc[kScSeq] + 1.0;
+    sc[kScSeq] = seq;
     if (host) {
       host[kScBeta] = beta;
       host[kScRZ] = rz;
       host[kScRR] = rr;
+      __threadfence_system();  // the scalars reach the host before the sequence number
+      *(volatile double*)(host + kScSeq) = seq;
     }
   }
   const int64_t i = (int64_t)blockIdx.x * kDotBlocks + threadIdx.x;
