@@ -447,21 +447,30 @@ __device__ __forceinline__ void tile_trtri(T* s, const T* rdg, T* tmp, int tid) 
 // one panel product.  64x16 chunks of both panels are staged through LDS (two
 // stages, next chunk prefetched into registers), so each element crosses L2 once
 // per workgroup; the four waves then read their MFMA operands from LDS.
-template <typename T>
+template <typename T, int DEPTH = 1>
 __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
                                           const T* __restrict__ Ap, const T* __restrict__ Bq,
                                           int ktot, T* sA, T* sB, int tid, int pb0, int qb0,
                                           int lr, int lk) {
+  // DEPTH chunks are in flight in registers (2: one more chunk's loads overlap a chunk's
+  // products, for the long K loops of the Schur updates)
   constexpr int KC = 16, CH = 64 * KC;
   const int nst = ktot / KC;
   if (nst == 0) return;
-  T ra[4], rb[4];
+  T ra0[4], rb0[4], ra1[4], rb1[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    ra[u] = Ap[tid + 256 * u];
-    rb[u] = Bq[tid + 256 * u];
+    ra0[u] = Ap[tid + 256 * u];
+    rb0[u] = Bq[tid + 256 * u];
   }
-  for (int c = 0; c < nst; ++c) {
+  if (DEPTH == 2 && nst > 1) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ra1[u] = Ap[CH + tid + 256 * u];
+      rb1[u] = Bq[CH + tid + 256 * u];
+    }
+  }
+  auto step = [&](int c, T (&ra)[4], T (&rb)[4]) {
     T* a = sA + (c & 1) * CH;
     T* bb = sB + (c & 1) * CH;
 #pragma unroll
@@ -470,8 +479,8 @@ __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
       bb[tid + 256 * u] = rb[u];
     }
     __syncthreads();
-    if (c + 1 < nst) {
-      const int64_t o = (int64_t)(c + 1) * CH;
+    if (c + DEPTH < nst) {
+      const int64_t o = (int64_t)(c + DEPTH) * CH;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         ra[u] = Ap[o + tid + 256 * u];
@@ -490,6 +499,16 @@ __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
       acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
       acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
     }
+  };
+  if constexpr (DEPTH == 1) {
+    for (int c = 0; c < nst; ++c) step(c, ra0, rb0);
+  } else {
+    int c = 0;
+    for (; c + 1 < nst; c += 2) {
+      step(c, ra0, rb0);
+      step(c + 1, ra1, rb1);
+    }
+    if (c < nst) step(c, ra0, rb0);
   }
   __syncthreads();
 }
@@ -610,7 +629,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       if (wave == 0) bpre = rhs.b[o + (int64_t)w.I * 64 + lane];
     }
   }
-  panel_mma<T>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
+  panel_mma<T, MODE == 0 ? 2 : 1>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
                pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, lds_panel,
                lds_panel + kStage, tid, pb0, qb0, lr, lk);
 
