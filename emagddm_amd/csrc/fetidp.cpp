@@ -91,7 +91,7 @@ struct fetidp_solver {
   hipStream_t st = nullptr;
   hipStream_t st2 = nullptr;  // side stream of the factorization (mode-3 launches)
   hipStream_t st3 = nullptr;  // operator extraction beside the coarse factorization
-  hipEvent_t ev_main = nullptr, ev_side = nullptr, ev_extract = nullptr;
+  hipEvent_t ev_main = nullptr, ev_side = nullptr, ev_extract = nullptr, ev_zero = nullptr;
   int32_t n_mat = 0;
   double symbolic_ms = 0.0, last_factor_ms = 0.0, upload_ms = 0.0;
   bool device_ready = false;
@@ -229,7 +229,7 @@ struct fetidp_solver {
     for (hipEvent_t e : guess_ev)
       if (e) (void)hipEventDestroy(e);
     if (hsc) (void)hipHostFree(hsc);
-    for (hipEvent_t e : {ev_main, ev_side, ev_extract})
+    for (hipEvent_t e : {ev_main, ev_side, ev_extract, ev_zero})
       if (e) (void)hipEventDestroy(e);
     if (st3) (void)hipStreamDestroy(st3);
     if (st2) (void)hipStreamDestroy(st2);
@@ -396,6 +396,7 @@ struct fetidp_solver {
       HIPCHECK(hipEventCreateWithFlags(&ev_main, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_side, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_extract, hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&ev_zero, hipEventDisableTiming));
       HIPCHECK(hipEventRecord(ev_side, st2));  // a recorded, complete event to start from
     }
     xyz.upload(h_xyz_);
@@ -696,15 +697,6 @@ struct fetidp_solver {
   template <typename T>
   void factor_impl(DBuf<T>& pool, DBuf<T>& inv, DBuf<T>& W) {
     const int64_t nt = topo.n_tets;
-    // element matrices A_e = K_e + M_e (and F_e when the RHS is given by J)
-    timed(0, [&] {
-      launch_integrate(nt, xyz.p, t2n.p, mat.p, nu.p, kappa.p, rhs_from_J ? jq.p : nullptr, 1, Ae.p,
-                       nullptr, Fe.p, st);
-    });
-    // global CSR values (FP64) for the residual check
-    launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st);
-    if (rhs_from_J)
-      launch_assemble_f64(plans.rhs_global.size(), rhs_target.p, rhs_ptr.p, rhs_src.p, Fe.p, f_glob.p, st);
     // subdomain pools
     pool.alloc(size_t(plans.pool_elems));
     inv.alloc(size_t(plans.inv_elems));
@@ -716,9 +708,31 @@ struct fetidp_solver {
       Pd.release();
       Pf.alloc(size_t(topo.p_elems));
     }
-    // only tiles with assembled entries are cleared: the schedule marks each item whose
-    // tile (or diagonal tile) nothing has written yet, and the kernels read those as zero
-    launch_zero_tiles<T>(int64_t(plans.assembled_tiles.size()), asm_tiles.p, pool.p, st);
+    // Only tiles with assembled entries are cleared: the schedule marks each item whose
+    // tile (or diagonal tile) nothing has written yet, and the kernels read those as zero.
+    // Outside profiled runs the clearing (st3) overlaps the element integration, and the
+    // global CSR values for the residual check follow the W extraction on st3, beside the
+    // coarse factorization (which leaves most of the GPU idle); st meets st3 again before
+    // the factor ends.
+    const bool side = !profiling();
+    hipStream_t sz = side ? st3 : st;
+    if (side) {
+      HIPCHECK(hipEventRecord(ev_main, st));  // st3 starts after what st has queued
+      HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
+    }
+    launch_zero_tiles<T>(int64_t(plans.assembled_tiles.size()), asm_tiles.p, pool.p, sz);
+    if (side) HIPCHECK(hipEventRecord(ev_zero, st3));
+    // element matrices A_e = K_e + M_e (and F_e when the RHS is given by J)
+    timed(0, [&] {
+      launch_integrate(nt, xyz.p, t2n.p, mat.p, nu.p, kappa.p, rhs_from_J ? jq.p : nullptr, 1, Ae.p,
+                       nullptr, Fe.p, st);
+    });
+    // global CSR values (FP64) for the residual check
+    if (!side)
+      launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st);
+    if (rhs_from_J)
+      launch_assemble_f64(plans.rhs_global.size(), rhs_target.p, rhs_ptr.p, rhs_src.p, Fe.p, f_glob.p, st);
+    if (side) HIPCHECK(hipStreamWaitEvent(st, ev_zero, 0));
     timed(1, [&] {
       if constexpr (sizeof(T) == 8)
         launch_assemble_f64(plans.pool.size(), pool_target.p, pool_ptr.p, pool_src.p, Ae.p, pool.p, st);
@@ -744,6 +758,7 @@ struct fetidp_solver {
       HIPCHECK(hipEventRecord(ev_main, st));
       HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
       launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st3);
+      launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st3);
       HIPCHECK(hipEventRecord(ev_extract, st3));
     }
     // coarse problem in FP64
