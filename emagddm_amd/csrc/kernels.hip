@@ -972,17 +972,29 @@ __device__ __forceinline__ void tile_matvec(const T* L, const double* b, double*
   __syncthreads();
 }
 
-// sum over column c of tile L (64x64 col-major) times u (64): thread t -> column t>>2,
-// rows (t&3)*16 .. +16; the four partial sums of a column are combined with shuffles.
+// Column sums of tile L (64x64 col-major) against u (64): thread t handles the rows
+// q + 16m (q = t & 15, m = 0..3) of the columns c = (t >> 4) + 16p, p = 0..3, so every load
+// instruction of a wave reads four whole 128-byte column segments; the 16 partial sums
+// of a column are combined with shuffles.  out[p] = sum_r L(r, (t >> 4) + 16p) u(r), in
+// all 16 lanes of the group.
 template <typename T>
-__device__ __forceinline__ double colsum_part(const T* L, const double* u, int tid) {
-  const int c = tid >> 2, r0 = (tid & 3) * 16;
-  double s = 0.0;
+__device__ __forceinline__ void colsum4(const T* L, const double* u, int tid, double (&out)[4]) {
+  const int q = tid & 15, cg = tid >> 4;
+  double uq[4];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) s += (double)L[c * 64 + r0 + r] * u[r0 + r];
-  s += __shfl_xor(s, 1);
-  s += __shfl_xor(s, 2);
-  return s;
+  for (int m = 0; m < 4; ++m) uq[m] = u[q + 16 * m];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const T* col = L + (cg + 16 * p) * 64 + q;
+    double sum = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) sum += (double)col[16 * m] * uq[m];
+    sum += __shfl_xor(sum, 1);
+    sum += __shfl_xor(sum, 2);
+    sum += __shfl_xor(sum, 4);
+    sum += __shfl_xor(sum, 8);
+    out[p] = sum;
+  }
 }
 
 // Forward step J: y_J = L_JJ^-1 b_J (written to yv by the row-J workgroup);
@@ -1052,13 +1064,18 @@ __global__ __launch_bounds__(1024) void k_int_prep(const DevSub* __restrict__ su
   if (K >= S.Ti) return;  // uniform per work-group
   const int q = threadIdx.x >> 8, tid = threadIdx.x & 255;
   const double* u = U + S.vec_off;
-  double sum = 0.0;
+  double sum[4] = {0.0, 0.0, 0.0, 0.0}, cs[4];
   for (int I = S.Ti + q; I < S.Tc; I += 4)
-    if (ftile[S.ft_off + I] <= K)
-      sum += colsum_part(pool + rowabs[S.ft_off + I] + (int64_t)K * kTileElems, u + (int64_t)I * 64, tid);
+    if (ftile[S.ft_off + I] <= K) {
+      colsum4(pool + rowabs[S.ft_off + I] + (int64_t)K * kTileElems, u + (int64_t)I * 64, tid, cs);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) sum[p] += cs[p];
+    }
   __shared__ double part[4][64];
   __shared__ double ts[64];
-  if ((tid & 3) == 0) part[q][tid >> 2] = sum;
+  if ((tid & 15) == 0)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) part[q][(tid >> 4) + 16 * p] = sum[p];
   __syncthreads();
   if (q == 0 && tid < 64) {
     const int64_t o = S.vec_off + (int64_t)K * 64 + tid;
@@ -1069,8 +1086,11 @@ __global__ __launch_bounds__(1024) void k_int_prep(const DevSub* __restrict__ su
   if (K != S.Ti - 1) return;  // uniform per work-group
   __syncthreads();
   if (q == 0) {
-    const double x = colsum_part(inv + inv_base[blockIdx.y] + (int64_t)K * kTileElems, ts, tid);
-    if ((tid & 3) == 0) xout[S.vec_off + (int64_t)K * 64 + (tid >> 2)] = x;
+    double x[4];
+    colsum4(inv + inv_base[blockIdx.y] + (int64_t)K * kTileElems, ts, tid, x);
+    if ((tid & 15) == 0)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) xout[S.vec_off + (int64_t)K * 64 + (tid >> 4) + 16 * p] = x[p];
   }
 }
 
@@ -1093,19 +1113,25 @@ __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ sub
   if (I >= J || (!in_env && I != J - 1)) return;
   const int tid = threadIdx.x;
   double* t = tbuf + S.vec_off;
-  const double s = in_env ? colsum_part(pool + rowabs[S.ft_off + J] + (int64_t)I * kTileElems,
-                                        xout + S.vec_off + (int64_t)J * 64, tid)
-                          : 0.0;
+  double cs[4] = {0.0, 0.0, 0.0, 0.0};
+  if (in_env)
+    colsum4(pool + rowabs[S.ft_off + J] + (int64_t)I * kTileElems, xout + S.vec_off + (int64_t)J * 64, tid, cs);
   __shared__ double ts[64];
-  if ((tid & 3) == 0) {
-    const double nt = t[I * 64 + (tid >> 2)] - s;
-    t[I * 64 + (tid >> 2)] = nt;
-    ts[tid >> 2] = nt;
-  }
+  if ((tid & 15) == 0)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int c = (tid >> 4) + 16 * p;
+      const double nt = t[I * 64 + c] - cs[p];
+      t[I * 64 + c] = nt;
+      ts[c] = nt;
+    }
   if (I != J - 1) return;  // uniform per work-group
   __syncthreads();
-  const double x = colsum_part(inv + inv_base[blockIdx.y] + (int64_t)I * kTileElems, ts, tid);
-  if ((tid & 3) == 0) xout[S.vec_off + (int64_t)I * 64 + (tid >> 2)] = x;
+  double x[4];
+  colsum4(inv + inv_base[blockIdx.y] + (int64_t)I * kTileElems, ts, tid, x);
+  if ((tid & 15) == 0)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) xout[S.vec_off + (int64_t)I * 64 + (tid >> 4) + 16 * p] = x[p];
 }
 
 template <typename T>
