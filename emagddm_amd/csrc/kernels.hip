@@ -1666,40 +1666,39 @@ void launch_gather_global(int64_t nf, const int8_t* kind, const int64_t* a, cons
                      ubar, uc, x);
 }
 
-__global__ __launch_bounds__(256) void k_csr_residual(int64_t n, const int32_t* __restrict__ rp,
-                                                      const int32_t* __restrict__ col,
-                                                      const double* __restrict__ val,
-                                                      const double* __restrict__ x,
-                                                      const double* __restrict__ f,
-                                                      double* __restrict__ r) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// CSR rows, eight lanes per row: lane g of a row's group takes the nonzeros rp[i] + g,
+// + 8, ... (coalesced runs of the row), the eight partial sums are added by a fixed
+// shuffle tree.  RESID: out = f - A x (f may be null), else out = A x.
+template <bool RESID>
+__global__ __launch_bounds__(256) void k_csr_rows(int64_t n, const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col,
+                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ x,
+                                                  const double* __restrict__ f,
+                                                  double* __restrict__ out) {
+  const int64_t i = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 3;
+  const int g = threadIdx.x & 7;
   double s = 0.0;
-  for (int32_t j = rp[i]; j < rp[i + 1]; ++j) s += val[j] * x[col[j]];
-  r[i] = (f ? f[i] : 0.0) - s;
+  if (i < n)
+    for (int32_t j = rp[i] + g; j < rp[i + 1]; j += 8) s += val[j] * x[col[j]];
+  s += __shfl_xor(s, 4);
+  s += __shfl_xor(s, 2);
+  s += __shfl_xor(s, 1);
+  if (i < n && g == 0) out[i] = RESID ? (f ? f[i] : 0.0) - s : s;
 }
 
 void launch_csr_residual(int64_t n, const int32_t* rp, const int32_t* col, const double* val,
                          const double* x, const double* f, double* r, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_csr_residual, dim3((n + 255) / 256), dim3(256), 0, st, n, rp, col, val, x, f, r);
-}
-
-__global__ __launch_bounds__(256) void k_csr_spmv(int64_t n, const int32_t* __restrict__ rp,
-                                                  const int32_t* __restrict__ col,
-                                                  const double* __restrict__ val,
-                                                  const double* __restrict__ x, double* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int32_t j = rp[i]; j < rp[i + 1]; ++j) s += val[j] * x[col[j]];
-  y[i] = s;
+  hipLaunchKernelGGL(k_csr_rows<true>, dim3((n * 8 + 255) / 256), dim3(256), 0, st, n, rp, col, val, x,
+                     f, r);
 }
 
 void launch_csr_spmv(int64_t n, const int32_t* rp, const int32_t* col, const double* val,
                      const double* x, double* y, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_csr_spmv, dim3((n + 255) / 256), dim3(256), 0, st, n, rp, col, val, x, y);
+  hipLaunchKernelGGL(k_csr_rows<false>, dim3((n * 8 + 255) / 256), dim3(256), 0, st, n, rp, col, val,
+                     x, nullptr, y);
 }
 
 struct DotArgs {
