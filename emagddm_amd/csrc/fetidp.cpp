@@ -91,7 +91,8 @@ struct fetidp_solver {
   hipStream_t st = nullptr;
   hipStream_t st2 = nullptr;  // side stream of the factorization (mode-3 launches)
   hipStream_t st3 = nullptr;  // operator extraction beside the coarse factorization
-  hipEvent_t ev_main = nullptr, ev_side = nullptr, ev_extract = nullptr, ev_zero = nullptr;
+  hipEvent_t ev_main = nullptr, ev_side = nullptr, ev_extract = nullptr, ev_zero = nullptr,
+             ev_early = nullptr;
   int32_t n_mat = 0;
   double symbolic_ms = 0.0, last_factor_ms = 0.0, upload_ms = 0.0;
   bool device_ready = false;
@@ -229,7 +230,7 @@ struct fetidp_solver {
     for (hipEvent_t e : guess_ev)
       if (e) (void)hipEventDestroy(e);
     if (hsc) (void)hipHostFree(hsc);
-    for (hipEvent_t e : {ev_main, ev_side, ev_extract, ev_zero})
+    for (hipEvent_t e : {ev_main, ev_side, ev_extract, ev_zero, ev_early})
       if (e) (void)hipEventDestroy(e);
     if (st3) (void)hipStreamDestroy(st3);
     if (st2) (void)hipStreamDestroy(st2);
@@ -397,6 +398,7 @@ struct fetidp_solver {
       HIPCHECK(hipEventCreateWithFlags(&ev_side, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_extract, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_zero, hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&ev_early, hipEventDisableTiming));
       HIPCHECK(hipEventRecord(ev_side, st2));  // a recorded, complete event to start from
     }
     xyz.upload(h_xyz_);
@@ -575,8 +577,32 @@ struct fetidp_solver {
                     int* failp, bool local, RhsArgs rhs = RhsArgs()) {
     const int slot0 = local ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     bool after_side = false;      // the previous launch was a mode-3 launch on st2
+    bool early = false;           // an early Schur launch (mode 4) is pending on st3
     for (size_t li = 0; li < S.launches.size(); ++li) {
       const TileLaunch& L = S.launches[li];
+      if (L.mode == 4) {
+        // Schur updates of finished subdomains: beside the following column launches on
+        // st3 (after everything queued on st so far); st meets them again before the
+        // S_DD snapshot and at the end
+        if (profiling()) {
+          timed(slot0, [&] {
+            launch_tile_mode<T>(0, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st,
+                                cfg.tile_waves == 0 ? 0 : 4, rhs);
+          });
+          continue;
+        }
+        HIPCHECK(hipEventRecord(ev_main, st));
+        HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
+        launch_tile_mode<T>(0, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st3,
+                            cfg.tile_waves == 0 ? 0 : 4, rhs);
+        HIPCHECK(hipEventRecord(ev_early, st3));
+        early = true;
+        continue;
+      }
+      if (L.mode < 0 && early) {
+        HIPCHECK(hipStreamWaitEvent(st, ev_early, 0));
+        early = false;
+      }
       if (L.mode == 3 && cfg.side_stream == 2 && !profiling()) {
         // the factoring items (flags & 16) go at the front of the column-step launch
         // that follows them: dispatched first, no second stream
@@ -625,6 +651,7 @@ struct fetidp_solver {
       });
     }
     HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
+    if (early) HIPCHECK(hipStreamWaitEvent(st, ev_early, 0));
   }
 
   // FP64 Dirichlet blocks when cond(S_DD) ~ max nu / (min kappa h^2) could exceed
@@ -1612,7 +1639,8 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
     for (int mode = 0; mode < 3; ++mode) {
       double fl = 0.0;
       for (const TileLaunch& L : sch[q]->launches)
-        if (L.mode == mode || (mode == 2 && L.mode == 3))  // mode 3 is timed with the diagonals
+        // mode 3 is timed with the diagonals, mode 4 (early Schur) with the Schur updates
+        if (L.mode == mode || (mode == 2 && L.mode == 3) || (mode == 0 && L.mode == 4))
           for (int64_t i = 0; i < L.n; ++i) {
             const TileWork& w = sch[q]->items[L.off + i];
             const bool trsm = (L.mode == 1 || L.mode == 3) && !(w.flags & 2);  // not a lookahead

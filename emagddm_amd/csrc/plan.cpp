@@ -490,6 +490,22 @@ void build_plans(const Topology& T, Plans& P) {
     // Lookahead (incremental variant): the mode-1 launch of column j also applies
     // K in [.., j) to the next column's mode-3 tile (j + 2, j + 1) (flags 2, update
     // only), so the side-stream item keeps one product and its diagonal factorization.
+    // Schur updates of a subdomain need only its own interior columns; once the column
+    // launches no longer fill the GPU (fewer items than kSideFill), the Schur updates of
+    // the subdomains already done ride beside them (mode 4: a mode-0 launch that may run
+    // on a second stream), the rest follow the last column.
+    constexpr int kSideFill = 1024;
+    auto schur1_of = [&](int64_t s, std::vector<TileWork>& out) {
+      const SubLayout& L = T.sub[s];
+      if (L.Ti == 0) return;
+      const int32_t* ft = &P.ftile[P.ftile_off[s]];
+      for (int J = L.Ti; J < L.Tc; ++J)
+        for (int I = inc ? J + 1 : J; I < L.Tc; ++I) {
+          const int k0 = std::max(ft[I], ft[J]);
+          if (k0 < L.Ti) out.push_back(TileWork{int32_t(s), I, J, k0, L.Ti, 0});
+        }
+    };
+    std::vector<uint8_t> schur1_done(size_t(ns), 0);
     for (int j = 0; j < maxTi; ++j) {
       std::vector<TileWork> diag, next, below;
       for (int64_t s = 0; s < ns; ++s) {
@@ -510,24 +526,39 @@ void build_plans(const Topology& T, Plans& P) {
           if (k0 < j) below.push_back(TileWork{int32_t(s), j + 2, j + 1, k0, j, 2});
         }
       }
+      if (inc && int64_t(diag.size() + next.size() + below.size()) < kSideFill &&
+          std::find(schur1_done.begin(), schur1_done.end(), 1) == schur1_done.end()) {
+        std::vector<TileWork> early;
+        for (int64_t s = 0; s < ns; ++s)
+          if (T.sub[s].Ti <= j) {
+            schur1_of(s, early);
+            schur1_done[size_t(s)] = 1;
+          }
+        add_tiles(S, 4, early);
+      }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
       add_tiles(S, 1, below);
     }
     std::vector<TileWork> schur1;
-    for (int64_t s = 0; s < ns; ++s) {
-      const SubLayout& L = T.sub[s];
-      if (L.Ti == 0) continue;
-      const int32_t* ft = &P.ftile[P.ftile_off[s]];
-      for (int J = L.Ti; J < L.Tc; ++J)
-        for (int I = inc ? J + 1 : J; I < L.Tc; ++I) {
-          const int k0 = std::max(ft[I], ft[J]);
-          if (k0 < L.Ti) schur1.push_back(TileWork{int32_t(s), I, J, k0, L.Ti, 0});
-        }
-    }
+    for (int64_t s = 0; s < ns; ++s)
+      if (!schur1_done[size_t(s)]) schur1_of(s, schur1);
     add_tiles(S, 0, schur1);
     S.launches.push_back(TileLaunch{-1, 0, 0});  // snapshot S_DD for the preconditioner
-    // phase 2: dual columns
+    // phase 2: dual columns (the Schur updates of finished subdomains again ride beside
+    // the column launches once those no longer fill the GPU)
+    auto schur2_of = [&](int64_t s, std::vector<TileWork>& out) {
+      const SubLayout& L = T.sub[s];
+      if (L.Tr == L.Ti) return;
+      const int32_t* ft = &P.ftile[P.ftile_off[s]];
+      for (int J = L.Tr; J < L.T; ++J)
+        for (int I = inc ? J + 1 : J; I < L.T; ++I) {
+          const int k0 = std::max(L.Ti, std::max(ft[I], ft[J]));
+          if (k0 < L.Tr) out.push_back(TileWork{int32_t(s), I, J, k0, L.Tr, 0});
+        }
+    };
+    std::vector<uint8_t> schur2_done(size_t(ns), 0);
+    bool early2 = false;
     for (int j = 0; j < maxTd; ++j) {
       std::vector<TileWork> diag, next, below;
       for (int64_t s = 0; s < ns; ++s) {
@@ -551,21 +582,23 @@ void build_plans(const Topology& T, Plans& P) {
           if (k0 < col) below.push_back(TileWork{int32_t(s), col + 2, col + 1, k0, col, 2});
         }
       }
+      if (inc && !early2 && int64_t(diag.size() + next.size() + below.size()) < kSideFill) {
+        early2 = true;
+        std::vector<TileWork> early;
+        for (int64_t s = 0; s < ns; ++s)
+          if (T.sub[s].Tr - T.sub[s].Ti <= j) {
+            schur2_of(s, early);
+            schur2_done[size_t(s)] = 1;
+          }
+        add_tiles(S, 4, early);
+      }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
       add_tiles(S, 1, below);
     }
     std::vector<TileWork> schur2;
-    for (int64_t s = 0; s < ns; ++s) {
-      const SubLayout& L = T.sub[s];
-      if (L.Tr == L.Ti) continue;
-      const int32_t* ft = &P.ftile[P.ftile_off[s]];
-      for (int J = L.Tr; J < L.T; ++J)
-        for (int I = inc ? J + 1 : J; I < L.T; ++I) {
-          const int k0 = std::max(L.Ti, std::max(ft[I], ft[J]));
-          if (k0 < L.Tr) schur2.push_back(TileWork{int32_t(s), I, J, k0, L.Tr, 0});
-        }
-    }
+    for (int64_t s = 0; s < ns; ++s)
+      if (!schur2_done[size_t(s)]) schur2_of(s, schur2);
     add_tiles(S, 0, schur2);
   }
 

@@ -31,7 +31,9 @@ struct CsrPlan {
 };
 
 // A run of the batched tile factorization: list of launches over `items`.
-//   mode 0/1/2 as launch_tile_mode; mode -1: snapshot the Dirichlet Schur blocks.
+//   mode 0/1/2/3 as launch_tile_mode; mode 4: Schur updates (mode 0) of subdomains whose
+//   columns are done, which may run beside the launches that follow them up to the next
+//   snapshot or the end; mode -1: snapshot the Dirichlet Schur blocks.
 struct TileLaunch {
   int mode;
   int64_t off, n;
