@@ -189,13 +189,15 @@ class FETIDPSolver:
         self._check(self._lib.fetidp_factor(self._h), "fetidp_factor")
 
     # --------------------------------------------------------------- solve
-    def solve(self, lambda0=None) -> SolverResult:
+    def solve(self, lambda0=None, copy_x=True) -> SolverResult:
+        """PCPG solve.  copy_x=False leaves the solution in device memory (x is None):
+        no PCIe transfer of the n_free-long solution."""
         n = self.sizes["n_free"]
-        x = np.zeros(n)
+        x = np.zeros(n) if copy_x else None
         st = _lib.Stats()
         lam0 = None if lambda0 is None else _f64(lambda0)
-        rc = self._lib.fetidp_solve(self._h, _dp(lam0) if lam0 is not None else None, _dp(x),
-                                    C.byref(st))
+        rc = self._lib.fetidp_solve(self._h, _dp(lam0) if lam0 is not None else None,
+                                    _dp(x) if x is not None else None, C.byref(st))
         return SolverResult(x=x, status=SolverStatus(rc), iterations=st.iterations,
                             residual_norm=st.residual_norm, solve_time_ms=st.solve_ms,
                             error_msg="" if rc == 0 else self.last_error(),

@@ -242,3 +242,17 @@ def test_degenerate_partitions(gpu_available, layout):
     assert np.linalg.norm(r.x - xd) <= 1e-8 * np.linalg.norm(xd)
     if layout == "one":
         assert r.iterations == 0
+
+
+def test_solve_without_host_copy(gpu_available):
+    """x_free = NULL leaves the solution in device memory; the solve itself is unchanged."""
+    p = W.make_workload("cube16")
+    print("[no host copy] cube16", flush=True)
+    s = FETIDPSolver().setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    r0 = s.solve(copy_x=False)
+    r1 = s.solve()
+    assert r0.x is None and r1.x is not None
+    assert r0.status == r1.status == SolverStatus.SUCCESS
+    assert r0.iterations == r1.iterations and r0.residual_norm == r1.residual_norm
