@@ -340,10 +340,12 @@ __device__ __forceinline__ int tile_potrf(T* s, T* rdg, int tid) {
         const T piv = readlane(r[k], c0 + k);
         const bool ok = piv > T(0) && piv < T(3.0e38);
         bad |= !ok;
-        const T rd = ok ? rsqrt(piv) : T(1);
-        const T d = ok ? piv * rd : T(1);
+        // rsqrt of the pivot or of a stand-in, then selects: no branch in the pivot chain
+        const T rq = rsqrt(ok ? piv : T(1));
+        const T rd = ok ? rq : T(1);
+        const T d = ok ? piv * rq : T(1);
         r[k] = lane == c0 + k ? d : (lane > c0 + k ? r[k] * rd : T(0));
-        if (lane == c0 + k) rdg[c0 + k] = rd;
+        rdg[c0 + k] = rd;  // the same value from every lane
 #pragma unroll
         for (int j = k + 1; j < 16; ++j) r[j] -= r[k] * readlane(r[k], c0 + j);
       }
