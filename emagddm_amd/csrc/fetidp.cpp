@@ -117,6 +117,7 @@ struct fetidp_solver {
   DBuf<DevSub> devsub;
   DBuf<SymTile> sym_tiles;
   DBuf<double> sympart;
+  DBuf<SymTile> snap_tiles;  // S_DD snapshot tiles in snapshot-launch order
   // deluxe scaling (interface faces)
   bool deluxe = false;
   DBuf<FaceDev> face_dev;
@@ -453,6 +454,7 @@ struct fetidp_solver {
     }
     devsub.upload(plans.devsub);
     sym_tiles.upload(plans.sym_tiles);
+    snap_tiles.upload(plans.snap_tiles);
     sympart.alloc(size_t(std::max<int64_t>(topo.sp_elems, 1)));
     {
       const FacePlan& F = plans.faces;
@@ -571,6 +573,15 @@ struct fetidp_solver {
   //   mode 3 of J waits for everything issued on st before it (column J-1,
   //   diagonal J); the mode-1 launch of J waits for mode 3 of J-1 only;
   //   every other launch, and the end of the schedule, wait for all of st2.
+  // S_DD snapshot of the tiles snap_tiles[L.off, L.off + L.n) (a snapshot launch)
+  template <typename T>
+  void snapshot(const TileLaunch& L, T* pool, hipStream_t s) {
+    if (L.n <= 0) return;
+    const uint8_t* snap = sizeof(T) == 8 ? ts64.p : ts32.p;
+    if (p64) launch_extract_P<T, double>(L.n, snap_tiles.p + L.off, devsub.p, pool, rowabs.p, ftile.p, snap, Pd.p, s);
+    else launch_extract_P<T, float>(L.n, snap_tiles.p + L.off, devsub.p, pool, rowabs.p, ftile.p, snap, Pf.p, s);
+  }
+
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& rowbase,
                     const DBuf<int64_t>& rabs, const DBuf<int64_t>& inv_base, T* pool, T* inv,
@@ -597,6 +608,15 @@ struct fetidp_solver {
                             cfg.tile_waves == 0 ? 0 : 4, rhs);
         HIPCHECK(hipEventRecord(ev_early, st3));
         early = true;
+        continue;
+      }
+      if (L.mode == -2) {  // S_DD snapshot of the early Schur's subdomains, beside the rest
+        if (profiling()) {
+          timed(5, [&] { snapshot<T>(L, pool, st); });
+        } else {
+          snapshot<T>(L, pool, st3);
+          HIPCHECK(hipEventRecord(ev_early, st3));
+        }
         continue;
       }
       if (L.mode < 0 && early) {
@@ -637,12 +657,7 @@ struct fetidp_solver {
       if (!after_side) HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
       after_side = false;
       if (L.mode < 0) {
-        timed(5, [&] {
-          const int64_t nt = int64_t(plans.sym_tiles.size());
-          const uint8_t* snap = sizeof(T) == 8 ? ts64.p : ts32.p;
-          if (p64) launch_extract_P<T, double>(nt, sym_tiles.p, devsub.p, pool, rowabs.p, ftile.p, snap, Pd.p, st);
-          else launch_extract_P<T, float>(nt, sym_tiles.p, devsub.p, pool, rowabs.p, ftile.p, snap, Pf.p, st);
-        });
+        timed(5, [&] { snapshot<T>(L, pool, st); });
         continue;
       }
       timed(slot0 + L.mode, [&] {

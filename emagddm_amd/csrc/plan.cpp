@@ -506,6 +506,10 @@ void build_plans(const Topology& T, Plans& P) {
         }
     };
     std::vector<uint8_t> schur1_done(size_t(ns), 0);
+    auto add_snap = [&](int64_t s) {
+      for (int I = 0; I < T.sub[s].ndp / kTile; ++I)
+        for (int J = 0; J <= I; ++J) P.snap_tiles.push_back(SymTile{int32_t(s), I, J});
+    };
     for (int j = 0; j < maxTi; ++j) {
       std::vector<TileWork> diag, next, below;
       for (int64_t s = 0; s < ns; ++s) {
@@ -535,6 +539,12 @@ void build_plans(const Topology& T, Plans& P) {
             schur1_done[size_t(s)] = 1;
           }
         add_tiles(S, 4, early);
+        // their S_DD snapshot can follow at once (the later launches do not touch them)
+        const int64_t off = int64_t(P.snap_tiles.size());
+        for (int64_t s = 0; s < ns; ++s)
+          if (schur1_done[size_t(s)]) add_snap(s);
+        if (int64_t(P.snap_tiles.size()) > off)
+          S.launches.push_back(TileLaunch{-2, off, int64_t(P.snap_tiles.size()) - off});
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
@@ -544,7 +554,18 @@ void build_plans(const Topology& T, Plans& P) {
     for (int64_t s = 0; s < ns; ++s)
       if (!schur1_done[size_t(s)]) schur1_of(s, schur1);
     add_tiles(S, 0, schur1);
-    S.launches.push_back(TileLaunch{-1, 0, 0});  // snapshot S_DD for the preconditioner
+    // snapshot S_DD for the preconditioner (the rest of the subdomains; all of them in the
+    // left-looking variant, over the same list)
+    {
+      if (inc) {
+        const int64_t off = int64_t(P.snap_tiles.size());
+        for (int64_t s = 0; s < ns; ++s)
+          if (!schur1_done[size_t(s)]) add_snap(s);
+        S.launches.push_back(TileLaunch{-1, off, int64_t(P.snap_tiles.size()) - off});
+      } else {  // built after the incremental variant, whose lists cover every subdomain
+        S.launches.push_back(TileLaunch{-1, 0, int64_t(P.snap_tiles.size())});
+      }
+    }
     // phase 2: dual columns (the Schur updates of finished subdomains again ride beside
     // the column launches once those no longer fill the GPU)
     auto schur2_of = [&](int64_t s, std::vector<TileWork>& out) {
@@ -609,8 +630,8 @@ void build_plans(const Topology& T, Plans& P) {
     std::vector<uint8_t> w = P.tile_assembled;
     std::vector<uint8_t>& snap = variant == 0 ? P.tile_written_snap : P.tile_written_snap_ll;
     for (const TileLaunch& L : S.launches) {
-      if (L.mode < 0) {
-        snap = w;
+      if (L.mode < 0) {  // mode -2 tiles do not change before the mode -1 launch
+        if (L.mode == -1) snap = w;
         continue;
       }
       for (int64_t i = L.off; i < L.off + L.n; ++i) {

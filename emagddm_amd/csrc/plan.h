@@ -33,7 +33,9 @@ struct CsrPlan {
 // A run of the batched tile factorization: list of launches over `items`.
 //   mode 0/1/2/3 as launch_tile_mode; mode 4: Schur updates (mode 0) of subdomains whose
 //   columns are done, which may run beside the launches that follow them up to the next
-//   snapshot or the end; mode -1: snapshot the Dirichlet Schur blocks.
+//   snapshot or the end; mode -1: snapshot the Dirichlet Schur blocks (snap_tiles[off,
+//   off + n)); mode -2: the same for the subdomains of the preceding mode-4 launch, which
+//   may run beside it.
 struct TileLaunch {
   int mode;
   int64_t off, n;
@@ -100,6 +102,9 @@ struct Plans {
   std::vector<int64_t> coarse_rowabs;
   int64_t inv_elems = 0, coarse_inv_elems = 0;
   std::vector<SymTile> sym_tiles;  // every lower tile of every subdomain's Z / P
+  // the same tiles for the S_DD snapshots: the subdomains whose Schur updates ran early
+  // first (snapshot launch mode -2), then the rest (mode -1)
+  std::vector<SymTile> snap_tiles;
   int max_Td = 0;                  // max ndp / 64
   std::vector<int32_t> d_self;     // D slot -> itself (-1 on padding)
   FacePlan faces;
