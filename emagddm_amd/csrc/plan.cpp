@@ -57,6 +57,14 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
   }
 }
 
+// Longest K loops first (stable): a column launch ends with its last work-groups, so the
+// long items go out early and the short ones fill the slots behind them.
+void longest_first(std::vector<TileWork>& list) {
+  std::stable_sort(list.begin(), list.end(), [](const TileWork& a, const TileWork& b) {
+    return a.k1 - a.k0 > b.k1 - b.k0;
+  });
+}
+
 }  // namespace
 
 // Incremental diagonal (inc = true): the column-step workgroups (mode 1, flags & 1)
@@ -548,6 +556,7 @@ void build_plans(const Topology& T, Plans& P) {
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
+      longest_first(below);
       add_tiles(S, 1, below);
     }
     std::vector<TileWork> schur1;
@@ -615,6 +624,7 @@ void build_plans(const Topology& T, Plans& P) {
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
+      longest_first(below);
       add_tiles(S, 1, below);
     }
     std::vector<TileWork> schur2;
