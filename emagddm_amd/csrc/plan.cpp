@@ -119,6 +119,7 @@ TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc, const int32_t* ft)
           if (f(I) < j && f(J) < j && chunk_in(I, J, j, w)) below.push_back(w);
         }
     add_tiles(S, 3, next);
+    longest_first(below);
     add_tiles(S, 1, below);
   }
   std::vector<TileWork> schur;
@@ -127,6 +128,7 @@ TileSchedule single_matrix_schedule(int Tp, int Ta, bool inc, const int32_t* ft)
       const int k0 = inc ? tail(I, J) : k0_of(I, J);
       if (k0 < Tp) schur.push_back(TileWork{0, I, J, k0, Tp, 0});
     }
+  longest_first(schur);
   add_tiles(S, 0, schur);
   return S;
 }
@@ -148,6 +150,7 @@ TileSchedule batched_dense_schedule(const std::vector<int>& Tp) {
     }
     add_tiles(S, 2, diag);
     add_tiles(S, 3, next);
+    longest_first(below);
     add_tiles(S, 1, below);
   }
   std::vector<TileWork> schur;
@@ -546,6 +549,7 @@ void build_plans(const Topology& T, Plans& P) {
             schur1_of(s, early);
             schur1_done[size_t(s)] = 1;
           }
+        longest_first(early);
         add_tiles(S, 4, early);
         // their S_DD snapshot can follow at once (the later launches do not touch them)
         const int64_t off = int64_t(P.snap_tiles.size());
@@ -562,6 +566,7 @@ void build_plans(const Topology& T, Plans& P) {
     std::vector<TileWork> schur1;
     for (int64_t s = 0; s < ns; ++s)
       if (!schur1_done[size_t(s)]) schur1_of(s, schur1);
+    longest_first(schur1);
     add_tiles(S, 0, schur1);
     // snapshot S_DD for the preconditioner (the rest of the subdomains; all of them in the
     // left-looking variant, over the same list)
@@ -620,6 +625,7 @@ void build_plans(const Topology& T, Plans& P) {
             schur2_of(s, early);
             schur2_done[size_t(s)] = 1;
           }
+        longest_first(early);
         add_tiles(S, 4, early);
       }
       add_tiles(S, 2, diag);
@@ -630,6 +636,7 @@ void build_plans(const Topology& T, Plans& P) {
     std::vector<TileWork> schur2;
     for (int64_t s = 0; s < ns; ++s)
       if (!schur2_done[size_t(s)]) schur2_of(s, schur2);
+    longest_first(schur2);
     add_tiles(S, 0, schur2);
   }
 
