@@ -51,19 +51,22 @@ def parse():
     ap.add_argument("--max-iterations", type=int, default=1000)
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the (untimed) launch-variant autotuner")
+    ap.add_argument("--graphs", type=int, choices=(0, 1), default=1,
+                    help="PCPG iterations as captured HIP graphs (1, the solver default) or eager "
+                         "launches (0); the autotuner, when on, chooses for itself")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="copy this many MiB device-to-device once before the variants, so a "
                          "counter profile sees a copy of known size")
     return ap.parse_args()
 
 
-def run_variant(prob, variant, steps, warmup, max_iterations=1000, autotune=True):
+def run_variant(prob, variant, steps, warmup, max_iterations=1000, autotune=True, graphs=True):
     import torch
     from emagddm_amd import FETIDPSolver, SolverStatus
     from emagddm_amd.lambda0 import LambdaMPNN, default_weights
     precision, _, extra = variant.partition("+")
-    s = FETIDPSolver(precision=precision, profile=False,  # timed run: graphs on, no events
-                     max_iterations=max_iterations)
+    s = FETIDPSolver(precision=precision, profile=False,  # timed run: no events
+                     max_iterations=max_iterations, graphs=graphs)
     s.setup_problem(prob)
     model = None
     if extra == "learned":
@@ -195,7 +198,8 @@ def main():
     variants = {}
     for prec in a.variants.split(","):
         try:
-            v = run_variant(prob, prec, a.steps, a.warmup, a.max_iterations, not a.no_autotune)
+            v = run_variant(prob, prec, a.steps, a.warmup, a.max_iterations, not a.no_autotune,
+                            bool(a.graphs))
         except RuntimeError as e:
             print(f"[bench] {prec}: failed: {e}", file=sys.stderr, flush=True)
             continue

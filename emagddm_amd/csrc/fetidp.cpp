@@ -999,10 +999,12 @@ struct fetidp_solver {
   // on the device, so the chain can be captured.  The lambda scatters carry the dot
   // products; alpha and beta are formed by the launches that use them.  Bitwise the
   // same arithmetic as apply_F / apply_M, k_dots, k_pcpg_alpha / beta and the updates.
+  // head_queued: W p of this iteration is already on the stream (queued by the
+  // previous iteration, see pcpg).
   template <typename T>
-  void pcpg_iteration_kernels(const DBuf<T>& W) {
+  void pcpg_iteration_kernels(const DBuf<T>& W, bool head_queued = false) {
     const int64_t nl = topo.n_lam;
-    timed(9, [&] { apply_W<T>(W, p.p); });
+    if (!head_queued) timed(9, [&] { apply_W<T>(W, p.p); });
     timed(10, [&] {
       launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
       launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
@@ -1035,11 +1037,12 @@ struct fetidp_solver {
 
   // Launch one iteration: replay of the captured graph, or the same kernels eagerly.
   template <typename T>
-  void pcpg_iteration(const DBuf<T>& W) {
+  void pcpg_iteration(const DBuf<T>& W, bool head_queued) {
     if (!cfg.graphs || profiling()) {
-      pcpg_iteration_kernels<T>(W);
+      pcpg_iteration_kernels<T>(W, head_queued);
       return;
     }
+    if (head_queued) throw std::logic_error("graph iterations form their own W p");
     const void* key[3] = {W.p, p64 ? (const void*)Pd.p : (const void*)Pf.p, sinv.p};
     const int key_t = int(sizeof(T)) | (deluxe ? 256 : 0);
     if (!pcpg_exec || pcpg_key_t != key_t || std::memcmp(key, pcpg_key, sizeof(key))) {
@@ -1098,9 +1101,18 @@ struct fetidp_solver {
       prev_ = last_dual;
     }
     if (last_dual < tol) return FETIDP_SUCCESS;
+    // Eager launches queue the next iteration's W p (its first and longest launch) before
+    // waiting for this iteration's scalars, so the GPU is not idle while the host polls
+    // and launches.  Not within a factor 100 of the tolerance, where it would likely go
+    // unused; when it does go unused it is only overwritten (recover() forms W lambda into
+    // the same buffers before anything reads them).
+    const bool queue_next = !cfg.graphs && !profiling();
+    bool head = false;
     for (int k = 0; k < max_it; ++k) {
       const double seq = hs[kScSeq] + 1.0;
-      pcpg_iteration<T>(W);
+      pcpg_iteration<T>(W, head);
+      head = queue_next && k + 1 < max_it && last_dual > 100.0 * tol;
+      if (head) apply_W<T>(W, p.p);
       wait_scalars(seq);
       if (!(std::fabs(hs[kScPQ]) >= 1e-300)) return FETIDP_NUMERICAL_ERROR;
       ++its;
@@ -1119,12 +1131,17 @@ struct fetidp_solver {
   // Wait until the iteration's scalars are on the host: its last launch writes them to the
   // mapped host block and then bumps the sequence slot, so the host polls that slot (the
   // next iteration can be queued before the rest of the stream drains) and falls back on
-  // the stream's completion.
+  // the stream's completion.  The stream is queried only after 2 ms without the slot:
+  // a query of a busy stream costs the launches queued behind it ~6 us each.
   void wait_scalars(double seq) {
     volatile const double* hs = hsc;
+    double t0 = -1.0;
     for (uint32_t spin = 1;; ++spin) {
       if (hs[kScSeq] == seq) break;
       if ((spin & 255) == 0) {
+        const double t = now_ms();
+        if (t0 < 0.0) t0 = t;
+        if (t - t0 < 2.0) continue;
         const hipError_t e = hipStreamQuery(st);
         if (e == hipSuccess) break;
         if (e != hipErrorNotReady) HIPCHECK(e);
