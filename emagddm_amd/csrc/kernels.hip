@@ -196,15 +196,27 @@ template void launch_gather_pool<float>(int64_t, const int64_t*, const int64_t*,
                                         const float*, const uint8_t*, double*, hipStream_t);
 
 // Clear the listed 64x64 tiles of a pool (tile t = elements 4096 t .. 4096 t + 4095).
+// Each work-group clears kZeroGroup consecutive list entries with 16-byte stores (tile
+// bases are 16-byte aligned: the pool comes from hipMalloc and tiles are 16 KB or
+// 32 KB), so the launch is not bound by work-group dispatch.
+constexpr int kZeroGroup = 4;
 template <typename T>
-__global__ __launch_bounds__(256) void k_zero_tiles(const int32_t* __restrict__ list, T* __restrict__ pool) {
-  T* t = pool + (int64_t)list[blockIdx.x] * kTileElems;
-  for (int e = threadIdx.x; e < kTileElems; e += 256) t[e] = T(0);
+__global__ __launch_bounds__(256) void k_zero_tiles(int64_t n, const int32_t* __restrict__ list,
+                                                    T* __restrict__ pool) {
+  constexpr int n4 = kTileElems * int(sizeof(T)) / 16;
+  const int64_t t0 = int64_t(blockIdx.x) * kZeroGroup;
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int k = 0; k < kZeroGroup && t0 + k < n; ++k) {
+    float4* t = reinterpret_cast<float4*>(pool + (int64_t)list[t0 + k] * kTileElems);
+#pragma unroll
+    for (int e = threadIdx.x; e < n4; e += 256) t[e] = z;
+  }
 }
 template <typename T>
 void launch_zero_tiles(int64_t n, const int32_t* list, T* pool, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_zero_tiles<T>, dim3(n), dim3(256), 0, st, list, pool);
+  hipLaunchKernelGGL(k_zero_tiles<T>, dim3((n + kZeroGroup - 1) / kZeroGroup), dim3(256), 0, st, n, list,
+                     pool);
 }
 template void launch_zero_tiles<double>(int64_t, const int32_t*, double*, hipStream_t);
 template void launch_zero_tiles<float>(int64_t, const int32_t*, float*, hipStream_t);
