@@ -1131,17 +1131,18 @@ struct fetidp_solver {
   // Wait until the iteration's scalars are on the host: its last launch writes them to the
   // mapped host block and then bumps the sequence slot, so the host polls that slot (the
   // next iteration can be queued before the rest of the stream drains) and falls back on
-  // the stream's completion.  The stream is queried only after 2 ms without the slot:
-  // a query of a busy stream costs the launches queued behind it ~6 us each.
+  // the stream's completion.  The stream is queried at most once per 2 ms (none in the
+  // first 2 ms): a query of a busy stream delayed the launch queued behind it by ~6 us.
   void wait_scalars(double seq) {
     volatile const double* hs = hsc;
-    double t0 = -1.0;
+    double t_next = -1.0;
     for (uint32_t spin = 1;; ++spin) {
       if (hs[kScSeq] == seq) break;
       if ((spin & 255) == 0) {
         const double t = now_ms();
-        if (t0 < 0.0) t0 = t;
-        if (t - t0 < 2.0) continue;
+        if (t_next < 0.0) t_next = t + 2.0;
+        if (t < t_next) continue;
+        t_next = t + 2.0;
         const hipError_t e = hipStreamQuery(st);
         if (e == hipSuccess) break;
         if (e != hipErrorNotReady) HIPCHECK(e);
