@@ -159,8 +159,15 @@ struct fetidp_solver {
   // or Voronoi partitions, whose staircase faces stiffness scaling handles badly.
   double box_overlap = 0.0;
   DBuf<int> fail;
-  DBuf<double> cpool, cinv, sinv;
-  int64_t sinv_ld() const { return (topo.n_c + 1) / 2 * 2; }  // even row stride of S~^-1
+  DBuf<double> cpool, cinv;
+  DBuf<double> cpart;        // partial sums of the coarse S~^-1 products (tn * tn * 64)
+  DBuf<int32_t> ctiles;      // (I, J <= I) tile pairs of the -S~^-1 block of the coarse pool
+  int coarse_tn = 0;         // its tile order, coarse_ncp / 64
+  // uc = S~^-1 x from the factored coarse pool
+  void coarse_apply(const double* x, double* y) {
+    launch_coarse_symv(topo.n_c, coarse_tn, int64_t(ctiles.n / 2), ctiles.p, cpool.p, coarse_rowabs.p, x,
+                       cpart.p, y, st);
+  }
   DBuf<int> cfail;
   DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc, yvec, tbuf;
   DBuf<double> lam, r, z, p, q, dv, ubar, x, xtot, rglob, partial, pf, qf;
@@ -519,7 +526,18 @@ struct fetidp_solver {
     std::memset(hsc, 0, sizeof(double) * kScCount);
     cpool.alloc(plans.coarse_pool_elems);
     cinv.alloc(plans.coarse_inv_elems);
-    sinv.alloc(size_t(topo.n_c) * size_t(std::max<int64_t>(sinv_ld(), 2)));
+    coarse_tn = int(plans.coarse_ncp / kTile);
+    {
+      std::vector<int32_t> ct;
+      ct.reserve(size_t(coarse_tn) * size_t(coarse_tn + 1));
+      for (int I = 0; I < coarse_tn; ++I)
+        for (int J = 0; J <= I; ++J) {
+          ct.push_back(I);
+          ct.push_back(J);
+        }
+      ctiles.upload(ct);
+    }
+    cpart.alloc(size_t(std::max(coarse_tn, 1)) * size_t(std::max(coarse_tn, 1)) * kTile);
     HIPCHECK(hipStreamSynchronize(st));
     device_ready = true;
     upload_ms = now_ms() - t0;
@@ -809,7 +827,6 @@ struct fetidp_solver {
                           pool.p, (sizeof(T) == 8 ? tw64.p : tw32.p), cpool.p, st);
     cfail.zero(st);
     run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, false);
-    launch_extract_sinv(topo.n_c, plans.coarse_ncp, sinv_ld(), cpool.p, coarse_rowabs.p, coarse_ftile.p, sinv.p, st);
     if (deluxe) deluxe_setup();
     if (!profiling()) HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
     HIPCHECK(hipStreamSynchronize(st));
@@ -881,7 +898,7 @@ struct fetidp_solver {
   template <typename T>
   void apply_F_rest(const DBuf<T>& W, double* qout) {
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
-    launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
+    coarse_apply(gc.p, uc.p);
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
@@ -964,7 +981,7 @@ struct fetidp_solver {
       fwd_valid = is_f;
     }
     launch_coarse_gather(topo.n_c, c_ptr.p, c_vpos.p, vec.p, nullptr, fc.p, st);
-    launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, fc.p, uc.p, st);
+    coarse_apply(fc.p, uc.p);
     // vec_D' holds -a_s: out = a_s - Phi u
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 0.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, dv.p, st);
@@ -975,7 +992,7 @@ struct fetidp_solver {
   void recover(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, double* xout) {
     apply_W<T>(W, lam.p);
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
-    launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
+    coarse_apply(gc.p, uc.p);
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
@@ -1007,7 +1024,7 @@ struct fetidp_solver {
     if (!head_queued) timed(9, [&] { apply_W<T>(W, p.p); });
     timed(10, [&] {
       launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
-      launch_dense_gemv(topo.n_c, sinv_ld(), sinv.p, gc.p, uc.p, st);
+      coarse_apply(gc.p, uc.p);
       launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0,
                              dbuf.p, st);
       const double* a1[1] = {p.p};
@@ -1043,7 +1060,7 @@ struct fetidp_solver {
       return;
     }
     if (head_queued) throw std::logic_error("graph iterations form their own W p");
-    const void* key[3] = {W.p, p64 ? (const void*)Pd.p : (const void*)Pf.p, sinv.p};
+    const void* key[3] = {W.p, p64 ? (const void*)Pd.p : (const void*)Pf.p, cpool.p};
     const int key_t = int(sizeof(T)) | (deluxe ? 256 : 0);
     if (!pcpg_exec || pcpg_key_t != key_t || std::memcmp(key, pcpg_key, sizeof(key))) {
       if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));

@@ -62,9 +62,11 @@ template <typename T>
 void launch_extract_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp, int max_ndp,
                       const DevSub* subs, const T* pool, const int64_t* rowabs, const int32_t* ftile,
                       const uint8_t* tflag, T* W, hipStream_t st);
-// S~^-1 (dense, row stride ld, even) from the trailing block of the coarse pool.
-void launch_extract_sinv(int64_t nc, int64_t ncp, int64_t ld, const double* cpool,
-                         const int64_t* rowabs, const int32_t* ftile, double* sinv, hipStream_t st);
+// y = S~^-1 x from the factored coarse pool (its trailing block holds -S~^-1 as lower
+// tiles): tiles lists the tn(tn+1)/2 tile pairs (I, J <= I) of that block, tn = ncp / 64;
+// part holds tn * tn * 64 doubles.
+void launch_coarse_symv(int64_t nc, int tn, int64_t ntiles, const int32_t* tiles, const double* cpool,
+                        const int64_t* cra, const double* x, double* part, double* y, hipStream_t st);
 
 // Forward solve, one launch per block column: yv_r = L_rr^-1 vec_r; afterwards
 // vec_c = f_c - L_cr y and vec_D' = -L_D'r y (= -E K_rr^-1 f_r).
@@ -110,9 +112,6 @@ void launch_lambda_scatter(int64_t nl, const int64_t* lam_pos, const double* sca
                            double* out, hipStream_t st);
 void launch_coarse_gather(int64_t nc, const int64_t* ptr, const int64_t* pos, const double* src,
                           const double* add, double* out, hipStream_t st);
-// y = A x, A dense row-major n x n with an even row stride ld (16-byte aligned rows).
-void launch_dense_gemv(int64_t n, int64_t ld, const double* A, const double* x, double* y,
-                       hipStream_t st);
 void launch_fill_U(int ns, int max_m, const DevSub* subs, const int32_t* d_lam, const double* ubar,
                    const int32_t* c_loc, const double* uc, double* U, hipStream_t st);
 void launch_split_rhs(int64_t n, const int32_t* vdof, const double* vw, const double* f, double* vec,

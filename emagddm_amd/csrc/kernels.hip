@@ -953,22 +953,108 @@ template void launch_extract_W<double>(int64_t, const SymTile*, int, int, int, c
 template void launch_extract_W<float>(int64_t, const SymTile*, int, int, int, const DevSub*,
                                       const float*, const int64_t*, const int32_t*, const uint8_t*, float*, hipStream_t);
 
-__global__ __launch_bounds__(256) void k_extract_sinv(int64_t nc, int64_t ncp, int64_t ld,
-                                                      const double* __restrict__ cpool,
-                                                      const int64_t* __restrict__ rowabs,
-                                                      const int32_t* __restrict__ ftile,
-                                                      double* __restrict__ sinv) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nc * ld) return;
-  const int a = int(e / ld), b = int(e % ld);
-  sinv[e] = b < nc ? -pool_sym(cpool, rowabs, ftile, int(ncp) + a, int(ncp) + b) : 0.0;
+// S~^-1 x for the coarse problem, straight from the factored coarse pool: the trailing
+// block of [[S~, I], [I, 0]] holds -S~^-1 as lower 64x64 tiles (column-major inside a
+// tile: element (r, c) of tile (I, J) at c * 64 + r; diagonal tiles full).  One work-group
+// per lower tile (I, J) reads it once and writes two 64-vectors of partial sums into
+// slot (I, J) (T_IJ x_J) and, for I > J, slot (J, I) (T_IJ^T x_I) of a (Tn x Tn) array;
+// k_coarse_reduce sums each row of slots (in a fixed order) and negates.  Lane = tile row and
+// wave w holds columns 16w .. 16w+15, so every load is one coalesced 512-byte column; the
+// column sums use the same halving butterfly as k_symv_tile, the row sums a 4-way LDS
+// reduction.  Entries of x past nc read as zero (padding rows of the block are zero).
+__global__ __launch_bounds__(256) void k_coarse_symv(int64_t nc, int tn, const int32_t* __restrict__ tiles,
+                                                     const double* __restrict__ cpool,
+                                                     const int64_t* __restrict__ cra,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ part) {
+  __shared__ double xI[64], xJ[64];
+  __shared__ double colp[4][64];
+  const int I = tiles[2 * blockIdx.x], J = tiles[2 * blockIdx.x + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* A = cpool + cra[tn + I] + (int64_t)(tn + J) * kTileElems;
+  double at[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) at[q] = A[(16 * wv + q) * 64 + lane];
+  if (tid < 128) {
+    const int64_t idx = int64_t((tid >> 6) ? I : J) * 64 + lane;
+    const double xv = idx < nc ? x[idx] : 0.0;
+    if (tid >> 6) xI[lane] = xv;
+    else xJ[lane] = xv;
+  }
+  __syncthreads();
+  const double xr = xI[lane];
+  double rv[16];
+  double rs_part = 0.0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    rv[q] = at[q] * xr;
+    rs_part += at[q] * xJ[16 * wv + q];
+  }
+  colp[wv][lane] = rs_part;
+  if (I != J) {  // uniform per work-group: column sums T_IJ^T x_I
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const bool hi = lane & 32;
+      const double keep = hi ? rv[i + 8] : rv[i], send = hi ? rv[i] : rv[i + 8];
+      rv[i] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool hi = lane & 16;
+      const double keep = hi ? rv[i + 4] : rv[i], send = hi ? rv[i] : rv[i + 4];
+      rv[i] = keep + __shfl_xor(send, 16);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const bool hi = lane & 8;
+      const double keep = hi ? rv[i + 2] : rv[i], send = hi ? rv[i] : rv[i + 2];
+      rv[i] = keep + __shfl_xor(send, 8);
+    }
+    {
+      const bool hi = lane & 4;
+      const double keep = hi ? rv[1] : rv[0], send = hi ? rv[0] : rv[1];
+      rv[0] = keep + __shfl_xor(send, 4);
+    }
+    double cs = rv[0];
+    cs += __shfl_xor(cs, 2);
+    cs += __shfl_xor(cs, 1);
+    if ((lane & 3) == 0) {
+      const int col = 16 * wv + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                      ((lane >> 2) & 1);
+      part[((int64_t)J * tn + I) * 64 + col] = cs;
+    }
+  }
+  __syncthreads();
+  if (tid < 64)
+    part[((int64_t)I * tn + J) * 64 + tid] = (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
 }
 
-void launch_extract_sinv(int64_t nc, int64_t ncp, int64_t ld, const double* cpool,
-                         const int64_t* rowabs, const int32_t* ftile, double* sinv, hipStream_t st) {
-  if (nc <= 0) return;
-  hipLaunchKernelGGL(k_extract_sinv, dim3((nc * ld + 255) / 256), dim3(256), 0, st, nc, ncp, ld, cpool,
-                     rowabs, ftile, sinv);
+// y[X*64 + e] = -sum_{Y < tn} part[(X, Y)][e] for X*64 + e < nc.  Wave w sums the slots
+// Y = w (mod 4) with two accumulators (even and odd steps), so each lane keeps
+// ~tn/4 independent loads in flight; the eight partials are then added in a fixed order.
+__global__ __launch_bounds__(256) void k_coarse_reduce(int64_t nc, int tn, const double* __restrict__ part,
+                                                       double* __restrict__ y) {
+  __shared__ double red[4][64];
+  const int X = blockIdx.x, e = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const double* p = part + (int64_t)X * tn * 64 + e;
+  double s0 = 0.0, s1 = 0.0;
+  int Y = w;
+  for (; Y + 4 < tn; Y += 8) {
+    s0 += p[(int64_t)Y * 64];
+    s1 += p[(int64_t)(Y + 4) * 64];
+  }
+  if (Y < tn) s0 += p[(int64_t)Y * 64];
+  red[w][e] = s0 + s1;
+  __syncthreads();
+  const int64_t i = int64_t(X) * 64 + e;
+  if (w == 0 && i < nc) y[i] = -((red[0][e] + red[1][e]) + (red[2][e] + red[3][e]));
+}
+
+void launch_coarse_symv(int64_t nc, int tn, int64_t ntiles, const int32_t* tiles, const double* cpool,
+                        const int64_t* cra, const double* x, double* part, double* y, hipStream_t st) {
+  if (nc <= 0 || ntiles <= 0) return;
+  hipLaunchKernelGGL(k_coarse_symv, dim3(ntiles), dim3(256), 0, st, nc, tn, tiles, cpool, cra, x, part);
+  hipLaunchKernelGGL(k_coarse_reduce, dim3(tn), dim3(256), 0, st, nc, tn, part, y);
 }
 
 // ---------------------------------------------------------------------------
@@ -1567,58 +1653,6 @@ void launch_coarse_gather(int64_t nc, const int64_t* ptr, const int64_t* pos, co
   if (nc <= 0) return;
   hipLaunchKernelGGL(k_coarse_gather, dim3((nc + 255) / 256), dim3(256), 0, st, nc, ptr, pos, src,
                      add, out);
-}
-
-// y = A x for a dense row-major n x n matrix (ld = row stride, a multiple of 2): one wave
-// per two rows, 16-byte loads, two independent accumulators per row; the fixed order
-// of the lane partials and the shuffle tree keeps the result bitwise repeatable.
-__global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, int64_t ld, const double* __restrict__ A,
-                                                    const double* __restrict__ x,
-                                                    double* __restrict__ y) {
-  const int64_t row = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 2;
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;
-  const bool two = row + 1 < n;
-  const double2* a0 = reinterpret_cast<const double2*>(A + row * ld);
-  const double2* a1 = reinterpret_cast<const double2*>(A + (two ? row + 1 : row) * ld);
-  const double2* x2 = reinterpret_cast<const double2*>(x);
-  const int64_t n2 = n / 2;
-  double s0 = 0.0, s1 = 0.0, t0 = 0.0, t1 = 0.0;
-  int64_t k = lane;
-  for (; k + 64 < n2; k += 128) {
-    const double2 u0 = a0[k], u1 = a1[k], v = x2[k];
-    const double2 w0 = a0[k + 64], w1 = a1[k + 64], z = x2[k + 64];
-    s0 = fma(u0.x, v.x, s0); s0 = fma(u0.y, v.y, s0);
-    s1 = fma(u1.x, v.x, s1); s1 = fma(u1.y, v.y, s1);
-    t0 = fma(w0.x, z.x, t0); t0 = fma(w0.y, z.y, t0);
-    t1 = fma(w1.x, z.x, t1); t1 = fma(w1.y, z.y, t1);
-  }
-  if (k < n2) {
-    const double2 u0 = a0[k], u1 = a1[k], v = x2[k];
-    s0 = fma(u0.x, v.x, s0); s0 = fma(u0.y, v.y, s0);
-    s1 = fma(u1.x, v.x, s1); s1 = fma(u1.y, v.y, s1);
-  }
-  if ((n & 1) && lane == 0) {  // odd order: the last column
-    s0 = fma(A[row * ld + n - 1], x[n - 1], s0);
-    s1 = fma(A[(two ? row + 1 : row) * ld + n - 1], x[n - 1], s1);
-  }
-  s0 += t0;
-  s1 += t1;
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    s0 += __shfl_xor(s0, o);
-    s1 += __shfl_xor(s1, o);
-  }
-  if (lane == 0) {
-    y[row] = s0;
-    if (two) y[row + 1] = s1;
-  }
-}
-
-void launch_dense_gemv(int64_t n, int64_t ld, const double* A, const double* x, double* y,
-                       hipStream_t st) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_dense_gemv, dim3((n + 7) / 8), dim3(256), 0, st, n, ld, A, x, y);
 }
 
 __global__ __launch_bounds__(256) void k_fill_U(const DevSub* __restrict__ subs,
