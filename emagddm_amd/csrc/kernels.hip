@@ -1077,24 +1077,39 @@ __device__ __forceinline__ void tile_matvec(const T* L, const double* b, double*
 // instruction of a wave reads four whole 128-byte column segments; the 16 partial sums
 // of a column are combined with shuffles.  out[p] = sum_r L(r, (t >> 4) + 16p) u(r), in
 // all 16 lanes of the group.
+// colsum4 in two parts: the thread's 16 tile values (v[4p + m] = L(q + 16m, cg + 16p)),
+// which can be loaded ahead, and the sums over them.
 template <typename T>
-__device__ __forceinline__ void colsum4(const T* L, const double* u, int tid, double (&out)[4]) {
+__device__ __forceinline__ void colsum4_load(const T* L, int tid, T (&v)[16]) {
   const int q = tid & 15, cg = tid >> 4;
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) v[4 * p + m] = L[(cg + 16 * p) * 64 + q + 16 * m];
+}
+template <typename T>
+__device__ __forceinline__ void colsum4_sum(const T (&v)[16], const double* u, int tid, double (&out)[4]) {
+  const int q = tid & 15;
   double uq[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) uq[m] = u[q + 16 * m];
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    const T* col = L + (cg + 16 * p) * 64 + q;
     double sum = 0.0;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) sum += (double)col[16 * m] * uq[m];
+    for (int m = 0; m < 4; ++m) sum += (double)v[4 * p + m] * uq[m];
     sum += __shfl_xor(sum, 1);
     sum += __shfl_xor(sum, 2);
     sum += __shfl_xor(sum, 4);
     sum += __shfl_xor(sum, 8);
     out[p] = sum;
   }
+}
+template <typename T>
+__device__ __forceinline__ void colsum4(const T* L, const double* u, int tid, double (&out)[4]) {
+  T v[16];
+  colsum4_load(L, tid, v);
+  colsum4_sum(v, u, tid, out);
 }
 
 // Forward step J: y_J = L_JJ^-1 b_J (written to yv by the row-J workgroup);
@@ -1212,6 +1227,10 @@ __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ sub
   const bool in_env = I >= ftile[S.ft_off + J];  // tile (J, I) is stored
   if (I >= J || (!in_env && I != J - 1)) return;
   const int tid = threadIdx.x;
+  const bool last = I == J - 1;  // uniform per work-group: also forms x_{J-1}
+  // the inverse diagonal tile does not depend on this step: its loads go out first
+  T vinv[16];
+  if (last) colsum4_load(inv + inv_base[blockIdx.y] + (int64_t)I * kTileElems, tid, vinv);
   double* t = tbuf + S.vec_off;
   double cs[4] = {0.0, 0.0, 0.0, 0.0};
   if (in_env)
@@ -1225,10 +1244,10 @@ __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ sub
       t[I * 64 + c] = nt;
       ts[c] = nt;
     }
-  if (I != J - 1) return;  // uniform per work-group
+  if (!last) return;
   __syncthreads();
   double x[4];
-  colsum4(inv + inv_base[blockIdx.y] + (int64_t)I * kTileElems, ts, tid, x);
+  colsum4_sum(vinv, ts, tid, x);
   if ((tid & 15) == 0)
 #pragma unroll
     for (int p = 0; p < 4; ++p) xout[S.vec_off + (int64_t)I * 64 + (tid >> 4) + 16 * p] = x[p];
