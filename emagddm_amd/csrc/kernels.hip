@@ -1604,7 +1604,8 @@ __global__ __launch_bounds__(256) void k_dual_correct(const DevSub* __restrict__
   const T* ph = W + S.w_off + sym_elems(S.ndp);  // Phi^T rows
   double s0 = 0.0, s1 = 0.0;
   int c = q;
-  for (; c + 4 < S.nc; c += 8) {
+#pragma unroll 4
+  for (; c + 4 < S.nc; c += 8) {  // unrolled: eight row loads in flight per lane
     s0 = fma((double)ph[(int64_t)c * S.ndp + k], ul[c], s0);
     s1 = fma((double)ph[(int64_t)(c + 4) * S.ndp + k], ul[c + 4], s1);
   }
