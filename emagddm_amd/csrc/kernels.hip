@@ -1550,7 +1550,8 @@ __global__ __launch_bounds__(64) void k_symv_reduce(const DevSub* __restrict__ s
   const int r = threadIdx.x;
   const double* p = part + S.sp_off + (int64_t)I * Td * 64 + r;
   double sum = 0.0;
-  for (int J = 0; J < Td; ++J) sum += p[(int64_t)J * 64];
+#pragma unroll 8
+  for (int J = 0; J < Td; ++J) sum += p[(int64_t)J * 64];  // unrolled: loads ahead of the adds
   out[(which == 0 ? S.y_off : S.dvec_off) + I * 64 + r] = sum;
 }
 
