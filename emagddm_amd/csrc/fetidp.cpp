@@ -347,6 +347,7 @@ struct fetidp_solver {
   }
 
   bool fell_back = false;  // FP32 requested but the factor fell back to FP64
+  bool stalled = false;    // the last FP32 outer refinement stalled (solve hands over to FP64)
   std::string fallback_reason;
   bool fp32() const { return cfg.precision == FETIDP_FP32_REFINE && !fell_back; }
 
@@ -360,8 +361,13 @@ struct fetidp_solver {
     if (nmat <= 0) throw InputError("setup: n_mat must be positive");
     for (int64_t t = 0; t < n_tets; ++t)
       if (h_mat[t] < 0 || h_mat[t] >= nmat) throw InputError("setup: material id out of range");
+    // nu must be positive; kappa only finite: kappa < 0 (an indefinite system, e.g. a
+    // -omega^2 eps term) is accepted here and reported as NUMERICAL_ERROR by the Cholesky
+    // factorization, as the reference's LLT backend reports a non-SPD matrix
+    // (include/numeric/eigen_solver_backends.h:159-170, src/numeric/unified_direct_solver.cpp:122-136)
     for (int32_t m = 0; m < nmat; ++m)
-      if (!(h_nu[m] > 0.0) || !(h_kappa[m] >= 0.0)) throw InputError("setup: need nu > 0, kappa >= 0");
+      if (!(h_nu[m] > 0.0) || !std::isfinite(h_nu[m]) || !std::isfinite(h_kappa[m]))
+        throw InputError("setup: need finite nu > 0 and finite kappa");
     const double t0 = now_ms();
     setup_done = device_ready = factored = rhs_set = false;
     graph_built = false;
@@ -547,7 +553,8 @@ struct fetidp_solver {
     if (!setup_done) throw InputError("set_materials before setup");
     if (nmat != n_mat || !h_nu || !h_kappa) throw InputError("set_materials: n_mat mismatch");
     for (int32_t m = 0; m < nmat; ++m)
-      if (!(h_nu[m] > 0.0) || !(h_kappa[m] >= 0.0)) throw InputError("set_materials: need nu > 0, kappa >= 0");
+      if (!(h_nu[m] > 0.0) || !std::isfinite(h_nu[m]) || !std::isfinite(h_kappa[m]))
+        throw InputError("set_materials: need finite nu > 0 and finite kappa");
     h_nu_.assign(h_nu, h_nu + nmat);
     h_kappa_.assign(h_kappa, h_kappa + nmat);
     if (device_ready) {
@@ -741,6 +748,26 @@ struct fetidp_solver {
     return hi / lo > 1e4;
   }
 
+  // Relative accuracy of an FP32 FETI-DP solve of this curl-curl system, estimated as
+  // cond(K_rr) * 2^-24 with cond(K_rr) ~ nu_max / (kappa_min h_min^2): FP32 rounding of
+  // the assembled entries, O(2^-24 nu / h^2), perturbs the discrete-gradient modes whose
+  // energy is only kappa.  Measured outer contraction of the FP64 refinement per step
+  // (uniform nu = 1, kappa = 1e-2): 0.19 at 16^3 (estimate 1.5e-3), 0.6 at 48^3 (1.4e-2).
+  double fp32_estimate() const {
+    double nu_max = 0.0, ka_min = 1e300;
+    for (int32_t m = 0; m < n_mat; ++m) {
+      nu_max = std::max(nu_max, h_nu_[m]);
+      ka_min = std::min(ka_min, h_kappa_[m]);
+    }
+    if (!(ka_min > 0.0) || !(h_min > 0.0)) return 1e300;
+    return nu_max / (ka_min * h_min * h_min) * 0x1p-24;
+  }
+  // Above this estimate FP32_REFINE factors in FP64 from the start (fell_back = 1).
+  static constexpr double kFp32Limit = 5e-3;
+  // FP32 outer refinement is abandoned for FP64 when two outer steps reduce the
+  // residual by less than this factor (a contraction worse than 0.5 per step).
+  static constexpr double kFp32Stall = 0.25;
+
   bool precond_fp64(bool dl) const {
     if (dl) return true;  // the face blocks are formed and inverted in FP64
     if (cfg.precond_precision == 1) return false;
@@ -856,6 +883,16 @@ struct fetidp_solver {
     fell_back = false;
     deluxe = use_deluxe() && plans.faces.n > 0;
     p64 = precond_fp64(deluxe);
+    if (fp32() && fp32_estimate() > kFp32Limit) {
+      // the FP32 solve could not reach the FP64 tolerance in a useful number of outer
+      // steps: factor in FP64 from the start
+      fell_back = true;
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "FP32 accuracy estimate nu/(kappa h^2) 2^-24 = %.3g exceeds %.3g",
+                    fp32_estimate(), kFp32Limit);
+      fallback_reason = buf;
+      if (cfg.verbose & 1) std::fprintf(stderr, "[fetidp] FP32 factor not used: %s\n", buf);
+    }
     if (fp32()) {
       pool64.release(); inv64.release(); W64.release();
       try {
@@ -1177,6 +1214,25 @@ struct fetidp_solver {
     int its = 0, status = FETIDP_SUCCESS;
     double dn = 0.0, res = 0.0;
     first_init = -1.0;
+    stalled = false;
+    {
+      // ||b|| < 1e-15: x = 0, no iteration, residual 0 (src/numeric/iterative_solvers.cpp:436-442)
+      double o[1];
+      const double* af[1] = {f_glob.p};
+      const double fn = std::sqrt(dots(1, af, af, nf, o));
+      if (!std::isfinite(fn)) throw InputError("solve: right-hand side is not finite");
+      if (fn < 1e-15) {
+        xtot.zero(st);
+        lam.zero(st);
+        last_dual = init_dual = 0.0;
+        S.iterations = 0;
+        S.refinements = 0;
+        S.residual_norm = 0.0;
+        S.dual_residual = 0.0;
+        S.initial_dual_residual = 0.0;
+        return FETIDP_SUCCESS;
+      }
+    }
     if (!fp32()) {
       condense<T>(pool, inv, W, f_glob.p, true);
       if (h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
@@ -1254,6 +1310,18 @@ struct fetidp_solver {
         ref_hist.push_back(res);
         if (res <= 0.5 * cfg.tolerance) { status = FETIDP_SUCCESS; break; }
         if (its >= cfg.max_iterations) { status = FETIDP_MAX_ITER_REACHED; break; }
+        const size_t nh = ref_hist.size();
+        if (nh >= 3 && res > kFp32Stall * ref_hist[nh - 3]) {
+          // the FP32 preconditioner gains less than 2x per outer step: hand over to FP64
+          stalled = true;
+          status = FETIDP_MAX_ITER_REACHED;
+          break;
+        }
+      }
+      if (stalled) {
+        S.refinements = k + 1;
+        S.iterations = its;
+        return status;
       }
       // the recursive residual is confirmed with the true FP64 residual
       res = global_residual(xtot.p, f_glob.p);
@@ -1276,6 +1344,24 @@ struct fetidp_solver {
     last_learned = false;
     int status = fp32() ? solve_impl<float>(pool32, inv32, W32, h_lam0, S)
                         : solve_impl<double>(pool64, inv64, W64, h_lam0, S);
+    if (stalled) {
+      // FP32 refinement stalled: refactor in FP64 and solve again (fell_back = 1); the
+      // FP32 iterations stay counted in the statistics
+      const int its32 = S.iterations;
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "FP32 refinement stalled at residual %.3g after %d outer steps",
+                    ref_hist.empty() ? 0.0 : ref_hist.back(), S.refinements);
+      fallback_reason = buf;
+      if (cfg.verbose & 1) std::fprintf(stderr, "[fetidp] %s: refactoring in FP64\n", buf);
+      fell_back = true;
+      stalled = false;
+      pool32.release(); inv32.release(); W32.release();
+      const double tf = now_ms();
+      factor_impl<double>(pool64, inv64, W64);
+      last_factor_ms += now_ms() - tf;
+      status = solve_impl<double>(pool64, inv64, W64, h_lam0, S);
+      S.iterations += its32;
+    }
     if (h_x) HIPCHECK(hipMemcpyAsync(h_x, xtot.p, sizeof(double) * topo.n_free, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     if (last_learned) {

@@ -131,8 +131,13 @@ class LambdaMPNN:
             pass
 
 
+# workloads served by a network trained on a smaller family of the same kind (the
+# network is local, so its weights do not depend on the mesh size)
+TRAINED_ON = {"cube96j": "cube48c"}
+
+
 def default_weights(workload: str):
-    path = os.path.join(WEIGHTS_DIR, f"lambda0_{workload}.npy")
+    path = os.path.join(WEIGHTS_DIR, f"lambda0_{TRAINED_ON.get(workload, workload)}.npy")
     return path if os.path.exists(path) else None
 
 

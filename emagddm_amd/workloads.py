@@ -187,6 +187,9 @@ WORKLOADS = {
     "cube32l": dict(n=32, parts=4, jitter=None, materials="nu_per_sub", loops=(1, 4)),
     # 96^3 hexes, 12x12x12 jittered boxes, contrast materials
     "cube96j": dict(n=96, parts=12, jitter=(6, 10), materials="contrast", loops=2),
+    # 48^3 hexes, 6x6x6 jittered boxes, contrast materials (training family of the
+    # lambda0 network used at 96^3)
+    "cube48c": dict(n=48, parts=6, jitter=(6, 10), materials="contrast", loops=2),
 }
 
 
@@ -207,17 +210,25 @@ def _draw_loops(rng, loops):
 
 def family_parameters(name: str, seed: int):
     """(nu, kappa, loops) that make_workload(name, seed) draws, for reuse on the
-    mesh, partition and material map of seed 0.  Families whose material map
-    itself depends on the seed ("contrast") are not supported."""
+    mesh, partition and material map of seed 0.  For "contrast" families the
+    material map (which tets lie in the nu = 1e-3 sphere) stays that of seed 0;
+    the per-subdomain nu and kappa and the loops are the seed's."""
     c = WORKLOADS[name]
     n, parts = c["n"], c["parts"]
-    if c["materials"] not in ("uniform", "nu_per_sub"):
-        raise ValueError(f"{name}: material map depends on the seed")
     rng = np.random.default_rng(np.random.SeedSequence([seed, n, parts]))
     if c["jitter"] is not None:  # consume the cut draws exactly as make_problem does
         for _ in range(3):
             jittered_widths(n, parts, c["jitter"][0], c["jitter"][1], rng)
-    nu, kappa = _draw_materials(rng, c["materials"], parts ** 3)
+    n_sub = parts ** 3
+    if c["materials"] == "contrast":
+        nu_s = 10.0 ** rng.uniform(-3, 0, size=n_sub)
+        ka_s = 10.0 ** rng.uniform(-4, 0, size=n_sub)
+        rng.uniform(0.3, 0.7, size=3)  # the sphere's centre and radius (map of seed 0)
+        rng.uniform(0.1, 0.2)
+        nu = np.concatenate([nu_s, np.full(n_sub, 1e-3)])
+        kappa = np.concatenate([ka_s, ka_s])
+    else:
+        nu, kappa = _draw_materials(rng, c["materials"], n_sub)
     return nu, kappa, _draw_loops(rng, c["loops"])
 
 

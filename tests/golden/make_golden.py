@@ -4,7 +4,8 @@ Run in the build container only (it needs oracle/_ref/libref_solvers.so, built
 from /root/reference by oracle/ref/Makefile).  The fixtures are data: the
 seeded problem's input checksums and the reference solvers' outputs.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # micro4, micro8c, cube16
+    python tests/golden/make_golden.py cube48j    # full-size summary (minutes)
 """
 import ctypes as C
 import os
@@ -44,6 +45,44 @@ def run(lib, p, cg_max=5000):
                 jq_sum=p.jq.sum(), xyz_sum=p.xyz.sum())
 
 
+N_SAMPLE, N_PROJ = 512, 8
+
+
+def summary(x, seed=48):
+    """A full-size solution as a small fixture: its 2-norm, its entries at N_SAMPLE
+    seeded indices, and its inner products with N_PROJ seeded standard-normal vectors
+    (regenerated by the test from the same seeds)."""
+    n = x.size
+    idx = np.sort(np.random.default_rng(seed).choice(n, N_SAMPLE, replace=False))
+    proj = np.array([np.random.default_rng(seed + 1 + k).standard_normal(n) @ x
+                     for k in range(N_PROJ)])
+    return dict(x_norm=np.linalg.norm(x), sample_idx=idx, sample_x=x[idx], proj=proj,
+                proj_seed=seed)
+
+
+def main_large(name="cube48j"):
+    """The reference Eigen LLT (through UnifiedDirectSolver) on the full 48^3 system,
+    kept as a summary (the 753,552-entry solution itself is 6 MB)."""
+    lib = ref_lib()
+    p = W.make_workload(name)
+    asm = assembly.assemble(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.pec_pairs, p.jq)
+    A, f = asm["A"], asm["f"]
+    n = A.shape[0]
+    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    x = np.zeros(n)
+    fm, sm = C.c_double(), C.c_double()
+    st = lib.ref_solve_llt(n, ip(rp), ip(col), dp(val), dp(f), dp(x), C.byref(fm), C.byref(sm))
+    assert st == 0
+    res = np.linalg.norm(f - A @ x) / np.linalg.norm(f)
+    out = dict(n_free=n, nnz=A.nnz, f_norm=np.linalg.norm(f), a_sum=A.data.sum(),
+               jq_sum=p.jq.sum(), xyz_sum=p.xyz.sum(), llt_residual=res,
+               llt_factor_ms=fm.value, llt_solve_ms=sm.value, **summary(x))
+    np.savez_compressed(os.path.join(HERE, f"{name}_llt.npz"), **out)
+    print(name, n, res, fm.value, sm.value)
+
+
 def main():
     lib = ref_lib()
     cases = {
@@ -60,4 +99,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1:
+        main_large(sys.argv[1])
+    else:
+        main()

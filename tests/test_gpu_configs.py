@@ -1,0 +1,111 @@
+"""Full-size GPU runs of the benchmark configurations, checked against the reference
+solver and by size-independent properties.
+
+* cube48j (48^3 hexes, 6x6x6 jittered boxes): the FP64 solution against the
+  reference's own Eigen LLT solution of the same system (UnifiedDirectSolver,
+  src/numeric/unified_direct_solver.cpp:89-139), kept as a summary fixture
+  (tests/golden/cube48j_llt.npz: norm, 512 sampled entries, 8 seeded projections);
+  residual recomputed on the host from the assembled system.
+* cube32l (32^3, 4x4x4 boxes, nu per box, 1-4 loops): the committed lambda0 weights on
+  held-out seeds, learned start vs lambda0 = 0 (the reference's x0 = 0,
+  src/numeric/iterative_solvers.cpp:424-503).
+* cube96j (96^3, 12x12x12 jittered boxes, nu/kappa contrast + sphere): stop rule on
+  the host-recomputed residual (:445-474), linearity in the source, learned vs zero
+  start.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from emagddm_amd import FETIDPSolver, SolverStatus, workloads as W
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _host_residual(s, x):
+    rp, col, val, f = s.global_system()
+    A = sp.csr_matrix((val, col, rp), shape=(f.size, f.size))
+    return float(np.linalg.norm(f - A @ x) / np.linalg.norm(f))
+
+
+def test_cube48j_matches_reference_llt(gpu_available):
+    g = np.load(os.path.join(GOLDEN, "cube48j_llt.npz"))
+    p = W.make_workload("cube48j")
+    print(f"[configs] cube48j tets={p.n_tets}", flush=True)
+    assert np.isclose(p.jq.sum(), g["jq_sum"], rtol=1e-12, atol=1e-12)
+    s = FETIDPSolver().setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    r = s.solve()
+    assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-8, r
+    res = _host_residual(s, r.x)
+    n = r.x.size
+    assert n == int(g["n_free"])
+    xn = float(g["x_norm"])
+    # each check below is implied by ||x - x_llt|| <= 1e-8 ||x_llt|| (Cauchy-Schwarz)
+    dnorm = abs(np.linalg.norm(r.x) - xn) / xn
+    dsamp = np.abs(r.x[g["sample_idx"]] - g["sample_x"]).max() / xn
+    seed = int(g["proj_seed"])
+    dproj = max(abs(np.random.default_rng(seed + 1 + k).standard_normal(n) @ r.x - pk)
+                / (np.sqrt(n) * xn) for k, pk in enumerate(g["proj"]))
+    print(f"[configs] cube48j its={r.iterations} res={r.residual_norm:.2e} host={res:.2e} "
+          f"dnorm={dnorm:.1e} dsample={dsamp:.1e} dproj={dproj:.1e}", flush=True)
+    assert res <= 1e-8
+    assert dnorm <= 1e-8 and dsamp <= 1e-8 and dproj <= 1e-8
+
+
+def test_cube32l_learned_heldout(gpu_available):
+    from emagddm_amd.lambda0 import Family, LambdaMPNN, default_weights, evaluate, summarize
+    path = default_weights("cube32l")
+    assert path is not None, "committed cube32l weights missing"
+    fam = Family("cube32l")
+    model = LambdaMPNN.load(path)
+    fam.solver.set_lambda0_model(None)
+    seeds = list(range(1_000_000, 1_000_000 + 256))  # held out: training used seeds 1..2048
+    print(f"[configs] cube32l held-out seeds={len(seeds)}", flush=True)
+    rows = evaluate(fam, model, seeds)
+    sm = summarize(rows)
+    print(f"[configs] cube32l {sm}", flush=True)
+    assert sm["all_ok"] and sm["max_res_learned"] <= 1e-8
+    assert sm["max_x_rel_diff"] <= 1e-7
+    # what the trained network gains over lambda0 = 0 on unseen sources and materials
+    assert sm["init_dual_learned_mean"] <= 0.4 * sm["init_dual_zero_mean"]
+    assert sm["its_learned_mean"] <= sm["its_zero_mean"] - 0.5
+
+
+def test_cube96j_full_size(gpu_available):
+    from emagddm_amd.lambda0 import LambdaMPNN, default_weights
+    p = W.make_workload("cube96j")
+    print(f"[configs] cube96j tets={p.n_tets}", flush=True)
+    s = FETIDPSolver().setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    r = s.solve()
+    print(f"[configs] cube96j its={r.iterations} res={r.residual_norm:.2e} "
+          f"refinements={r.refinements} deluxe={r.deluxe}", flush=True)
+    assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-8
+    res = _host_residual(s, r.x)
+    print(f"[configs] cube96j host residual {res:.2e}", flush=True)
+    assert res <= 1e-8 and abs(res - r.residual_norm) <= 1e-3 * r.residual_norm + 2e-12
+    # linearity in the source on the same factor
+    s.set_rhs_tet_J(-2.5 * p.jq)
+    r2 = s.solve()
+    assert r2.status == SolverStatus.SUCCESS
+    assert np.linalg.norm(r2.x + 2.5 * r.x) <= 1e-7 * np.linalg.norm(2.5 * r.x)
+    # learned start: same solution to 1e-7, same stop rule
+    path = default_weights("cube96j")
+    if path is None:
+        pytest.skip("no lambda0 weights for cube96j")
+    s.set_rhs_tet_J(p.jq)
+    model = LambdaMPNN.load(path)
+    s.set_lambda0_model(model)
+    r3 = s.solve()
+    s.set_lambda0_model(None)
+    d = np.linalg.norm(r3.x - r.x) / np.linalg.norm(r.x)
+    print(f"[configs] cube96j learned its={r3.iterations} (zero {r.iterations}) "
+          f"init={r3.initial_dual_residual:.3f} diff={d:.1e}", flush=True)
+    assert r3.status == SolverStatus.SUCCESS and r3.learned_guess and r3.residual_norm <= 1e-8
+    assert d <= 1e-7
