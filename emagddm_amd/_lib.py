@@ -50,8 +50,19 @@ EXPORTS = [
     "emag_mpnn_create", "emag_mpnn_destroy", "emag_mpnn_last_error", "emag_mpnn_num_params",
     "emag_mpnn_get_params", "emag_mpnn_set_params", "emag_mpnn_set_graph", "emag_mpnn_forward",
     "emag_mpnn_gradient", "emag_mpnn_train_step", "emag_mpnn_mfma_probe",
+    "emag_problem_write", "emag_problem_read", "emag_problem_free", "emag_problem_last_error",
 ]
-HEADERS = ["emagddm_fetidp.h", "emagddm_mpnn.h"]
+HEADERS = ["emagddm_fetidp.h", "emagddm_mpnn.h", "emagddm_problem.h"]
+
+
+class Problem(C.Structure):
+    """struct emag_problem (include/emagddm_problem.h)."""
+    _fields_ = [("n_nodes", C.c_int64), ("n_tets", C.c_int64), ("n_pec", C.c_int64),
+                ("n_mat", C.c_int32), ("n_sub", C.c_int32),
+                ("xyz", C.POINTER(C.c_double)), ("tet2node", C.POINTER(C.c_int32)),
+                ("tet_material", C.POINTER(C.c_int32)), ("tet_subdomain", C.POINTER(C.c_int32)),
+                ("nu", C.POINTER(C.c_double)), ("kappa", C.POINTER(C.c_double)),
+                ("pec_edge_nodes", C.POINTER(C.c_int32)), ("jq", C.POINTER(C.c_double))]
 
 _lib = None
 
@@ -127,6 +138,12 @@ def load():
     lib.emag_mpnn_gradient.argtypes = [P, C.c_int64, P, P, P, fp, dp, P]
     lib.emag_mpnn_train_step.argtypes = [P, C.c_int64, P, P, P, C.c_float, dp, P]
     lib.emag_mpnn_mfma_probe.argtypes = [fp, fp, fp]
+    lib.emag_problem_write.argtypes = [C.c_char_p, C.POINTER(Problem)]
+    lib.emag_problem_read.argtypes = [C.c_char_p, C.POINTER(Problem)]
+    lib.emag_problem_free.argtypes = [C.POINTER(Problem)]
+    lib.emag_problem_free.restype = None
+    lib.emag_problem_last_error.argtypes = []
+    lib.emag_problem_last_error.restype = C.c_char_p
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is C.c_int:  # default int return -> status
