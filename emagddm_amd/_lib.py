@@ -51,8 +51,10 @@ EXPORTS = [
     "emag_mpnn_get_params", "emag_mpnn_set_params", "emag_mpnn_set_graph", "emag_mpnn_forward",
     "emag_mpnn_gradient", "emag_mpnn_train_step", "emag_mpnn_mfma_probe",
     "emag_problem_write", "emag_problem_read", "emag_problem_free", "emag_problem_last_error",
+    "emag_assembly_create", "emag_assembly_sizes", "emag_assembly_pattern", "emag_assemble",
+    "emag_assembly_last_error", "emag_assembly_destroy", "emag_csr_residual",
 ]
-HEADERS = ["emagddm_fetidp.h", "emagddm_mpnn.h", "emagddm_problem.h"]
+HEADERS = ["emagddm_fetidp.h", "emagddm_mpnn.h", "emagddm_problem.h", "emagddm_assembly.h"]
 
 
 class Problem(C.Structure):
@@ -144,6 +146,16 @@ def load():
     lib.emag_problem_free.restype = None
     lib.emag_problem_last_error.argtypes = []
     lib.emag_problem_last_error.restype = C.c_char_p
+    lib.emag_assembly_create.argtypes = [C.c_int64, C.c_int64, ip, C.c_int64, ip]
+    lib.emag_assembly_create.restype = P
+    lib.emag_assembly_sizes.argtypes = [P, lp, lp]
+    lib.emag_assembly_pattern.argtypes = [P, ip, ip, ip, C.POINTER(C.c_int8)]
+    lib.emag_assemble.argtypes = [P, P, P, P, P, P, P, P, P, P, P]
+    lib.emag_assembly_last_error.argtypes = [P]
+    lib.emag_assembly_last_error.restype = C.c_char_p
+    lib.emag_assembly_destroy.argtypes = [P]
+    lib.emag_assembly_destroy.restype = None
+    lib.emag_csr_residual.argtypes = [C.c_int64, P, P, P, P, P, P, dp, P]
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is C.c_int:  # default int return -> status

@@ -1,0 +1,204 @@
+// GPU global Nedelec assembly and FP64 residual operator behind include/emagddm_assembly.h.
+//
+// EMAssembly::assemble (src/numeric/em_assembly.cpp:144-411) scatters element matrices
+// into COO and builds a CSR on the host.  Here the host builds, once per mesh, the CSR
+// pattern and a gather plan (for every CSR entry the signed list of element entries that
+// sum into it, in a fixed order); the numeric part is two launches: k_integrate writes
+// the element matrices (structure of arrays) and k_assemble gathers each CSR value in one
+// thread with no atomics, so results are bitwise repeatable.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "emagddm_assembly.h"
+#include "emagddm_fetidp.h"
+#include "kernels.h"
+#include "plan.h"
+#include "topology.h"
+
+using namespace emag;
+
+namespace {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+struct DevArray {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+  void upload(const std::vector<T>& v) {
+    alloc(v.size());
+    if (!v.empty()) check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevArray() { release(); }
+};
+
+}  // namespace
+
+struct emag_assembly {
+  Topology topo;
+  CsrPlan csr;
+  NzPlan rhs;
+  std::string err;
+  DevArray<int32_t> t2n, csr_src, rhs_src;
+  DevArray<int64_t> csr_ptr, rhs_target, rhs_ptr;
+  DevArray<double> Ke, Me, Fe;  // element workspace: 21 + 21 + 6 doubles per tet
+  std::vector<int32_t> h_t2n;
+  bool on_device = false;
+
+  // plans and workspace to the current device, at the first numeric call (creating the
+  // handle touches no GPU)
+  void ensure_device() {
+    if (on_device) return;
+    const int64_t nt = topo.n_tets;
+    t2n.upload(h_t2n);
+    csr_ptr.upload(csr.ptr);
+    csr_src.upload(csr.src);
+    rhs_target.upload(rhs.target);
+    rhs_ptr.upload(rhs.ptr);
+    rhs_src.upload(rhs.src);
+    Ke.alloc(size_t(21 * nt));
+    Me.alloc(size_t(21 * nt));
+    Fe.alloc(size_t(6 * nt));
+    on_device = true;
+  }
+};
+
+extern "C" {
+
+emag_assembly_handle emag_assembly_create(int64_t n_nodes, int64_t n_tets, const int32_t* tet2node,
+                                          int64_t n_pec, const int32_t* pec_edge_nodes) {
+  if (n_nodes <= 0 || n_tets <= 0 || !tet2node || n_pec < 0 || (n_pec > 0 && !pec_edge_nodes))
+    return nullptr;
+  auto* h = new (std::nothrow) emag_assembly;
+  if (!h) return nullptr;
+  try {
+    // one subdomain holding every tet: the FETI-DP classes are not used here, only the
+    // edge ids, orientation signs and free numbering the topology builds
+    std::vector<int32_t> sub(size_t(n_tets), 0);
+    MeshInput in{n_nodes, n_tets, tet2node, sub.data(), n_pec, pec_edge_nodes, nullptr};
+    const std::string e = build_topology(in, h->topo);
+    if (!e.empty()) throw std::invalid_argument(e);
+    build_global_plan(h->topo, h->csr, h->rhs);
+    h->h_t2n.assign(tet2node, tet2node + 4 * n_tets);
+  } catch (const std::exception&) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int32_t emag_assembly_sizes(emag_assembly_handle h, int64_t* n_free, int64_t* nnz) {
+  if (!h) return FETIDP_INVALID_INPUT;
+  if (n_free) *n_free = h->topo.n_free;
+  if (nnz) *nnz = int64_t(h->csr.col.size());
+  return FETIDP_SUCCESS;
+}
+
+int32_t emag_assembly_pattern(emag_assembly_handle h, int32_t* row_ptr, int32_t* col,
+                              int32_t* tet_free, int8_t* tet_sign) {
+  if (!h) return FETIDP_INVALID_INPUT;
+  if (row_ptr) std::memcpy(row_ptr, h->csr.row_ptr.data(), sizeof(int32_t) * h->csr.row_ptr.size());
+  if (col) std::memcpy(col, h->csr.col.data(), sizeof(int32_t) * h->csr.col.size());
+  if (tet_free) std::memcpy(tet_free, h->topo.tet_free.data(), sizeof(int32_t) * h->topo.tet_free.size());
+  if (tet_sign) std::memcpy(tet_sign, h->topo.tet_sign.data(), h->topo.tet_sign.size());
+  return FETIDP_SUCCESS;
+}
+
+int32_t emag_assemble(emag_assembly_handle h, const double* d_xyz, const int32_t* d_tet_material,
+                      const double* d_nu, const double* d_kappa, const double* d_jq, double* d_K,
+                      double* d_M, double* d_A, double* d_f, void* stream) {
+  if (!h) return FETIDP_INVALID_INPUT;
+  h->err.clear();
+  if (!d_xyz || !d_tet_material || !d_nu || !d_kappa || (d_f && !d_jq)) {
+    h->err = "emag_assemble: null input (f needs jq)";
+    return FETIDP_INVALID_INPUT;
+  }
+  try {
+    h->ensure_device();
+  } catch (const std::exception& ex) {
+    h->err = ex.what();
+    return FETIDP_NUMERICAL_ERROR;
+  }
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t nt = h->topo.n_tets, nnz = int64_t(h->csr.col.size());
+  const double* jq = d_f ? d_jq : nullptr;
+  if (d_K || d_M) {  // K_e and M_e apart (mode 0)
+    launch_integrate(nt, d_xyz, h->t2n.p, d_tet_material, d_nu, d_kappa, jq, 0, h->Ke.p, h->Me.p,
+                     h->Fe.p, st);
+    if (d_K) launch_assemble_f64(nnz, nullptr, h->csr_ptr.p, h->csr_src.p, h->Ke.p, d_K, st);
+    if (d_M) launch_assemble_f64(nnz, nullptr, h->csr_ptr.p, h->csr_src.p, h->Me.p, d_M, st);
+    jq = nullptr;  // F_e is done
+  }
+  if (d_A || jq) {  // K_e + M_e in one pass (mode 1), as the FETI-DP factorization forms it
+    launch_integrate(nt, d_xyz, h->t2n.p, d_tet_material, d_nu, d_kappa, jq, 1, h->Ke.p, nullptr,
+                     h->Fe.p, st);
+    if (d_A) launch_assemble_f64(nnz, nullptr, h->csr_ptr.p, h->csr_src.p, h->Ke.p, d_A, st);
+  }
+  if (d_f)
+    launch_assemble_f64(h->rhs.size(), h->rhs_target.p, h->rhs_ptr.p, h->rhs_src.p, h->Fe.p, d_f, st);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    h->err = std::string("emag_assemble: ") + hipGetErrorString(e);
+    return FETIDP_NUMERICAL_ERROR;
+  }
+  return FETIDP_SUCCESS;
+}
+
+const char* emag_assembly_last_error(emag_assembly_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+void emag_assembly_destroy(emag_assembly_handle h) { delete h; }
+
+int32_t emag_csr_residual(int64_t n, const int32_t* d_row_ptr, const int32_t* d_col,
+                          const double* d_val, const double* d_x, const double* d_f, double* d_r,
+                          double* rel_norm, void* stream) {
+  if (n <= 0 || !d_row_ptr || !d_col || !d_val || !d_x || !d_f || !rel_norm) return FETIDP_INVALID_INPUT;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  try {
+    DevArray<double> r, partial;
+    double* rv = d_r;
+    if (!rv) {
+      r.alloc(size_t(n));
+      rv = r.p;
+    }
+    partial.alloc(2 * kDotBlocks);
+    launch_csr_residual(n, d_row_ptr, d_col, d_val, d_x, d_f, rv, st);
+    const double* a[2] = {rv, d_f};
+    launch_dots(n, 2, a, a, partial.p, st);
+    double h[2 * kDotBlocks];
+    check(hipMemcpyAsync(h, partial.p, sizeof h, hipMemcpyDeviceToHost, st), "residual copy");
+    check(hipStreamSynchronize(st), "residual sync");
+    double rr = 0.0, ff = 0.0;
+    for (int b = 0; b < kDotBlocks; ++b) {
+      rr += h[b];
+      ff += h[kDotBlocks + b];
+    }
+    *rel_norm = ff > 0.0 ? std::sqrt(rr / ff) : 0.0;
+  } catch (const std::exception&) {
+    return FETIDP_NUMERICAL_ERROR;
+  }
+  return FETIDP_SUCCESS;
+}
+
+}  // extern "C"

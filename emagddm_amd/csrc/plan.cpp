@@ -237,6 +237,50 @@ void build_faces(const Topology& T, Plans& P) {
 }
 }  // namespace
 
+// Global free-dof CSR (A = K + M, duplicates summed in a fixed order) and RHS plans:
+// every (tet, local entry) with both dofs free contributes s_a s_b elem to its entry.
+void build_global_plan(const Topology& T, CsrPlan& csr, NzPlan& rhs) {
+  const int64_t nt = T.n_tets, nf = T.n_free;
+  {
+    std::vector<uint64_t> tgt;
+    std::vector<int32_t> code;
+    tgt.reserve(size_t(nt) * 36);
+    code.reserve(size_t(nt) * 36);
+    for (int64_t t = 0; t < nt; ++t)
+      for (int a = 0; a < 6; ++a) {
+        const int32_t da = T.tet_free[6 * t + a];
+        if (da < 0) continue;
+        for (int b = 0; b < 6; ++b) {
+          const int32_t db = T.tet_free[6 * t + b];
+          if (db < 0) continue;
+          tgt.push_back(uint64_t(da) * uint64_t(nf) + uint64_t(db));
+          code.push_back(enc(int64_t(packed_index(a, b)) * nt + t, T.tet_sign[6 * t + a] * T.tet_sign[6 * t + b]));
+        }
+      }
+    std::vector<int64_t> keys;
+    group_emissions(tgt, code, uint64_t(nf) * uint64_t(nf), keys, csr.ptr, csr.src);
+    csr.row_ptr.assign(nf + 1, 0);
+    csr.col.resize(keys.size());
+    for (size_t i = 0; i < keys.size(); ++i) {
+      csr.row_ptr[keys[i] / nf + 1]++;
+      csr.col[i] = int32_t(keys[i] % nf);
+    }
+    for (int64_t r = 0; r < nf; ++r) csr.row_ptr[r + 1] += csr.row_ptr[r];
+    if (keys.size() >= (size_t(1) << 31)) throw std::runtime_error("global CSR too large for int32");
+
+    tgt.clear();
+    code.clear();
+    for (int64_t t = 0; t < nt; ++t)
+      for (int a = 0; a < 6; ++a) {
+        const int32_t da = T.tet_free[6 * t + a];
+        if (da < 0) continue;
+        tgt.push_back(uint64_t(da));
+        code.push_back(enc(int64_t(a) * nt + t, T.tet_sign[6 * t + a]));
+      }
+    group_emissions(tgt, code, uint64_t(nf), rhs.target, rhs.ptr, rhs.src);
+  }
+}
+
 void build_plans(const Topology& T, Plans& P) {
   P = Plans();
   const int64_t nt = T.n_tets, ns = T.n_sub, nf = T.n_free;
@@ -356,44 +400,7 @@ void build_plans(const Topology& T, Plans& P) {
   }
 
   // ---- global CSR and RHS ---------------------------------------------------
-  {
-    std::vector<uint64_t> tgt;
-    std::vector<int32_t> code;
-    tgt.reserve(size_t(nt) * 36);
-    code.reserve(size_t(nt) * 36);
-    for (int64_t t = 0; t < nt; ++t)
-      for (int a = 0; a < 6; ++a) {
-        const int32_t da = T.tet_free[6 * t + a];
-        if (da < 0) continue;
-        for (int b = 0; b < 6; ++b) {
-          const int32_t db = T.tet_free[6 * t + b];
-          if (db < 0) continue;
-          tgt.push_back(uint64_t(da) * uint64_t(nf) + uint64_t(db));
-          code.push_back(enc(int64_t(packed_index(a, b)) * nt + t, T.tet_sign[6 * t + a] * T.tet_sign[6 * t + b]));
-        }
-      }
-    std::vector<int64_t> keys;
-    group_emissions(tgt, code, uint64_t(nf) * uint64_t(nf), keys, P.csr.ptr, P.csr.src);
-    P.csr.row_ptr.assign(nf + 1, 0);
-    P.csr.col.resize(keys.size());
-    for (size_t i = 0; i < keys.size(); ++i) {
-      P.csr.row_ptr[keys[i] / nf + 1]++;
-      P.csr.col[i] = int32_t(keys[i] % nf);
-    }
-    for (int64_t r = 0; r < nf; ++r) P.csr.row_ptr[r + 1] += P.csr.row_ptr[r];
-    if (keys.size() >= (size_t(1) << 31)) throw std::runtime_error("global CSR too large for int32");
-
-    tgt.clear();
-    code.clear();
-    for (int64_t t = 0; t < nt; ++t)
-      for (int a = 0; a < 6; ++a) {
-        const int32_t da = T.tet_free[6 * t + a];
-        if (da < 0) continue;
-        tgt.push_back(uint64_t(da));
-        code.push_back(enc(int64_t(a) * nt + t, T.tet_sign[6 * t + a]));
-      }
-    group_emissions(tgt, code, uint64_t(nf), P.rhs_global.target, P.rhs_global.ptr, P.rhs_global.src);
-  }
+  build_global_plan(T, P.csr, P.rhs_global);
 
   // ---- coarse pool: [[S~, I], [I, 0]], padded, envelope-packed ------------------
   {

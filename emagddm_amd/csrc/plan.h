@@ -114,6 +114,8 @@ struct Plans {
 int packed_index(int a, int b);
 
 void build_plans(const Topology& T, Plans& P);
+// Only the global free-dof CSR and RHS plans (also part of build_plans).
+void build_global_plan(const Topology& T, CsrPlan& csr, NzPlan& rhs);
 
 // Left-looking schedule of one matrix of Ta tile rows: eliminate the first Tp
 // tile columns, then update the trailing (Ta-Tp)^2 block with the Schur term.
