@@ -129,12 +129,13 @@ def quad_points_bary(rule: str):
 
 
 def element_matrices_quadrature(X, nu, kappa, rule="4pt"):
-    """K_e, M_e by quadrature with the reference-element map (D2/D3 fixed)."""
+    """K_e, M_e by quadrature with the reference-element map (D2/D3 fixed).  `rule` is a
+    name of quad_points_bary or a (points (q,4) barycentric, weights (q,)) pair."""
     X = np.asarray(X, dtype=np.float64)
     det, _ = tet_geometry(X)
     J = np.stack([X[:, 0] - X[:, 3], X[:, 1] - X[:, 3], X[:, 2] - X[:, 3]], axis=2)
     JinvT = np.transpose(np.linalg.inv(J), (0, 2, 1))
-    P, W = quad_points_bary(rule)
+    P, W = quad_points_bary(rule) if isinstance(rule, str) else rule
     t = X.shape[0]
     K = np.zeros((t, 6, 6))
     M = np.zeros((t, 6, 6))

@@ -65,11 +65,18 @@ def test_element_closed_form_vs_quadrature(random_tets):
         Kq, Mq = nedelec.element_matrices_quadrature(X, nu, ka, rule)
         assert np.abs(K - Kq).max() <= 1e-12 * np.abs(K).max()
         assert np.abs(M - Mq).max() <= 1e-12 * np.abs(M).max()
-    # the reference's 3-point rule (D4) does not integrate M: total weight 1/8
-    P3 = np.array([[nedelec.QA, nedelec.QA, nedelec.QB], [nedelec.QA, nedelec.QB, nedelec.QA],
-                   [nedelec.QA, nedelec.QA, nedelec.QA]])
-    assert not np.isclose(3 / 24.0, 1 / 6.0)
-    assert P3.shape[0] == 3
+    # D4: the reference's "4-point" rule keeps 3 points of weight 1/24 (src/numeric/
+    # gauss_quadrature.cpp:200-219: (a,a,b), (a,b,a), then (b,a,a) replaced by (a,a,a)).
+    # It misintegrates M_e, even rescaled to the right total weight 1/6, while K_e (constant
+    # curls) only scales with the weight; the oracle's M_e is not that rule's.
+    a, b = nedelec.QA, nedelec.QB
+    xi3 = np.array([[a, a, b], [a, b, a], [a, a, a]])
+    P3 = np.hstack([xi3, 1.0 - xi3.sum(1, keepdims=True)])
+    K3, M3 = nedelec.element_matrices_quadrature(X, nu, ka, (P3, np.full(3, 1.0 / 24.0)))
+    assert np.abs(K3 * (4.0 / 3.0) - K).max() <= 1e-12 * np.abs(K).max()
+    rel = lambda A, B: np.abs(A - B).max() / np.abs(B).max()
+    assert rel(M3, M) > 0.1
+    assert rel(M3 * (4.0 / 3.0), M) > 1e-2
 
 
 def test_element_invariants(random_tets):
@@ -95,10 +102,45 @@ def test_element_rhs_brute_force(random_tets):
     F = nedelec.element_rhs(X, jq)
     Fd = nedelec.element_rhs_function(X, lin, "duffy6")
     assert np.abs(F - Fd).max() <= 1e-12 * np.abs(Fd).max()
-    # grad field J = grad(phi): F_e = G^T-consistent: F . (edge circulation) relation
-    # int W_i . grad(phi) for phi linear = -int div W_i phi + boundary; skip; use sign flip
     Fm = nedelec.element_rhs(X, -jq)
     assert np.allclose(Fm, -F)
+
+
+def _node_divergence(p, jq):
+    """(G^T f)_n = int grad(phi_n) . J over the mesh, for the interior nodes n."""
+    asm = assembly.assemble(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.pec_pairs, jq)
+    f, pairs, free = asm["f"], asm["pairs"], asm["free"]
+    # grad(phi_n) = sum_e G[e, n] W_e with the global edge e running low -> high node id
+    e = np.nonzero(free >= 0)[0]
+    lo, hi = pairs[e].min(1), pairs[e].max(1)
+    div = np.zeros(p.n_nodes)
+    np.add.at(div, hi, f[free[e]])
+    np.subtract.at(div, lo, f[free[e]])
+    x = p.xyz
+    interior = np.all((x > 1e-12) & (x < 1 - 1e-12), axis=1)
+    return div[interior], np.linalg.norm(f)
+
+
+def test_source_divergence_free():
+    """G^T F -> 0 for a divergence-free J (div J = 0 and phi_n = 0 on the boundary):
+    exactly for a linear J (the 4-point rule integrates grad(phi) . J exactly), and
+    vanishing under refinement for the Gaussian current loops."""
+    p = W.make_problem(6, 1, seed=3)
+    lin = lambda x: np.stack([x[:, 1] - 0.5, 0.5 - x[:, 0] + 0.3 * x[:, 2], 0.2 * x[:, 0]], 1)
+    pts = W.quadrature_points(p.xyz, p.tet2node)
+    jq = lin(pts.reshape(-1, 3)).reshape(pts.shape)
+    d, fn = _node_divergence(p, jq)
+    assert np.abs(d).max() <= 1e-13 * fn
+    # a J with divergence 1 is caught
+    jq2 = jq + np.concatenate([pts[..., :1], 0 * pts[..., 1:]], -1)
+    d2, fn2 = _node_divergence(p, jq2)
+    assert np.abs(d2).max() > 1e-3 * fn2
+    ratios = []
+    for n in (8, 16):
+        q = W.make_problem(n, 1, seed=5)
+        d, fn = _node_divergence(q, q.jq)
+        ratios.append(np.linalg.norm(d) / fn)
+    assert ratios[1] < 0.5 * ratios[0] and ratios[1] < 0.2
 
 
 def test_edge_ids_loop_equals_vectorised():
