@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -53,6 +54,33 @@ struct DevArray {
   }
   ~DevArray() { release(); }
 };
+
+// Workspace of emag_csr_residual_noalloc: the residual vector (when the caller passes
+// no d_r) followed by the 2 x kDotBlocks partial sums, in a file-scope device array that
+// every GPU the library runs on holds until the process ends (8 MiB + 4 KiB).  Host code
+// never uses the name `scratch` as an address: each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr int64_t kNoAllocMaxRows = int64_t(1) << 20;
+constexpr int64_t kScratchDoubles = kNoAllocMaxRows + 2 * kDotBlocks;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address (set up at the first call on that device).
+double* scratch_on_current_device() {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw HipError("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
 
 }  // namespace
 
@@ -188,6 +216,36 @@ int32_t emag_csr_residual(int64_t n, const int32_t* d_row_ptr, const int32_t* d_
     launch_dots(n, 2, a, a, partial.p, st);
     double h[2 * kDotBlocks];
     check(hipMemcpyAsync(h, partial.p, sizeof h, hipMemcpyDeviceToHost, st), "residual copy");
+    check(hipStreamSynchronize(st), "residual sync");
+    double rr = 0.0, ff = 0.0;
+    for (int b = 0; b < kDotBlocks; ++b) {
+      rr += h[b];
+      ff += h[kDotBlocks + b];
+    }
+    *rel_norm = ff > 0.0 ? std::sqrt(rr / ff) : 0.0;
+  } catch (const std::exception&) {
+    return FETIDP_NUMERICAL_ERROR;
+  }
+  return FETIDP_SUCCESS;
+}
+
+int32_t emag_csr_residual_noalloc(int64_t n, const int32_t* d_row_ptr, const int32_t* d_col,
+                                  const double* d_val, const double* d_x, const double* d_f,
+                                  double* d_r, double* rel_norm, void* stream) {
+  if (n <= 0 || !d_row_ptr || !d_col || !d_val || !d_x || !d_f || !rel_norm) return FETIDP_INVALID_INPUT;
+  if (!d_r && n > kNoAllocMaxRows) return FETIDP_INVALID_INPUT;  // residual would not fit
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  try {
+    double* ws = scratch_on_current_device();
+    double* partial = ws + kNoAllocMaxRows;
+    double* rv = d_r ? d_r : ws;
+    // the same launches and host sum as emag_csr_residual: bitwise the same result; both
+    // kernels overwrite every scratch element they read, so nothing needs clearing
+    launch_csr_residual(n, d_row_ptr, d_col, d_val, d_x, d_f, rv, st);
+    const double* a[2] = {rv, d_f};
+    launch_dots(n, 2, a, a, partial, st);
+    double h[2 * kDotBlocks];
+    check(hipMemcpyAsync(h, partial, sizeof h, hipMemcpyDeviceToHost, st), "residual copy");
     check(hipStreamSynchronize(st), "residual sync");
     double rr = 0.0, ff = 0.0;
     for (int b = 0; b < kDotBlocks; ++b) {

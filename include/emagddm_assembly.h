@@ -67,6 +67,18 @@ int32_t emag_csr_residual(int64_t n, const int32_t* d_row_ptr, const int32_t* d_
                           const double* d_val, const double* d_x, const double* d_f, double* d_r,
                           double* rel_norm, void* stream);
 
+/* emag_csr_residual with no allocation: no reference counterpart.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the workspace is a static device array of the
+ * library (8 MiB + 4 KiB on every GPU the library runs on, held until the process ends).
+ * d_r == NULL holds the residual in that array, so then n may be at most 1,048,576 rows
+ * (larger n: INVALID_INPUT, nothing launched); with d_r given, any n.  Same launches and
+ * bitwise the same result as emag_csr_residual.  All calls on one GPU share the array:
+ * calls that overlap on one GPU (other host threads, other streams) race. */
+int32_t emag_csr_residual_noalloc(int64_t n, const int32_t* d_row_ptr, const int32_t* d_col,
+                                  const double* d_val, const double* d_x, const double* d_f,
+                                  double* d_r, double* rel_norm, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
