@@ -23,7 +23,7 @@ class Config(C.Structure):
                 ("max_refinements", C.c_int32), ("verbose", C.c_int32),
                 ("inner_tolerance", C.c_double), ("graphs", C.c_int32), ("precond_precision", C.c_int32),
                 ("scaling", C.c_int32), ("side_stream", C.c_int32), ("tile_waves", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("coarse_solve", C.c_int32), ("coarse_group", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):

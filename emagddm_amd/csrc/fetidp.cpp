@@ -165,8 +165,26 @@ struct fetidp_solver {
   int coarse_tn = 0;         // its tile order, coarse_ncp / 64
   // uc = S~^-1 x from the factored coarse pool
   void coarse_apply(const double* x, double* y) {
-    launch_coarse_symv(topo.n_c, coarse_tn, int64_t(ctiles.n / 2), ctiles.p, cpool.p, coarse_rowabs.p, x,
-                       cpart.p, y, st);
+    if (!plans.coarse_grouped) {
+      launch_coarse_symv(topo.n_c, coarse_tn, int64_t(ctiles.n / 2), ctiles.p, cpool.p, coarse_rowabs.p, x,
+                         cpart.p, y, st);
+      return;
+    }
+    // S~^-1 x = L^-T L^-1 x: forward sweep over the groups, then backward
+    const std::vector<int32_t>& gs = plans.cg_start;
+    const int ng = int(gs.size()) - 1;
+    for (int g = 0; g < ng; ++g) {
+      const int64_t o = plans.cg_fwd_off[g];
+      launch_cg_forward(gs[g], gs[g + 1] - gs[g], cg_fwd_tiles_d.p + 2 * o, plans.cg_fwd_off[g + 1] - o,
+                        cpool.p, coarse_rowabs.p, cgX.p + plans.cg_xoff[g], x, topo.n_c, cg_fwd_rowptr_d.p,
+                        cg_pf.p, o, cg_tv.p, cg_yv.p, st);
+    }
+    for (int g = ng - 1; g >= 0; --g) {
+      const int64_t o = plans.cg_bwd_off[g];
+      launch_cg_backward(gs[g], gs[g + 1] - gs[g], cg_bwd_tiles_d.p + 2 * o, plans.cg_bwd_off[g + 1] - o,
+                         cpool.p, coarse_rowabs.p, cgX.p + plans.cg_xoff[g], cg_yv.p, plans.coarse_ncp,
+                         cg_bwd_colptr_d.p, cg_pb.p, o, cg_tv.p, cg_xv.p, y, topo.n_c, st);
+    }
   }
   DBuf<int> cfail;
   DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc, yvec, tbuf;
@@ -346,6 +364,22 @@ struct fetidp_solver {
     last_learned = true;
   }
 
+  // Coarse solve resolved from the configuration and n_c (0 = dense inverse, else the
+  // grouped solve's tile columns per group).  Dense up to kDenseCoarseMax: there the
+  // n_c^3 inverse costs little and one symmetric product per iteration is the fastest.
+  static constexpr int64_t kDenseCoarseMax = 12288;
+  int coarse_group_used = 0;
+  int coarse_group_tiles() const {
+    const bool grouped = topo.n_c > 0 && (cfg.coarse_solve == 2 ||
+                                          (cfg.coarse_solve == 0 && topo.n_c > kDenseCoarseMax));
+    return grouped ? (cfg.coarse_group > 0 ? cfg.coarse_group : 32) : 0;
+  }
+  // device data of the grouped coarse solve
+  DBuf<int32_t> cg_start_d, cg_group_of_d, cg_fwd_tiles_d, cg_fwd_rowptr_d, cg_bwd_tiles_d,
+      cg_bwd_colptr_d, cg_inv_items_d;
+  DBuf<int64_t> cg_xoff_d;
+  DBuf<double> cgX, cg_pf, cg_pb, cg_yv, cg_xv, cg_tv;
+
   bool fell_back = false;  // FP32 requested but the factor fell back to FP64
   bool stalled = false;    // the last FP32 outer refinement stalled (solve hands over to FP64)
   std::string fallback_reason;
@@ -377,7 +411,8 @@ struct fetidp_solver {
     MeshInput in{n_nodes, n_tets, h_t2n, h_sub, n_pec, h_pec, h_xyz};
     std::string e = build_topology(in, topo);
     if (!e.empty()) throw InputError("setup: " + e);
-    build_plans(topo, plans);
+    coarse_group_used = coarse_group_tiles();
+    build_plans(topo, plans, coarse_group_used);
     n_mat = nmat;
     h_xyz_.assign(h_xyz, h_xyz + 3 * n_nodes);
     h_min = 1e300;
@@ -533,7 +568,29 @@ struct fetidp_solver {
     cpool.alloc(plans.coarse_pool_elems);
     cinv.alloc(plans.coarse_inv_elems);
     coarse_tn = int(plans.coarse_ncp / kTile);
-    {
+    if (plans.coarse_grouped) {
+      cg_start_d.upload(plans.cg_start);
+      cg_group_of_d.upload(plans.cg_group_of);
+      cg_fwd_tiles_d.upload(plans.cg_fwd_tiles);
+      cg_fwd_rowptr_d.upload(plans.cg_fwd_rowptr);
+      cg_bwd_tiles_d.upload(plans.cg_bwd_tiles);
+      cg_bwd_colptr_d.upload(plans.cg_bwd_colptr);
+      cg_inv_items_d.upload(plans.cg_inv_items);
+      cg_xoff_d.upload(plans.cg_xoff);
+      cgX.alloc(size_t(plans.cg_x_elems));
+      cg_pf.alloc(size_t(std::max<int64_t>(plans.cg_fwd_off.back(), 1)) * kTile);
+      cg_pb.alloc(size_t(std::max<int64_t>(plans.cg_bwd_off.back(), 1)) * kTile);
+      cg_yv.alloc(size_t(plans.coarse_ncp));
+      cg_xv.alloc(size_t(plans.coarse_ncp));
+      cg_tv.alloc(size_t(plans.coarse_ncp));
+      ctiles.release();
+      cpart.release();
+    } else {
+      cgX.release();
+      cg_pf.release();
+      cg_pb.release();
+    }
+    if (!plans.coarse_grouped) {
       std::vector<int32_t> ct;
       ct.reserve(size_t(coarse_tn) * size_t(coarse_tn + 1));
       for (int I = 0; I < coarse_tn; ++I)
@@ -543,7 +600,8 @@ struct fetidp_solver {
         }
       ctiles.upload(ct);
     }
-    cpart.alloc(size_t(std::max(coarse_tn, 1)) * size_t(std::max(coarse_tn, 1)) * kTile);
+    if (!plans.coarse_grouped)
+      cpart.alloc(size_t(std::max(coarse_tn, 1)) * size_t(std::max(coarse_tn, 1)) * kTile);
     HIPCHECK(hipStreamSynchronize(st));
     device_ready = true;
     upload_ms = now_ms() - t0;
@@ -858,6 +916,11 @@ struct fetidp_solver {
                           pool.p, (sizeof(T) == 8 ? tw64.p : tw32.p), cpool.p, st);
     cfail.zero(st);
     run_schedule<double>(plans.coarse_sched, coarse_items, coarse_rowbase, coarse_rowabs, coarse_inv_base, cpool.p, cinv.p, cfail.p, false);
+    if (plans.coarse_grouped)  // explicit inverses of the groups' diagonal blocks
+      timed(8, [&] {
+        launch_cg_invert(plans.coarse_group, plans.cg_inv_step.data(), cg_inv_items_d.p, cpool.p, coarse_rowabs.p, coarse_ftile.p, cinv.p, cg_start_d.p,
+                         cg_group_of_d.p, cg_xoff_d.p, cgX.p, st);
+      });
     if (deluxe) deluxe_setup();
     if (!profiling()) HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
     HIPCHECK(hipStreamSynchronize(st));
@@ -1539,6 +1602,8 @@ static bool normalized_config(const fetidp_config* in, fetidp_config& out) {
   if (!(out.inner_tolerance > 0)) out.inner_tolerance = 1e-2;
   if (out.tile_waves != 0) out.tile_waves = 4;
   if (out.side_stream < 0 || out.side_stream > 2) out.side_stream = 2;
+  if (out.coarse_solve < 0 || out.coarse_solve > 2) return false;
+  if (out.coarse_group < 0 || out.coarse_group > 64) return false;
   return true;
 }
 
@@ -1563,6 +1628,15 @@ int32_t fetidp_set_config(fetidp_handle h, const fetidp_config* cfg) {
         c.precond_precision != h->cfg.precond_precision)
       h->factored = false;
     h->cfg = c;
+    if (h->setup_done && h->coarse_group_tiles() != h->coarse_group_used) {
+      // another coarse solve: new coarse plan; the device data follow at the next factor
+      h->coarse_group_used = h->coarse_group_tiles();
+      build_coarse_plan(h->topo, h->plans, h->coarse_group_used);
+      h->factored = false;
+      h->device_ready = false;
+      if (h->pcpg_exec) HIPCHECK(hipGraphExecDestroy(h->pcpg_exec));
+      h->pcpg_exec = nullptr;
+    }
     return int32_t(FETIDP_SUCCESS);
   });
 }

@@ -68,6 +68,28 @@ void launch_extract_W(int64_t ntiles, const SymTile* tiles, int ns, int max_ncp,
 void launch_coarse_symv(int64_t nc, int tn, int64_t ntiles, const int32_t* tiles, const double* cpool,
                         const int64_t* cra, const double* x, double* part, double* y, hipStream_t st);
 
+// Grouped coarse solve (S~ = L L^T, L in the coarse pool; groups of tile columns
+// [a, a + w) with X_g = (L_gg)^-1 dense column-major of order 64 w):
+//  * launch_cg_invert forms every X_g after the factorization (one launch per distance
+//    from the diagonal, max_group launches);
+//  * launch_cg_forward: y_g = X_g (rhs_g - L_{g, <a} y), the panel tiles of group g give
+//    partial slots [slot0, slot0 + npanel) of part, rowptr maps tile rows to their slots;
+//  * launch_cg_backward: x_g = X_g^T (y_g - L_{>g, g}^T x), colptr maps tile columns to
+//    slots; out (length nc, may be null) receives x too.
+// y, x and the group right-hand sides t have ncp entries; rhs has nc.
+void launch_cg_invert(int max_group, const int64_t* step, const int32_t* items, const double* cpool,
+                      const int64_t* cra, const int32_t* cft, const double* cinv,
+                      const int32_t* cg_start, const int32_t* cg_group_of, const int64_t* xoff,
+                      double* X, hipStream_t st);
+void launch_cg_forward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
+                       const int64_t* cra, const double* Xg, const double* rhs, int64_t nc,
+                       const int32_t* rowptr, double* part, int64_t slot0, double* t, double* y,
+                       hipStream_t st);
+void launch_cg_backward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
+                        const int64_t* cra, const double* Xg, const double* y, int64_t ncp,
+                        const int32_t* colptr, double* part, int64_t slot0, double* t, double* x,
+                        double* out, int64_t nc, hipStream_t st);
+
 // Forward solve, one launch per block column: yv_r = L_rr^-1 vec_r; afterwards
 // vec_c = f_c - L_cr y and vec_D' = -L_D'r y (= -E K_rr^-1 f_r).
 template <typename T>

@@ -1058,6 +1058,337 @@ void launch_coarse_symv(int64_t nc, int tn, int64_t ntiles, const int32_t* tiles
 }
 
 // ---------------------------------------------------------------------------
+// Grouped coarse solve: S~ = L L^T with L in the envelope-packed coarse pool (tiles
+// column-major), groups of tile columns g = [a, a + w), X_g = (L_gg)^-1 dense
+// column-major of order n = 64 w (element (r, c) at c * n + r).
+// ---------------------------------------------------------------------------
+
+// X_g tile (i, j), i >= j: X_ii = L_ii^-1, X_ij = -L_ii^-1 sum_{k = max(j, f_i)}^{i-1} L_ik X_kj.
+// One work-group per tile; a launch holds the tiles of one distance d = i - j, so every
+// X_kj it reads (k - j < d) is complete.  Each thread owns a 4x4 block of the output.
+__global__ __launch_bounds__(256) void k_cg_inv_step(const int32_t* __restrict__ items,
+                                                     const double* __restrict__ cpool,
+                                                     const int64_t* __restrict__ cra,
+                                                     const int32_t* __restrict__ cft,
+                                                     const double* __restrict__ cinv,
+                                                     const int32_t* __restrict__ cg_start,
+                                                     const int32_t* __restrict__ cg_group_of,
+                                                     const int64_t* __restrict__ xoff,
+                                                     double* __restrict__ X) {
+  // products in two halves of the inner dimension: sA holds 64 x 32 of the left factor,
+  // sB 32 x 64 of the right one (or all of M = the sum, 64 x 64, for the last product)
+  constexpr int kH = 33;
+  __shared__ double sA[64 * kH], sB[64 * kLd];
+  const int i = items[2 * blockIdx.x], j = items[2 * blockIdx.x + 1];
+  const int g = cg_group_of[i], a = cg_start[g];
+  const int64_t n = int64_t(cg_start[g + 1] - a) * 64;
+  double* Xg = X + xoff[g];
+  const int tid = threadIdx.x;
+  const double* Li = cinv + (int64_t)i * kTileElems;
+  double* Xij = Xg + (int64_t)(j - a) * 64 * n + (int64_t)(i - a) * 64;
+  if (i == j) {
+    for (int e = tid; e < kTileElems; e += 256) Xij[(int64_t)(e >> 6) * n + (e & 63)] = Li[e];
+    return;
+  }
+  const int r0 = (tid & 15) * 4, c0 = (tid >> 4) * 4;
+  double acc[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) acc[q][p] = 0.0;
+  const int k0 = max(j, cft[i]);
+  for (int k = k0; k < i; ++k) {
+    const double* Lk = cpool + cra[i] + (int64_t)k * kTileElems;
+    const double* Xk = Xg + (int64_t)(j - a) * 64 * n + (int64_t)(k - a) * 64;
+    for (int h = 0; h < 2; ++h) {
+      for (int e = tid; e < 64 * 32; e += 256) {
+        const int r = e & 63, c = e >> 6;  // row r, column 32h + c of L_ik
+        sA[r * kH + c] = Lk[(32 * h + c) * 64 + r];
+        const int t = e & 31, col = e >> 5;  // row 32h + t, column col of X_kj
+        sB[t * kLd + col] = Xk[(int64_t)col * n + 32 * h + t];
+      }
+      __syncthreads();
+      for (int t = 0; t < 32; ++t) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          av[q] = sA[(r0 + q) * kH + t];
+          bv[q] = sB[t * kLd + c0 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int p = 0; p < 4; ++p) acc[q][p] = fma(av[q], bv[p], acc[q][p]);
+      }
+      __syncthreads();
+    }
+  }
+  // X_ij = -L_ii^-1 M with M = acc (all of M in sB, L_ii^-1 in two halves in sA)
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) sB[(r0 + q) * kLd + c0 + p] = acc[q][p];
+  double out[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) out[q][p] = 0.0;
+  for (int h = 0; h < 2; ++h) {
+    for (int e = tid; e < 64 * 32; e += 256) {
+      const int r = e & 63, c = e >> 6;
+      sA[r * kH + c] = Li[(32 * h + c) * 64 + r];
+    }
+    __syncthreads();
+    for (int t = 0; t < 32; ++t) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        av[q] = sA[(r0 + q) * kH + t];
+        bv[q] = sB[(32 * h + t) * kLd + c0 + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) out[q][p] = fma(av[q], bv[p], out[q][p]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Xij[(int64_t)(c0 + p) * n + r0 + q] = -out[q][p];
+}
+
+void launch_cg_invert(int max_group, const int64_t* step, const int32_t* items, const double* cpool,
+                      const int64_t* cra, const int32_t* cft, const double* cinv,
+                      const int32_t* cg_start, const int32_t* cg_group_of, const int64_t* xoff,
+                      double* X, hipStream_t st) {
+  for (int d = 0; d < max_group; ++d) {
+    const int64_t n = step[d + 1] - step[d];
+    if (n > 0)
+      hipLaunchKernelGGL(k_cg_inv_step, dim3(n), dim3(256), 0, st, items + 2 * step[d], cpool, cra, cft,
+                         cinv, cg_start, cg_group_of, xoff, X);
+  }
+}
+
+// Panel products of the sweeps, one work-group per tile (I, J) of L, partial 64-vector
+// slot = work-group index: forward (TRANS = false) part = L_IJ v_J, backward (TRANS =
+// true) part = L_IJ^T v_I.  Lane = tile row, wave w holds columns 16w .. 16w+15 (every
+// load one coalesced 512-byte column); row sums through LDS, column sums by the halving
+// butterfly of k_coarse_symv.
+template <bool TRANS>
+__global__ __launch_bounds__(256) void k_cg_panel(const int32_t* __restrict__ tiles,
+                                                  const double* __restrict__ cpool,
+                                                  const int64_t* __restrict__ cra,
+                                                  const double* __restrict__ v,
+                                                  double* __restrict__ part) {
+  __shared__ double xs[64];
+  __shared__ double colp[4][64];
+  const int I = tiles[2 * blockIdx.x], J = tiles[2 * blockIdx.x + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* A = cpool + cra[I] + (int64_t)J * kTileElems;
+  double at[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) at[q] = A[(16 * wv + q) * 64 + lane];
+  double* out = part + (int64_t)blockIdx.x * 64;
+  if constexpr (!TRANS) {
+    if (tid < 64) xs[tid] = v[(int64_t)J * 64 + tid];
+    __syncthreads();
+    double rs = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) rs = fma(at[q], xs[16 * wv + q], rs);
+    colp[wv][lane] = rs;
+    __syncthreads();
+    if (tid < 64) out[tid] = (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
+  } else {
+    const double xr = v[(int64_t)I * 64 + lane];
+    double rv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) rv[q] = at[q] * xr;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const bool hi = lane & 32;
+      const double keep = hi ? rv[i + 8] : rv[i], send = hi ? rv[i] : rv[i + 8];
+      rv[i] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool hi = lane & 16;
+      const double keep = hi ? rv[i + 4] : rv[i], send = hi ? rv[i] : rv[i + 4];
+      rv[i] = keep + __shfl_xor(send, 16);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const bool hi = lane & 8;
+      const double keep = hi ? rv[i + 2] : rv[i], send = hi ? rv[i] : rv[i + 2];
+      rv[i] = keep + __shfl_xor(send, 8);
+    }
+    {
+      const bool hi = lane & 4;
+      const double keep = hi ? rv[1] : rv[0], send = hi ? rv[0] : rv[1];
+      rv[0] = keep + __shfl_xor(send, 4);
+    }
+    double cs = rv[0];
+    cs += __shfl_xor(cs, 2);
+    cs += __shfl_xor(cs, 1);
+    if ((lane & 3) == 0) {
+      const int col = 16 * wv + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                      ((lane >> 2) & 1);
+      out[col] = cs;
+    }
+  }
+}
+
+// Right-hand side of one group step: for tile row (column) J = a + blockIdx.x of the
+// group, t_J = base_J - sum of its partial slots [ptr[J], ptr[J+1]) (base entries past
+// nb read as zero).  Four threads per entry each sum every fourth slot; the four sums are
+// added in a fixed order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_cg_reduce(int a, const double* __restrict__ base, int64_t nb,
+                                                   const int32_t* __restrict__ ptr,
+                                                   const double* __restrict__ part,
+                                                   double* __restrict__ t) {
+  __shared__ double red[4][64];
+  const int J = a + blockIdx.x, e = threadIdx.x & 63, q = threadIdx.x >> 6;
+  double s0 = 0.0, s1 = 0.0;
+  int k = ptr[J] + q;
+  const int k1 = ptr[J + 1];
+  for (; k + 4 < k1; k += 8) {
+    s0 += part[(int64_t)k * 64 + e];
+    s1 += part[(int64_t)(k + 4) * 64 + e];
+  }
+  if (k < k1) s0 += part[(int64_t)k * 64 + e];
+  red[q][e] = s0 + s1;
+  __syncthreads();
+  if (q == 0) {
+    const int64_t gi = int64_t(J) * 64 + e;
+    const double b = gi < nb ? base[gi] : 0.0;
+    t[gi] = b - ((red[0][e] + red[1][e]) + (red[2][e] + red[3][e]));
+  }
+}
+
+// Forward group step: work-group b computes y on tile row a + b of group [a, a + w):
+//   y_{a+b} = sum_{J <= a+b} X_{a+b, J} t_J   (rows of X_g: lane = row, the 16 waves
+// split the columns, eight independent loads in flight per thread; the 16 wave sums are
+// added in a fixed order).
+__global__ __launch_bounds__(1024) void k_cg_fwd(int a, int64_t n, const double* __restrict__ Xg,
+                                                 const double* __restrict__ tv,
+                                                 double* __restrict__ y) {
+  extern __shared__ double t[];  // 64 * (b + 1) entries
+  __shared__ double red[16][64];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int ne = 64 * (b + 1);
+  for (int e = tid; e < ne; e += 1024) t[e] = tv[(int64_t)a * 64 + e];
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6;
+  const double* Xr = Xg + (int64_t)b * 64 + lane;
+  double acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+  int c = wv;
+  for (; c + 7 * 16 < ne; c += 8 * 16) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = fma(Xr[(int64_t)(c + 16 * u) * n], t[c + 16 * u], acc[u]);
+  }
+  for (; c < ne; c += 16) acc[0] = fma(Xr[(int64_t)c * n], t[c], acc[0]);
+  red[wv][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  __syncthreads();
+  if (tid < 64) {
+    double s = 0.0;
+    for (int k = 0; k < 16; ++k) s += red[k][tid];
+    y[(int64_t)(a + b) * 64 + tid] = s;
+  }
+}
+
+// Backward group step: work-group b computes x on tile column a + b of group [a, a + w):
+//   x_{a+b} = sum_{I >= a+b} X_{I, a+b}^T t_I   (column dot products of X_g).
+// Wave w owns output columns 16 (w % 4) .. +15 and the rows of phase w / 4 (lanes stride
+// the rows by 256), 16 independent loads in flight per thread; butterfly column sums, then
+// the four row phases added in a fixed order.
+__global__ __launch_bounds__(1024) void k_cg_bwd(int a, int w, int64_t n, const double* __restrict__ Xg,
+                                                 const double* __restrict__ tv,
+                                                 double* __restrict__ x, double* __restrict__ out,
+                                                 int64_t nc) {
+  extern __shared__ double t[];  // 64 * (w - b) entries
+  __shared__ double csum[4][64];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int ne = 64 * (w - b);
+  for (int e = tid; e < ne; e += 1024) t[e] = tv[(int64_t)(a + b) * 64 + e];
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6, cg = wv & 3, rp = wv >> 2;
+  double rv[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) rv[q] = 0.0;
+  const double* Xc = Xg + (int64_t)(b * 64 + 16 * cg) * n + (int64_t)b * 64;  // rows from 64 b
+  for (int r = lane + 64 * rp; r < ne; r += 256) {
+    const double tv_r = t[r];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) rv[q] = fma(Xc[(int64_t)q * n + r], tv_r, rv[q]);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const bool hi = lane & 32;
+    const double keep = hi ? rv[i + 8] : rv[i], send = hi ? rv[i] : rv[i + 8];
+    rv[i] = keep + __shfl_xor(send, 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool hi = lane & 16;
+    const double keep = hi ? rv[i + 4] : rv[i], send = hi ? rv[i] : rv[i + 4];
+    rv[i] = keep + __shfl_xor(send, 16);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const bool hi = lane & 8;
+    const double keep = hi ? rv[i + 2] : rv[i], send = hi ? rv[i] : rv[i + 2];
+    rv[i] = keep + __shfl_xor(send, 8);
+  }
+  {
+    const bool hi = lane & 4;
+    const double keep = hi ? rv[1] : rv[0], send = hi ? rv[0] : rv[1];
+    rv[0] = keep + __shfl_xor(send, 4);
+  }
+  double cs = rv[0];
+  cs += __shfl_xor(cs, 2);
+  cs += __shfl_xor(cs, 1);
+  if ((lane & 3) == 0) {
+    const int col = 16 * cg + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                    ((lane >> 2) & 1);
+    csum[rp][col] = cs;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const double v = (csum[0][tid] + csum[1][tid]) + (csum[2][tid] + csum[3][tid]);
+    const int64_t gi = int64_t(a + b) * 64 + tid;
+    x[gi] = v;
+    if (out && gi < nc) out[gi] = v;
+  }
+}
+
+void launch_cg_forward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
+                       const int64_t* cra, const double* Xg, const double* rhs, int64_t nc,
+                       const int32_t* rowptr, double* part, int64_t slot0, double* t, double* y,
+                       hipStream_t st) {
+  if (npanel > 0)
+    hipLaunchKernelGGL(k_cg_panel<false>, dim3(npanel), dim3(256), 0, st, panel_tiles, cpool, cra, y,
+                       part + slot0 * 64);
+  hipLaunchKernelGGL(k_cg_reduce, dim3(w), dim3(256), 0, st, a, rhs, nc, rowptr, part, t);
+  hipLaunchKernelGGL(k_cg_fwd, dim3(w), dim3(1024), sizeof(double) * 64 * w, st, a, int64_t(64) * w, Xg,
+                     t, y);
+}
+
+void launch_cg_backward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
+                        const int64_t* cra, const double* Xg, const double* y, int64_t ncp,
+                        const int32_t* colptr, double* part, int64_t slot0, double* t, double* x,
+                        double* out, int64_t nc, hipStream_t st) {
+  if (npanel > 0)
+    hipLaunchKernelGGL(k_cg_panel<true>, dim3(npanel), dim3(256), 0, st, panel_tiles, cpool, cra, x,
+                       part + slot0 * 64);
+  hipLaunchKernelGGL(k_cg_reduce, dim3(w), dim3(256), 0, st, a, y, ncp, colptr, part, t);
+  hipLaunchKernelGGL(k_cg_bwd, dim3(w), dim3(1024), sizeof(double) * 64 * w, st, a, w, int64_t(64) * w, Xg,
+                     t, x, out, nc);
+}
+
+// ---------------------------------------------------------------------------
 // Triangular solves with the stored partial factor: one launch per block column,
 // one workgroup per (row tile, subdomain); diagonal blocks use the stored inverse.
 // ---------------------------------------------------------------------------

@@ -281,7 +281,167 @@ void build_global_plan(const Topology& T, CsrPlan& csr, NzPlan& rhs) {
   }
 }
 
-void build_plans(const Topology& T, Plans& P) {
+// Grouped coarse solve: S~ = L L^T (envelope); L^-1 v by a forward sweep over groups of
+// tile columns g = [a, b): t_g = v_g - L_{g, <a} y (panel products), y_g = X_g t_g with
+// X_g = (L_gg)^-1 explicit; L^-T y by the backward sweep with the panels below the groups
+// and X_g^T.
+static void build_coarse_groups(Plans& P, int G) {
+  const int tn = int(P.coarse_ncp / kTile);
+  const int32_t* ft = P.coarse_ftile.data();
+  P.coarse_grouped = true;
+  P.coarse_group = G;
+  P.cg_start.clear();
+  for (int a = 0; a < tn; a += G) P.cg_start.push_back(a);
+  P.cg_start.push_back(tn);
+  const int ng = int(P.cg_start.size()) - 1;
+  P.cg_xoff.assign(size_t(ng), 0);
+  int64_t xo = 0;
+  for (int g = 0; g < ng; ++g) {
+    P.cg_xoff[g] = xo;
+    const int64_t n = int64_t(P.cg_start[g + 1] - P.cg_start[g]) * kTile;
+    xo += n * n;
+  }
+  P.cg_x_elems = std::max<int64_t>(xo, 1);
+  P.cg_group_of.assign(size_t(tn), 0);
+  for (int g = 0; g < ng; ++g)
+    for (int i = P.cg_start[g]; i < P.cg_start[g + 1]; ++i) P.cg_group_of[i] = g;
+  // forward panels, ordered by group, row I, column J: the slots of row I are contiguous
+  P.cg_fwd_tiles.clear();
+  P.cg_fwd_off.assign(size_t(ng) + 1, 0);
+  P.cg_fwd_rowptr.assign(size_t(tn) + 1, 0);
+  for (int g = 0; g < ng; ++g) {
+    P.cg_fwd_off[g] = int64_t(P.cg_fwd_tiles.size() / 2);
+    for (int I = P.cg_start[g]; I < P.cg_start[g + 1]; ++I) {
+      P.cg_fwd_rowptr[I] = int32_t(P.cg_fwd_tiles.size() / 2);
+      for (int J = ft[I]; J < P.cg_start[g]; ++J) {
+        P.cg_fwd_tiles.push_back(I);
+        P.cg_fwd_tiles.push_back(J);
+      }
+    }
+  }
+  P.cg_fwd_off[ng] = int64_t(P.cg_fwd_tiles.size() / 2);
+  P.cg_fwd_rowptr[tn] = int32_t(P.cg_fwd_tiles.size() / 2);
+  // backward panels, ordered by group, column J, row I: the slots of column J are contiguous
+  P.cg_bwd_tiles.clear();
+  P.cg_bwd_off.assign(size_t(ng) + 1, 0);
+  P.cg_bwd_colptr.assign(size_t(tn) + 1, 0);
+  for (int g = 0; g < ng; ++g) {
+    P.cg_bwd_off[g] = int64_t(P.cg_bwd_tiles.size() / 2);
+    for (int J = P.cg_start[g]; J < P.cg_start[g + 1]; ++J) {
+      P.cg_bwd_colptr[J] = int32_t(P.cg_bwd_tiles.size() / 2);
+      for (int I = P.cg_start[g + 1]; I < tn; ++I)
+        if (ft[I] <= J) {
+          P.cg_bwd_tiles.push_back(I);
+          P.cg_bwd_tiles.push_back(J);
+        }
+    }
+  }
+  P.cg_bwd_off[ng] = int64_t(P.cg_bwd_tiles.size() / 2);
+  P.cg_bwd_colptr[tn] = int32_t(P.cg_bwd_tiles.size() / 2);
+  // items forming X_g by distance d = i - j from the diagonal (d = 0: X_ii = L_ii^-1)
+  P.cg_inv_items.clear();
+  P.cg_inv_step.assign(size_t(G) + 1, 0);
+  for (int d = 0; d < G; ++d) {
+    P.cg_inv_step[d] = int64_t(P.cg_inv_items.size() / 2);
+    for (int g = 0; g < ng; ++g)
+      for (int j = P.cg_start[g]; j + d < P.cg_start[g + 1]; ++j) {
+        P.cg_inv_items.push_back(j + d);
+        P.cg_inv_items.push_back(j);
+      }
+  }
+  P.cg_inv_step[G] = int64_t(P.cg_inv_items.size() / 2);
+}
+
+// Coarse pool, coarse assembly plan and coarse schedule.  group_tiles = 0: the pool is
+// [[S~, I], [I, 0]] and its partial factorization leaves -S~^-1 (dense) as lower tiles;
+// group_tiles > 0: the pool is S~ alone, factored completely, and the solve sweeps over
+// groups of group_tiles tile columns (build_coarse_groups).
+void build_coarse_plan(const Topology& T, Plans& P, int group_tiles) {
+  const int64_t ns = T.n_sub;
+  P.coarse = PoolPlan();
+  P.coarse_grouped = false;
+  P.coarse_group = 0;
+  {
+    const int64_t nc = T.n_c;
+    const int64_t ncp = round_up(std::max<int64_t>(nc, 1), kTile);
+    P.coarse_ncp = ncp;
+    const bool grouped = group_tiles > 0;
+    const int64_t Tc = (grouped ? 1 : 2) * ncp / kTile;
+    std::vector<int64_t> cfrow(size_t(Tc * kTile));
+    for (int64_t r = 0; r < Tc * kTile; ++r) cfrow[r] = r;
+    if (!grouped)
+      for (int64_t k = 0; k < nc; ++k) cfrow[ncp + k] = k;
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& L = T.sub[s];
+      for (int ka = 0; ka < L.nc; ++ka) {
+        const int64_t Ca = T.c_loc[L.cvec_off + ka];
+        for (int kb = 0; kb <= ka; ++kb) {
+          const int64_t Cb = T.c_loc[L.cvec_off + kb];
+          if (Ca < Cb) throw std::runtime_error("primal order not monotone");
+          cfrow[Ca] = std::min(cfrow[Ca], Cb);
+        }
+      }
+    }
+    P.coarse_ftile.assign(Tc, 0);
+    P.coarse_rowabs.assign(Tc, 0);
+    int64_t off = 0;
+    for (int64_t I = 0; I < Tc; ++I) {
+      int64_t f = I;
+      for (int64_t r = I * kTile; r < (I + 1) * kTile; ++r) f = std::min(f, cfrow[r] / kTile);
+      P.coarse_ftile[I] = int32_t(f);
+      P.coarse_rowabs[I] = off - f * kTileElems;
+      off += (I - f + 1) * kTileElems;
+    }
+    P.coarse_pool_elems = off;
+    const int64_t* cra = P.coarse_rowabs.data();
+    std::vector<uint64_t> tgt;
+    std::vector<int64_t> srcpos;
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& L = T.sub[s];
+      const int64_t* ra = &P.rowabs[P.ftile_off[s]];
+      for (int ka = 0; ka < L.nc; ++ka) {
+        const int64_t Ca = T.c_loc[L.cvec_off + ka];
+        for (int kb = 0; kb <= ka; ++kb) {
+          const int64_t Cb = T.c_loc[L.cvec_off + kb];
+          const int64_t sp = env_pos(ra, L.o_c + ka, L.o_c + kb);
+          tgt.push_back(uint64_t(env_pos(cra, Ca, Cb)));
+          srcpos.push_back(sp);
+          if (Ca != Cb && Ca / kTile == Cb / kTile) {
+            tgt.push_back(uint64_t(cra[Ca / kTile] + (Ca / kTile) * kTileElems + (Ca % kTile) * kTile + Cb % kTile));
+            srcpos.push_back(sp);
+          }
+        }
+      }
+    }
+    for (int64_t r = nc; r < ncp; ++r) {
+      tgt.push_back(uint64_t(env_pos(cra, r, r)));
+      srcpos.push_back(-1);
+    }
+    if (!grouped)
+      for (int64_t k = 0; k < nc; ++k) {
+        tgt.push_back(uint64_t(env_pos(cra, ncp + k, k)));
+        srcpos.push_back(-1);
+      }
+    std::vector<uint32_t> idx(tgt.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = uint32_t(i);
+    radix_sort_pairs(tgt, idx, uint64_t(P.coarse_pool_elems));
+    for (size_t i = 0; i < tgt.size(); ++i) {
+      if (i == 0 || tgt[i] != tgt[i - 1]) {
+        P.coarse.target.push_back(int64_t(tgt[i]));
+        P.coarse.ptr.push_back(int64_t(i));
+      }
+      P.coarse.srcpos.push_back(srcpos[idx[i]]);
+    }
+    P.coarse.ptr.push_back(int64_t(tgt.size()));
+    // coarse schedule (single matrix, eliminate the first ncp/64 tile columns: the S~
+    // block, which is all of a grouped pool)
+    P.coarse_sched = single_matrix_schedule(int(ncp / kTile), int(Tc), true, P.coarse_ftile.data());
+    if (grouped) build_coarse_groups(P, group_tiles);
+  }
+  P.coarse_inv_elems = P.coarse_ncp * kTile;  // (ncp / 64) tiles of 4096
+}
+
+void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
   P = Plans();
   const int64_t nt = T.n_tets, ns = T.n_sub, nf = T.n_free;
   const int64_t one_code = enc(21 * nt, 1);  // trailing constant 1.0 of the element array
@@ -402,79 +562,8 @@ void build_plans(const Topology& T, Plans& P) {
   // ---- global CSR and RHS ---------------------------------------------------
   build_global_plan(T, P.csr, P.rhs_global);
 
-  // ---- coarse pool: [[S~, I], [I, 0]], padded, envelope-packed ------------------
-  {
-    const int64_t nc = T.n_c;
-    const int64_t ncp = round_up(std::max<int64_t>(nc, 1), kTile);
-    P.coarse_ncp = ncp;
-    const int64_t Tc = 2 * ncp / kTile;
-    std::vector<int64_t> cfrow(2 * ncp);
-    for (int64_t r = 0; r < 2 * ncp; ++r) cfrow[r] = r;
-    for (int64_t k = 0; k < nc; ++k) cfrow[ncp + k] = k;
-    for (int64_t s = 0; s < ns; ++s) {
-      const SubLayout& L = T.sub[s];
-      for (int ka = 0; ka < L.nc; ++ka) {
-        const int64_t Ca = T.c_loc[L.cvec_off + ka];
-        for (int kb = 0; kb <= ka; ++kb) {
-          const int64_t Cb = T.c_loc[L.cvec_off + kb];
-          if (Ca < Cb) throw std::runtime_error("primal order not monotone");
-          cfrow[Ca] = std::min(cfrow[Ca], Cb);
-        }
-      }
-    }
-    P.coarse_ftile.assign(Tc, 0);
-    P.coarse_rowabs.assign(Tc, 0);
-    int64_t off = 0;
-    for (int64_t I = 0; I < Tc; ++I) {
-      int64_t f = I;
-      for (int64_t r = I * kTile; r < (I + 1) * kTile; ++r) f = std::min(f, cfrow[r] / kTile);
-      P.coarse_ftile[I] = int32_t(f);
-      P.coarse_rowabs[I] = off - f * kTileElems;
-      off += (I - f + 1) * kTileElems;
-    }
-    P.coarse_pool_elems = off;
-    const int64_t* cra = P.coarse_rowabs.data();
-    std::vector<uint64_t> tgt;
-    std::vector<int64_t> srcpos;
-    for (int64_t s = 0; s < ns; ++s) {
-      const SubLayout& L = T.sub[s];
-      const int64_t* ra = &P.rowabs[P.ftile_off[s]];
-      for (int ka = 0; ka < L.nc; ++ka) {
-        const int64_t Ca = T.c_loc[L.cvec_off + ka];
-        for (int kb = 0; kb <= ka; ++kb) {
-          const int64_t Cb = T.c_loc[L.cvec_off + kb];
-          const int64_t sp = env_pos(ra, L.o_c + ka, L.o_c + kb);
-          tgt.push_back(uint64_t(env_pos(cra, Ca, Cb)));
-          srcpos.push_back(sp);
-          if (Ca != Cb && Ca / kTile == Cb / kTile) {
-            tgt.push_back(uint64_t(cra[Ca / kTile] + (Ca / kTile) * kTileElems + (Ca % kTile) * kTile + Cb % kTile));
-            srcpos.push_back(sp);
-          }
-        }
-      }
-    }
-    for (int64_t r = nc; r < ncp; ++r) {
-      tgt.push_back(uint64_t(env_pos(cra, r, r)));
-      srcpos.push_back(-1);
-    }
-    for (int64_t k = 0; k < nc; ++k) {
-      tgt.push_back(uint64_t(env_pos(cra, ncp + k, k)));
-      srcpos.push_back(-1);
-    }
-    std::vector<uint32_t> idx(tgt.size());
-    for (size_t i = 0; i < idx.size(); ++i) idx[i] = uint32_t(i);
-    radix_sort_pairs(tgt, idx, uint64_t(P.coarse_pool_elems));
-    for (size_t i = 0; i < tgt.size(); ++i) {
-      if (i == 0 || tgt[i] != tgt[i - 1]) {
-        P.coarse.target.push_back(int64_t(tgt[i]));
-        P.coarse.ptr.push_back(int64_t(i));
-      }
-      P.coarse.srcpos.push_back(srcpos[idx[i]]);
-    }
-    P.coarse.ptr.push_back(int64_t(tgt.size()));
-    // coarse schedule (single matrix, eliminate the first ncp/64 tile columns)
-    P.coarse_sched = single_matrix_schedule(int(ncp / kTile), int(Tc), true, P.coarse_ftile.data());
-  }
+  // ---- coarse pool and its schedule ------------------------------------------
+  build_coarse_plan(T, P, coarse_group_tiles);
 
   // ---- coarse gather positions ----------------------------------------------
   P.c_ypos.resize(T.c_gather_slot.size());
@@ -740,7 +829,7 @@ void build_plans(const Topology& T, Plans& P) {
         for (int64_t i = L.off; i < L.off + L.n; ++i)
           check(S->items[i].I == S->items[i].J + 1 && (S->items[i].flags & 1), "mode-3 item");
   {
-    const int Ta = int(2 * P.coarse_ncp / kTile);
+    const int Ta = int(P.coarse_ftile.size());  // 2 ncp / 64 (dense inverse) or ncp / 64 (grouped)
     for (const TileWork& w : P.coarse_sched.items) {
       check(w.mat == 0 && w.J <= w.I && w.I < Ta && w.k1 <= w.J && w.J >= P.coarse_ftile[w.I],
             "coarse tile");

@@ -79,6 +79,23 @@ struct Plans {
   std::vector<int64_t> rho_pos;  // per D slot: pool position of its diagonal (-1 padding)
   int64_t coarse_ncp = 0;        // padded coarse order
   int64_t coarse_pool_elems = 0;
+  // Grouped coarse solve (coarse_grouped): the coarse pool holds L of S~ = L L^T and
+  // S~^-1 v is a forward and a backward sweep over groups of coarse_group tile columns
+  // (group g = [cg_start[g], cg_start[g+1])).  Per group, the panel tiles left of (fwd) or
+  // below (bwd) its diagonal block, as (I, J) pairs whose products land in one partial
+  // 64-vector slot each (slots of one row / column contiguous: cg_fwd_rowptr per tile row,
+  // cg_bwd_colptr per tile column), and X_g = (L_gg)^-1, dense column-major of order
+  // 64 * (group width) at cg_xoff[g].  cg_inv_items: the (i, j) tiles of every X_g by
+  // distance i - j (step d at [cg_inv_step[d], cg_inv_step[d+1])).
+  bool coarse_grouped = false;
+  int coarse_group = 0;
+  std::vector<int32_t> cg_start, cg_group_of;
+  std::vector<int64_t> cg_xoff;
+  int64_t cg_x_elems = 0;
+  std::vector<int32_t> cg_fwd_tiles, cg_fwd_rowptr, cg_bwd_tiles, cg_bwd_colptr;
+  std::vector<int64_t> cg_fwd_off, cg_bwd_off;
+  std::vector<int32_t> cg_inv_items;
+  std::vector<int64_t> cg_inv_step;
   TileSchedule local_sched, local_sched_ll, coarse_sched;
   std::vector<DevSub> devsub;
   int max_ndp = 0, max_ncp = 0, max_m = 0, max_wrows = 0;
@@ -113,7 +130,10 @@ struct Plans {
 // Packed index of (a, b), a <= b, in the 21-entry upper triangle of a 6x6.
 int packed_index(int a, int b);
 
-void build_plans(const Topology& T, Plans& P);
+// coarse_group_tiles: 0 = dense coarse inverse, > 0 = grouped coarse solve with groups of
+// that many tile columns (build_coarse_plan).
+void build_plans(const Topology& T, Plans& P, int coarse_group_tiles = 0);
+void build_coarse_plan(const Topology& T, Plans& P, int coarse_group_tiles);
 // Only the global free-dof CSR and RHS plans (also part of build_plans).
 void build_global_plan(const Topology& T, CsrPlan& csr, NzPlan& rhs);
 

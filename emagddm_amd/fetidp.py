@@ -66,7 +66,8 @@ class FETIDPSolver:
 
     def __init__(self, precision="fp64", tolerance=1e-8, dual_tolerance=1e-11,
                  max_iterations=1000, max_refinements=12, profile=False,
-                 inner_tolerance=1e-2, graphs=True, precond_precision="auto", scaling="auto"):
+                 inner_tolerance=1e-2, graphs=True, precond_precision="auto", scaling="auto",
+                 coarse_solve="auto", coarse_group=0):
         self._lib = _lib.load()
         cfg = _lib.Config()
         self._lib.fetidp_config_default(C.byref(cfg))
@@ -80,6 +81,8 @@ class FETIDPSolver:
         cfg.graphs = 1 if graphs else 0
         cfg.precond_precision = {"auto": 0, "fp32": 1, "fp64": 2}[precond_precision]
         cfg.scaling = {"auto": 0, "stiffness": 1, "deluxe": 2}[scaling]
+        cfg.coarse_solve = {"auto": 0, "dense": 1, "grouped": 2}[coarse_solve]
+        cfg.coarse_group = int(coarse_group)
         self.precision = precision
         self._h = self._lib.fetidp_create(C.byref(cfg))
         if not self._h:

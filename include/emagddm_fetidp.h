@@ -63,6 +63,16 @@ typedef struct fetidp_config {
                                stream beside it; 0 as a launch of their own, one stream */
   int32_t tile_waves;       /* throughput tile kernels: 4 = capped at 128 registers for four
                                waves per SIMD (default), 0 = the compiler's own allocation */
+  int32_t coarse_solve;     /* coarse problem S~ (order n_c): 0 auto (grouped when n_c > 12288,
+                               else dense), 1 dense: the factorization forms -S~^-1 and each
+                               iteration applies it as one symmetric product (n_c^3 flops at
+                               setup, n_c^2/2 words per iteration), 2 grouped: S~ = L L^T
+                               (envelope) and each iteration sweeps forward and backward over
+                               groups of tile columns with the explicit inverses of the
+                               groups' diagonal blocks.  Read at fetidp_setup; changing it
+                               afterwards rebuilds the coarse plan and drops the factor. */
+  int32_t coarse_group;     /* grouped coarse solve: tile columns (64 rows) per group, 1..64;
+                               0 = 32 */
   int32_t reserved;
 } fetidp_config;
 

@@ -26,11 +26,48 @@ int main(int argc, char** argv) {
     return 1;
   }
   emag::Plans P;
+  // the grouped coarse plan first (groups of 2 and 3 tile columns), with its lists checked
+  // against the bounds the sweep kernels assume, then the default plan
+  long long cg_tiles = 0;
+  for (int G : {2, 3}) {
+    emag::build_plans(T, P, G);
+    const int tn = int(P.coarse_ncp / emag::kTile);
+    const int ng = int(P.cg_start.size()) - 1;
+    bool ok = P.coarse_grouped && P.cg_start.front() == 0 && P.cg_start.back() == tn &&
+              int(P.coarse_ftile.size()) == tn;
+    for (int g = 0; g < ng && ok; ++g) {
+      const int a = P.cg_start[g], b = P.cg_start[g + 1];
+      ok = b - a >= 1 && b - a <= G;
+      for (int64_t k = P.cg_fwd_off[g]; k < P.cg_fwd_off[g + 1] && ok; ++k) {
+        const int I = P.cg_fwd_tiles[2 * k], J = P.cg_fwd_tiles[2 * k + 1];
+        ok = I >= a && I < b && J >= P.coarse_ftile[I] && J < a && k >= P.cg_fwd_rowptr[I] &&
+             k < P.cg_fwd_rowptr[I + 1];
+      }
+      for (int64_t k = P.cg_bwd_off[g]; k < P.cg_bwd_off[g + 1] && ok; ++k) {
+        const int I = P.cg_bwd_tiles[2 * k], J = P.cg_bwd_tiles[2 * k + 1];
+        ok = J >= a && J < b && I >= b && I < tn && P.coarse_ftile[I] <= J &&
+             k >= P.cg_bwd_colptr[J] && k < P.cg_bwd_colptr[J + 1];
+      }
+      int64_t need = int64_t(b - a) * 64;
+      ok = ok && P.cg_xoff[g] + need * need <= P.cg_x_elems;
+    }
+    for (int d = 0; d < G && ok; ++d)
+      for (int64_t k = P.cg_inv_step[d]; k < P.cg_inv_step[d + 1] && ok; ++k) {
+        const int i = P.cg_inv_items[2 * k], j = P.cg_inv_items[2 * k + 1];
+        ok = i - j == d && P.cg_group_of[i] == P.cg_group_of[j];
+      }
+    if (!ok) {
+      std::printf("{\"error\": \"grouped coarse plan out of bounds (G=%d)\"}\n", G);
+      emag_problem_free(&p);
+      return 1;
+    }
+    cg_tiles += (long long)(P.cg_fwd_tiles.size() + P.cg_bwd_tiles.size()) / 2;
+  }
   emag::build_plans(T, P);
   std::printf("{\"n_edges\": %lld, \"n_free\": %lld, \"n_lambda\": %lld, \"n_primal\": %lld, "
-              "\"n_sub\": %d, \"pool_elems\": %lld, \"csr_nnz\": %zu}\n",
+              "\"n_sub\": %d, \"pool_elems\": %lld, \"csr_nnz\": %zu, \"grouped_panel_tiles\": %lld}\n",
               (long long)T.n_edges, (long long)T.n_free, (long long)T.n_lam, (long long)T.n_c,
-              T.n_sub, (long long)P.pool_elems, P.csr.col.size());
+              T.n_sub, (long long)P.pool_elems, P.csr.col.size(), cg_tiles);
   emag_problem_free(&p);
   return 0;
 }
