@@ -153,11 +153,12 @@ def roofline(v, precision, workload):
         fp64 = precision == "fp64" or cls in COARSE_CLASSES
         key = (mode, fp64)
         ms, n = prof[cls]
-        a = by_mode.setdefault(key, [0.0, 0, 0.0])
+        a = by_mode.setdefault(key, [0.0, 0, 0.0, 0.0])
         a[0] += ms
         a[1] += n
         a[2] += prof["flops"].get(cls, 0.0) * PROFILE_STEPS
-    (mode, fp64), (ms, launches, flops) = max(by_mode.items(), key=lambda kv: kv[1][0])
+        a[3] += prof.get("useful_flops", {}).get(cls, 0.0) * PROFILE_STEPS
+    (mode, fp64), (ms, launches, flops, useful) = max(by_mode.items(), key=lambda kv: kv[1][0])
     if launches == 0 or ms == 0:
         return None
     per_launch_flops = flops / launches
@@ -174,7 +175,11 @@ def roofline(v, precision, workload):
                flop_unit="executed 64x64 tile products (padding, full diagonal tiles and the "
                          "TRSM as a product by the stored inverse included)",
                launches_per_step=launches // PROFILE_STEPS,
-               flops_per_launch=per_launch_flops, avg_launch_ms=avg_ms)
+               flops_per_launch=per_launch_flops, avg_launch_ms=avg_ms,
+               # the same launches counted in useful flops: real rows only, triangular
+               # TRSM, lower half of the diagonal-tile share
+               useful_flops_per_launch=useful / launches,
+               useful_frac=round(useful / launches / (avg_ms * 1e-3) / 1e12 / peak, 4))
     if intensity is not None:
         out.update(flop_per_byte=round(intensity, 2), ridge_flop_per_byte=round(ridge, 2),
                    hbm_gbs=round(traffic / (avg_ms * 1e-3) / 1e9, 1))

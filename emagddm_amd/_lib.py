@@ -53,7 +53,7 @@ EXPORTS = [
     "emag_problem_write", "emag_problem_read", "emag_problem_free", "emag_problem_last_error",
     "emag_assembly_create", "emag_assembly_sizes", "emag_assembly_pattern", "emag_assemble",
     "emag_assembly_last_error", "emag_assembly_destroy", "emag_csr_residual",
-    "emag_csr_residual_noalloc",
+    "emag_csr_residual_noalloc", "emag_tree_gauge",
 ]
 HEADERS = ["emagddm_fetidp.h", "emagddm_mpnn.h", "emagddm_problem.h", "emagddm_assembly.h"]
 
@@ -158,6 +158,8 @@ def load():
     lib.emag_assembly_destroy.restype = None
     lib.emag_csr_residual.argtypes = [C.c_int64, P, P, P, P, P, P, dp, P]
     lib.emag_csr_residual_noalloc.argtypes = [C.c_int64, P, P, P, P, P, P, dp, P]
+    lib.emag_tree_gauge.argtypes = [C.c_int64, C.c_int64, ip, C.c_int64, ip, ip]
+    lib.emag_tree_gauge.restype = C.c_int64
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is C.c_int:  # default int return -> status

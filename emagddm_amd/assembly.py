@@ -91,3 +91,18 @@ def csr_residual(row_ptr, col, val, x, f, want_r=False, no_alloc=False):
     if rc != 0:
         raise RuntimeError(f"csr_residual: status {rc}")
     return (rel.value, r) if want_r else rel.value
+
+
+def tree_gauge_pairs(n_nodes: int, tet2node, pec_pairs) -> np.ndarray:
+    """Tree edges of the tree-cotree gauge (emag_tree_gauge) as (n_tree, 2) node pairs;
+    pass them with the PEC pairs to fix A = 0 on them for kappa = 0 problems."""
+    lib = _lib.load()
+    t2n = np.ascontiguousarray(tet2node, dtype=np.int32)
+    pec = np.ascontiguousarray(pec_pairs, dtype=np.int32).reshape(-1, 2)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    n = lib.emag_tree_gauge(int(n_nodes), t2n.shape[0], ip(t2n), pec.shape[0], ip(pec), None)
+    if n < 0:
+        raise RuntimeError("emag_tree_gauge: bad mesh or PEC input")
+    out = np.zeros((max(n, 1), 2), dtype=np.int32)
+    lib.emag_tree_gauge(int(n_nodes), t2n.shape[0], ip(t2n), pec.shape[0], ip(pec), ip(out))
+    return out[:n]

@@ -8,6 +8,7 @@
 // thread with no atomics, so results are bitwise repeatable.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -197,6 +198,72 @@ int32_t emag_assemble(emag_assembly_handle h, const double* d_xyz, const int32_t
 const char* emag_assembly_last_error(emag_assembly_handle h) { return h ? h->err.c_str() : "null handle"; }
 
 void emag_assembly_destroy(emag_assembly_handle h) { delete h; }
+
+int64_t emag_tree_gauge(int64_t n_nodes, int64_t n_tets, const int32_t* tet2node, int64_t n_pec,
+                        const int32_t* pec_edge_nodes, int32_t* tree_edge_nodes) {
+  if (n_nodes <= 0 || n_tets <= 0 || !tet2node || n_pec < 0 || (n_pec > 0 && !pec_edge_nodes)) return -1;
+  try {
+    std::vector<int32_t> sub(size_t(n_tets), 0);
+    Topology T;
+    MeshInput in{n_nodes, n_tets, tet2node, sub.data(), n_pec, pec_edge_nodes, nullptr};
+    if (!build_topology(in, T).empty()) return -1;
+    // node -> free edges incidence, edges in ascending global id
+    std::vector<int64_t> ptr(size_t(n_nodes) + 1, 0);
+    for (int64_t e = 0; e < T.n_edges; ++e)
+      if (T.edge_free[e] >= 0) {
+        ++ptr[size_t(T.edge_nodes[2 * e]) + 1];
+        ++ptr[size_t(T.edge_nodes[2 * e + 1]) + 1];
+      }
+    for (int64_t v = 0; v < n_nodes; ++v) ptr[v + 1] += ptr[v];
+    std::vector<int32_t> inc(static_cast<size_t>(ptr[n_nodes]));
+    {
+      std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+      for (int64_t e = 0; e < T.n_edges; ++e)
+        if (T.edge_free[e] >= 0) {
+          inc[size_t(pos[T.edge_nodes[2 * e]]++)] = int32_t(e);
+          inc[size_t(pos[T.edge_nodes[2 * e + 1]]++)] = int32_t(e);
+        }
+    }
+    // breadth-first search from every PEC node at once: the PEC boundary (tangential A = 0)
+    // acts as one contracted root, so each other node gets exactly one tree edge; further
+    // components without PEC start from their lowest node
+    std::vector<uint8_t> seen(size_t(n_nodes), 0);
+    std::vector<int32_t> queue;
+    queue.reserve(size_t(n_nodes));
+    for (int64_t p = 0; p < 2 * n_pec; ++p)
+      if (!seen[size_t(pec_edge_nodes[p])]) {
+        seen[size_t(pec_edge_nodes[p])] = 1;
+        queue.push_back(pec_edge_nodes[p]);
+      }
+    std::sort(queue.begin(), queue.end());
+    int64_t n_tree = 0;
+    size_t head = 0;
+    for (int64_t root = 0;; ) {
+      while (head < queue.size()) {
+        const int32_t u = queue[head++];
+        for (int64_t k = ptr[u]; k < ptr[u + 1]; ++k) {
+          const int32_t e = inc[size_t(k)];
+          const int32_t v = T.edge_nodes[2 * e] == u ? T.edge_nodes[2 * e + 1] : T.edge_nodes[2 * e];
+          if (seen[size_t(v)]) continue;
+          seen[size_t(v)] = 1;
+          queue.push_back(v);
+          if (tree_edge_nodes) {
+            tree_edge_nodes[2 * n_tree] = T.edge_nodes[2 * e];
+            tree_edge_nodes[2 * n_tree + 1] = T.edge_nodes[2 * e + 1];
+          }
+          ++n_tree;
+        }
+      }
+      while (root < n_nodes && (seen[size_t(root)] || ptr[root] == ptr[root + 1])) ++root;
+      if (root >= n_nodes) break;
+      seen[size_t(root)] = 1;
+      queue.push_back(int32_t(root));
+    }
+    return n_tree;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
 
 int32_t emag_csr_residual(int64_t n, const int32_t* d_row_ptr, const int32_t* d_col,
                           const double* d_val, const double* d_x, const double* d_f, double* d_r,

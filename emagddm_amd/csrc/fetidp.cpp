@@ -1858,19 +1858,48 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
   }
   const TileSchedule* sch[2] = {h->fp32() ? &h->plans.local_sched_ll : &h->plans.local_sched,
                                 &h->plans.coarse_sched};
-  for (int q = 0; q < 2; ++q)
-    for (int mode = 0; mode < 3; ++mode) {
-      double fl = 0.0;
-      for (const TileLaunch& L : sch[q]->launches)
-        // mode 3 is timed with the diagonals, mode 4 (early Schur) with the Schur updates
-        if (L.mode == mode || (mode == 2 && L.mode == 3) || (mode == 0 && L.mode == 4))
-          for (int64_t i = 0; i < L.n; ++i) {
-            const TileWork& w = sch[q]->items[L.off + i];
-            const bool trsm = (L.mode == 1 || L.mode == 3) && !(w.flags & 2);  // not a lookahead
-            fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (trsm ? 1.0 + (w.flags & 1) : 0.0));
-          }
-      if (k < cap) out[k++] = fl;
+  // Real (unpadded) rows of tile row I: subdomain matrices [i | D | c | D'] pad each block
+  // to 64; the coarse pool pads S~ (and, for the dense inverse, the identity block).
+  const Topology& T = h->topo;
+  auto real_rows = [&](int q, int mat, int I) -> double {
+    int64_t r0 = int64_t(I) * kTile, real;
+    if (q == 1) {
+      if (r0 >= h->plans.coarse_ncp) r0 -= h->plans.coarse_ncp;
+      real = T.n_c - r0;
+    } else {
+      const SubLayout& L = T.sub[size_t(mat)];
+      if (r0 < L.o_d) real = L.ni - r0;
+      else if (r0 < L.o_c) real = L.nd - (r0 - L.o_d);
+      else if (r0 < L.o_a) real = L.nc - (r0 - L.o_c);
+      else real = L.nd - (r0 - L.o_a);
     }
+    return double(std::min<int64_t>(std::max<int64_t>(real, 0), kTile));
+  };
+  // executed: 2 * 64^3 per tile product (TRSM as a product with the stored inverse, the
+  // full diagonal-tile share); useful: the same products on the real rows only, the TRSM
+  // as a triangular solve (r_I r_J^2) and the lower half of the diagonal share (r_I^2 r_J)
+  for (int useful = 0; useful < 2; ++useful)
+    for (int q = 0; q < 2; ++q)
+      for (int mode = 0; mode < 3; ++mode) {
+        double fl = 0.0;
+        for (const TileLaunch& L : sch[q]->launches)
+          // mode 3 is timed with the diagonals, mode 4 (early Schur) with the Schur updates
+          if (L.mode == mode || (mode == 2 && L.mode == 3) || (mode == 0 && L.mode == 4))
+            for (int64_t i = 0; i < L.n; ++i) {
+              const TileWork& w = sch[q]->items[L.off + i];
+              const bool trsm = (L.mode == 1 || L.mode == 3) && !(w.flags & 2);  // not a lookahead
+              if (!useful) {
+                fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (trsm ? 1.0 + (w.flags & 1) : 0.0));
+                continue;
+              }
+              const double rI = real_rows(q, w.mat, w.I), rJ = real_rows(q, w.mat, w.J);
+              double kr = 0.0;
+              for (int K = w.k0; K < w.k1; ++K) kr += real_rows(q, w.mat, K);
+              fl += 2.0 * rI * rJ * kr;
+              if (trsm) fl += rI * rJ * rJ + ((w.flags & 1) ? rI * rI * rJ : 0.0);
+            }
+        if (k < cap) out[k++] = fl;
+      }
   return k;
 }
 

@@ -299,9 +299,12 @@ class FETIDPSolver:
         n = self._lib.fetidp_get_profile(self._h, _dp(buf), 64)
         out = {name: (float(buf[2 * i]), int(buf[2 * i + 1]))
                for i, name in enumerate(self.PROFILE_SLOTS)}
+        names = ["tile_schur", "tile_column", "tile_diag", "coarse_schur", "coarse_column",
+                 "coarse_diag"]
         fl = buf[32:38] if n >= 38 else np.zeros(6)
-        out["flops"] = dict(zip(["tile_schur", "tile_column", "tile_diag", "coarse_schur",
-                                 "coarse_column", "coarse_diag"], (float(v) for v in fl)))
+        uf = buf[38:44] if n >= 44 else np.zeros(6)
+        out["flops"] = dict(zip(names, (float(v) for v in fl)))
+        out["useful_flops"] = dict(zip(names, (float(v) for v in uf)))
         return out
 
     def reset_profile(self):

@@ -57,6 +57,19 @@ int32_t emag_assemble(emag_assembly_handle h, const double* d_xyz, const int32_t
 const char* emag_assembly_last_error(emag_assembly_handle h);
 void emag_assembly_destroy(emag_assembly_handle h);
 
+/* Tree-cotree gauge for kappa = 0 (magnetostatics), after EMDOFManager::applyTreeGauge
+ * (src/fe_em/em_dof_manager.cpp:448-495) and TreeGauge (include/fe_em/tree_gauge.hpp:
+ * 131-293): a spanning tree of the mesh graph over the free edges, grown breadth-first from
+ * all PEC nodes at once (the PEC boundary is one contracted root, so every other node gets
+ * exactly one tree edge; a component without PEC starts from its lowest node).  The tree
+ * edges are returned as node pairs, tree_edge_nodes[2 * n_tree] (may be NULL to count):
+ * given to emag_assembly_create or fetidp_setup together with the PEC pairs they are set
+ * to zero like the PEC edges, and the curl-curl system on the remaining cotree edges is
+ * SPD for kappa = 0 on a simply connected domain.  Returns n_tree (at most n_nodes), or
+ * -1 on bad input.  Host only. */
+int64_t emag_tree_gauge(int64_t n_nodes, int64_t n_tets, const int32_t* tet2node, int64_t n_pec,
+                        const int32_t* pec_edge_nodes, int32_t* tree_edge_nodes);
+
 /* FP64 residual operator: r = f - A x over a CSR (device arrays, row_ptr[n + 1]), and
  * *rel_norm = ||r||_2 / ||f||_2 (0 when f = 0) on the host after the stream finishes.
  * d_r may be NULL when only the norm is wanted.  The library allocates the workspace (the

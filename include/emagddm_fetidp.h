@@ -190,8 +190,11 @@ int32_t fetidp_get_coarse_matrix(fetidp_handle h, double* S);
  * launch count of kernel class k (0 integrate, 1 pool assembly, 2..4 local tile
  * modes 0..2, 5 operator extraction, 6..8 coarse tile modes 0..2, 9 dual GEMV
  * of F, 10 rest of F, 11 preconditioner GEMV, 12 forward solve, 13 interior
- * solve, 14-15 spare), then the algorithmic flops of
- * local modes 0..2 and coarse modes 0..2.  Returns the number of values written. */
+ * solve, 14-15 spare), then the executed flops of local modes 0..2 and coarse modes 0..2
+ * (2*64^3 per tile product: padding, the full diagonal-tile share and the TRSM as a
+ * product with the stored inverse included), then the useful flops of the same six
+ * classes (real rows only, triangular TRSM, lower half of the diagonal share).  Returns
+ * the number of values written. */
 int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap);
 void fetidp_reset_profile(fetidp_handle h);
 

@@ -80,3 +80,24 @@ def test_csr_residual(gpu_available):
     # f = 0 -> 0
     z = torch.zeros(g.n_free, dtype=torch.float64, device=dev)
     assert csr_residual(trp, tcol, out["A"], z, z) == 0.0
+
+
+def test_gauged_kappa0_assembly(gpu_available):
+    """kappa = 0 with the tree-cotree gauge (tree edges fixed like PEC): the GPU assembles
+    the gauged system as the oracle does, and it is SPD (its CG converges from zero)."""
+    from emagddm_amd.assembly import tree_gauge_pairs
+    p = W.make_problem(12, 2, seed=6, name="gauge12")
+    zero = np.zeros_like(p.kappa)
+    t = tree_gauge_pairs(p.n_nodes, p.tet2node, p.pec_pairs)
+    pairs = np.vstack([p.pec_pairs, t])
+    g = GlobalAssembly(p.n_nodes, p.tet2node, pairs)
+    out = g.assemble(p.xyz, p.tet_material, p.nu, zero, p.jq, outputs=("A",))
+    o = assembly.assemble(p.xyz, p.tet2node, p.tet_material, p.nu, zero, pairs, p.jq)
+    rp, col, _, _ = g.pattern()
+    assert np.array_equal(rp, o["A"].indptr) and np.array_equal(col, o["A"].indices)
+    a = out["A"].cpu().numpy()
+    assert np.abs(a - o["A"].data).max() <= 1e-13 * np.abs(o["A"].data).max()
+    A = sp.csr_matrix((a, col, rp), shape=(g.n_free, g.n_free))
+    y = np.random.default_rng(1).standard_normal(g.n_free)
+    x, st, its, res = solvers.pcg_jacobi(A, A @ y, 1e-12, 20000)
+    assert st == solvers.SUCCESS and np.linalg.norm(x - y) <= 1e-8 * np.linalg.norm(y)
