@@ -177,13 +177,13 @@ struct fetidp_solver {
       const int64_t o = plans.cg_fwd_off[g];
       launch_cg_forward(gs[g], gs[g + 1] - gs[g], cg_fwd_tiles_d.p + 2 * o, plans.cg_fwd_off[g + 1] - o,
                         cpool.p, coarse_rowabs.p, cgX.p + plans.cg_xoff[g], x, topo.n_c, cg_fwd_rowptr_d.p,
-                        cg_pf.p, o, cg_tv.p, cg_yv.p, st);
+                        cg_pf.p, o, cg_tv.p, cg_px.p, cg_yv.p, st);
     }
     for (int g = ng - 1; g >= 0; --g) {
       const int64_t o = plans.cg_bwd_off[g];
       launch_cg_backward(gs[g], gs[g + 1] - gs[g], cg_bwd_tiles_d.p + 2 * o, plans.cg_bwd_off[g + 1] - o,
                          cpool.p, coarse_rowabs.p, cgX.p + plans.cg_xoff[g], cg_yv.p, plans.coarse_ncp,
-                         cg_bwd_colptr_d.p, cg_pb.p, o, cg_tv.p, cg_xv.p, y, topo.n_c, st);
+                         cg_bwd_colptr_d.p, cg_pb.p, o, cg_tv.p, cg_px.p, cg_xv.p, y, topo.n_c, st);
     }
   }
   DBuf<int> cfail;
@@ -378,7 +378,7 @@ struct fetidp_solver {
   DBuf<int32_t> cg_start_d, cg_group_of_d, cg_fwd_tiles_d, cg_fwd_rowptr_d, cg_bwd_tiles_d,
       cg_bwd_colptr_d, cg_inv_items_d;
   DBuf<int64_t> cg_xoff_d;
-  DBuf<double> cgX, cg_pf, cg_pb, cg_yv, cg_xv, cg_tv;
+  DBuf<double> cgX, cg_pf, cg_pb, cg_yv, cg_xv, cg_tv, cg_px;
 
   bool fell_back = false;  // FP32 requested but the factor fell back to FP64
   bool stalled = false;    // the last FP32 outer refinement stalled (solve hands over to FP64)
@@ -583,6 +583,7 @@ struct fetidp_solver {
       cg_yv.alloc(size_t(plans.coarse_ncp));
       cg_xv.alloc(size_t(plans.coarse_ncp));
       cg_tv.alloc(size_t(plans.coarse_ncp));
+      cg_px.alloc(size_t(plans.coarse_group) * size_t(plans.coarse_group + 1) / 2 * kTile);
       ctiles.release();
       cpart.release();
     } else {

@@ -76,19 +76,20 @@ void launch_coarse_symv(int64_t nc, int tn, int64_t ntiles, const int32_t* tiles
 //    partial slots [slot0, slot0 + npanel) of part, rowptr maps tile rows to their slots;
 //  * launch_cg_backward: x_g = X_g^T (y_g - L_{>g, g}^T x), colptr maps tile columns to
 //    slots; out (length nc, may be null) receives x too.
-// y, x and the group right-hand sides t have ncp entries; rhs has nc.
+// y, x and the group right-hand sides t have ncp entries; rhs has nc.  The X_g products
+// use xpart (w(w+1)/2 partial 64-vectors).
 void launch_cg_invert(int max_group, const int64_t* step, const int32_t* items, const double* cpool,
                       const int64_t* cra, const int32_t* cft, const double* cinv,
                       const int32_t* cg_start, const int32_t* cg_group_of, const int64_t* xoff,
                       double* X, hipStream_t st);
 void launch_cg_forward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
                        const int64_t* cra, const double* Xg, const double* rhs, int64_t nc,
-                       const int32_t* rowptr, double* part, int64_t slot0, double* t, double* y,
-                       hipStream_t st);
+                       const int32_t* rowptr, double* part, int64_t slot0, double* t, double* xpart,
+                       double* y, hipStream_t st);
 void launch_cg_backward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
                         const int64_t* cra, const double* Xg, const double* y, int64_t ncp,
-                        const int32_t* colptr, double* part, int64_t slot0, double* t, double* x,
-                        double* out, int64_t nc, hipStream_t st);
+                        const int32_t* colptr, double* part, int64_t slot0, double* t, double* xpart,
+                        double* x, double* out, int64_t nc, hipStream_t st);
 
 // Forward solve, one launch per block column: yv_r = L_rr^-1 vec_r; afterwards
 // vec_c = f_c - L_cr y and vec_D' = -L_D'r y (= -E K_rr^-1 f_r).

@@ -1266,126 +1266,123 @@ __global__ __launch_bounds__(256) void k_cg_reduce(int a, const double* __restri
   }
 }
 
-// Forward group step: work-group b computes y on tile row a + b of group [a, a + w):
-//   y_{a+b} = sum_{J <= a+b} X_{a+b, J} t_J   (rows of X_g: lane = row, the 16 waves
-// split the columns, eight independent loads in flight per thread; the 16 wave sums are
-// added in a fixed order).
-__global__ __launch_bounds__(1024) void k_cg_fwd(int a, int64_t n, const double* __restrict__ Xg,
-                                                 const double* __restrict__ tv,
-                                                 double* __restrict__ y) {
-  extern __shared__ double t[];  // 64 * (b + 1) entries
-  __shared__ double red[16][64];
-  const int b = blockIdx.x, tid = threadIdx.x;
-  const int ne = 64 * (b + 1);
-  for (int e = tid; e < ne; e += 1024) t[e] = tv[(int64_t)a * 64 + e];
-  __syncthreads();
-  const int lane = tid & 63, wv = tid >> 6;
-  const double* Xr = Xg + (int64_t)b * 64 + lane;
-  double acc[8];
+// Group product with X_g, one work-group per lower tile (i, j) of X_g (row-major
+// enumeration T = i(i+1)/2 + j), each writing one partial 64-vector slot T:
+//   forward  (TRANS = false): y_{a+i} = sum_{j <= i} X_ij t_{a+j}
+//   backward (TRANS = true):  x_{a+j} = sum_{i >= j} X_ij^T t_{a+i}
+// k_cg_xreduce then adds each output block's slots in a fixed order.
+template <bool TRANS>
+__global__ __launch_bounds__(256) void k_cg_xgemv(int a, int64_t n, const double* __restrict__ Xg,
+                                                  const double* __restrict__ tv,
+                                                  double* __restrict__ part) {
+  __shared__ double xs[64];
+  __shared__ double colp[4][64];
+  const int T = blockIdx.x;
+  int i = int((sqrt(8.0 * double(T) + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= T) ++i;
+  while (i * (i + 1) / 2 > T) --i;
+  const int j = T - i * (i + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double at[16];
 #pragma unroll
-  for (int u = 0; u < 8; ++u) acc[u] = 0.0;
-  int c = wv;
-  for (; c + 7 * 16 < ne; c += 8 * 16) {
+  for (int q = 0; q < 16; ++q) at[q] = Xg[(int64_t)(j * 64 + 16 * wv + q) * n + i * 64 + lane];
+  double* pout = part + (int64_t)T * 64;
+  if constexpr (!TRANS) {
+    if (tid < 64) xs[tid] = tv[(int64_t)(a + j) * 64 + tid];
+    __syncthreads();
+    double rs = 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc[u] = fma(Xr[(int64_t)(c + 16 * u) * n], t[c + 16 * u], acc[u]);
-  }
-  for (; c < ne; c += 16) acc[0] = fma(Xr[(int64_t)c * n], t[c], acc[0]);
-  red[wv][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-  __syncthreads();
-  if (tid < 64) {
-    double s = 0.0;
-    for (int k = 0; k < 16; ++k) s += red[k][tid];
-    y[(int64_t)(a + b) * 64 + tid] = s;
+    for (int q = 0; q < 16; ++q) rs = fma(at[q], xs[16 * wv + q], rs);
+    colp[wv][lane] = rs;
+    __syncthreads();
+    if (tid < 64) pout[tid] = (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
+  } else {
+    const double xr = tv[(int64_t)(a + i) * 64 + lane];
+    double rv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) rv[q] = at[q] * xr;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const bool hi = lane & 32;
+      const double keep = hi ? rv[u + 8] : rv[u], send = hi ? rv[u] : rv[u + 8];
+      rv[u] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool hi = lane & 16;
+      const double keep = hi ? rv[u + 4] : rv[u], send = hi ? rv[u] : rv[u + 4];
+      rv[u] = keep + __shfl_xor(send, 16);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const bool hi = lane & 8;
+      const double keep = hi ? rv[u + 2] : rv[u], send = hi ? rv[u] : rv[u + 2];
+      rv[u] = keep + __shfl_xor(send, 8);
+    }
+    {
+      const bool hi = lane & 4;
+      const double keep = hi ? rv[1] : rv[0], send = hi ? rv[0] : rv[1];
+      rv[0] = keep + __shfl_xor(send, 4);
+    }
+    double cs = rv[0];
+    cs += __shfl_xor(cs, 2);
+    cs += __shfl_xor(cs, 1);
+    if ((lane & 3) == 0) {
+      const int col = 16 * wv + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                      ((lane >> 2) & 1);
+      pout[col] = cs;
+    }
   }
 }
 
-// Backward group step: work-group b computes x on tile column a + b of group [a, a + w):
-//   x_{a+b} = sum_{I >= a+b} X_{I, a+b}^T t_I   (column dot products of X_g).
-// Wave w owns output columns 16 (w % 4) .. +15 and the rows of phase w / 4 (lanes stride
-// the rows by 256), 16 independent loads in flight per thread; butterfly column sums, then
-// the four row phases added in a fixed order.
-__global__ __launch_bounds__(1024) void k_cg_bwd(int a, int w, int64_t n, const double* __restrict__ Xg,
-                                                 const double* __restrict__ tv,
-                                                 double* __restrict__ x, double* __restrict__ out,
-                                                 int64_t nc) {
-  extern __shared__ double t[];  // 64 * (w - b) entries
-  __shared__ double csum[4][64];
-  const int b = blockIdx.x, tid = threadIdx.x;
-  const int ne = 64 * (w - b);
-  for (int e = tid; e < ne; e += 1024) t[e] = tv[(int64_t)(a + b) * 64 + e];
+// y_{a+b} (forward) or x_{a+b} (backward) = the fixed-order sum of that block's slots of
+// k_cg_xgemv; one work-group per output block, four threads per entry.
+template <bool TRANS>
+__global__ __launch_bounds__(256) void k_cg_xreduce(int a, int w, const double* __restrict__ part,
+                                                    double* __restrict__ outv, double* __restrict__ out2,
+                                                    int64_t nc) {
+  __shared__ double red[4][64];
+  const int b = blockIdx.x, e = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int k0 = TRANS ? b : 0, k1 = TRANS ? w : b + 1;  // slots of the block: i (bwd) or j (fwd)
+  double s0 = 0.0;
+  for (int k = k0 + q; k < k1; k += 4) {
+    const int T = TRANS ? k * (k + 1) / 2 + b : b * (b + 1) / 2 + k;
+    s0 += part[(int64_t)T * 64 + e];
+  }
+  red[q][e] = s0;
   __syncthreads();
-  const int lane = tid & 63, wv = tid >> 6, cg = wv & 3, rp = wv >> 2;
-  double rv[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) rv[q] = 0.0;
-  const double* Xc = Xg + (int64_t)(b * 64 + 16 * cg) * n + (int64_t)b * 64;  // rows from 64 b
-  for (int r = lane + 64 * rp; r < ne; r += 256) {
-    const double tv_r = t[r];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) rv[q] = fma(Xc[(int64_t)q * n + r], tv_r, rv[q]);
-  }
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const bool hi = lane & 32;
-    const double keep = hi ? rv[i + 8] : rv[i], send = hi ? rv[i] : rv[i + 8];
-    rv[i] = keep + __shfl_xor(send, 32);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const bool hi = lane & 16;
-    const double keep = hi ? rv[i + 4] : rv[i], send = hi ? rv[i] : rv[i + 4];
-    rv[i] = keep + __shfl_xor(send, 16);
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const bool hi = lane & 8;
-    const double keep = hi ? rv[i + 2] : rv[i], send = hi ? rv[i] : rv[i + 2];
-    rv[i] = keep + __shfl_xor(send, 8);
-  }
-  {
-    const bool hi = lane & 4;
-    const double keep = hi ? rv[1] : rv[0], send = hi ? rv[0] : rv[1];
-    rv[0] = keep + __shfl_xor(send, 4);
-  }
-  double cs = rv[0];
-  cs += __shfl_xor(cs, 2);
-  cs += __shfl_xor(cs, 1);
-  if ((lane & 3) == 0) {
-    const int col = 16 * cg + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
-                    ((lane >> 2) & 1);
-    csum[rp][col] = cs;
-  }
-  __syncthreads();
-  if (tid < 64) {
-    const double v = (csum[0][tid] + csum[1][tid]) + (csum[2][tid] + csum[3][tid]);
-    const int64_t gi = int64_t(a + b) * 64 + tid;
-    x[gi] = v;
-    if (out && gi < nc) out[gi] = v;
+  if (q == 0) {
+    const double v = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+    const int64_t gi = int64_t(a + b) * 64 + e;
+    outv[gi] = v;
+    if (out2 && gi < nc) out2[gi] = v;
   }
 }
 
 void launch_cg_forward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
                        const int64_t* cra, const double* Xg, const double* rhs, int64_t nc,
-                       const int32_t* rowptr, double* part, int64_t slot0, double* t, double* y,
-                       hipStream_t st) {
+                       const int32_t* rowptr, double* part, int64_t slot0, double* t, double* xpart,
+                       double* y, hipStream_t st) {
   if (npanel > 0)
     hipLaunchKernelGGL(k_cg_panel<false>, dim3(npanel), dim3(256), 0, st, panel_tiles, cpool, cra, y,
                        part + slot0 * 64);
   hipLaunchKernelGGL(k_cg_reduce, dim3(w), dim3(256), 0, st, a, rhs, nc, rowptr, part, t);
-  hipLaunchKernelGGL(k_cg_fwd, dim3(w), dim3(1024), sizeof(double) * 64 * w, st, a, int64_t(64) * w, Xg,
-                     t, y);
+  hipLaunchKernelGGL(k_cg_xgemv<false>, dim3(w * (w + 1) / 2), dim3(256), 0, st, a, int64_t(64) * w, Xg, t,
+                     xpart);
+  hipLaunchKernelGGL(k_cg_xreduce<false>, dim3(w), dim3(256), 0, st, a, w, xpart, y, nullptr, int64_t(0));
 }
 
 void launch_cg_backward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
                         const int64_t* cra, const double* Xg, const double* y, int64_t ncp,
-                        const int32_t* colptr, double* part, int64_t slot0, double* t, double* x,
-                        double* out, int64_t nc, hipStream_t st) {
+                        const int32_t* colptr, double* part, int64_t slot0, double* t, double* xpart,
+                        double* x, double* out, int64_t nc, hipStream_t st) {
   if (npanel > 0)
     hipLaunchKernelGGL(k_cg_panel<true>, dim3(npanel), dim3(256), 0, st, panel_tiles, cpool, cra, x,
                        part + slot0 * 64);
   hipLaunchKernelGGL(k_cg_reduce, dim3(w), dim3(256), 0, st, a, y, ncp, colptr, part, t);
-  hipLaunchKernelGGL(k_cg_bwd, dim3(w), dim3(1024), sizeof(double) * 64 * w, st, a, w, int64_t(64) * w, Xg,
-                     t, x, out, nc);
+  hipLaunchKernelGGL(k_cg_xgemv<true>, dim3(w * (w + 1) / 2), dim3(256), 0, st, a, int64_t(64) * w, Xg, t,
+                     xpart);
+  hipLaunchKernelGGL(k_cg_xreduce<true>, dim3(w), dim3(256), 0, st, a, w, xpart, x, out, nc);
 }
 
 // ---------------------------------------------------------------------------
