@@ -72,7 +72,8 @@ def parse():
 def run_variant(prob, variant, steps, warmup, max_iterations=1000, autotune=True, graphs=True,
                 launch=None, keep_system=False):
     """Time `steps` factor + solve steps of one variant.  `launch` = (side_stream, graphs,
-    tile_waves) chosen by an earlier variant's autotuner (reused instead of tuning again)."""
+    tile_waves, coarse_group) chosen by an earlier variant's autotuner (reused instead of
+    tuning again)."""
     import torch
     from emagddm_amd import FETIDPSolver, SolverStatus
     from emagddm_amd.lambda0 import LambdaMPNN, default_weights
@@ -93,6 +94,8 @@ def run_variant(prob, variant, steps, warmup, max_iterations=1000, autotune=True
     s.set_rhs_tet_J(prob.jq)          # inputs now resident in HBM
     if launch is not None:
         s.set_config_values(side_stream=launch[0], graphs=launch[1], tile_waves=launch[2])
+        if launch[3] > 0:
+            s.set_config_values(coarse_group=launch[3])
     elif autotune:  # untimed, like the symbolic setup: picks the fastest launch variant
         tuning = s.autotune(repetitions=1)
     for _ in range(warmup):
@@ -130,7 +133,7 @@ def run_variant(prob, variant, steps, warmup, max_iterations=1000, autotune=True
     # the free-dof system for the CPU baseline, read back before the handle's device
     # memory is released (the next variant needs it at 96^3)
     csr = s.global_system() if keep_system else None
-    launch_used = (cfg.side_stream, cfg.graphs, cfg.tile_waves)
+    launch_used = (cfg.side_stream, cfg.graphs, cfg.tile_waves, cfg.coarse_group)
     s.clear()
     if model is not None:
         model.close()
@@ -321,9 +324,9 @@ def main():
                              initial_dual_residual=v["result"].initial_dual_residual,
                              launch=list(v["launch"]),
                              autotune=(None if v["tuning"] is None else dict(
-                                 chosen=list(min(v["tuning"], key=lambda t: t[3])[:3]),
-                                 variants_ms={f"side{a}_graph{b}_waves{c}": round(d, 2)
-                                              for a, b, c, d in v["tuning"]})))
+                                 chosen=list(v["launch"]),
+                                 variants_ms={f"side{a}_graph{b}_waves{c}_group{g}": round(d, 2)
+                                              for a, b, c, g, d in v["tuning"]})))
                      for k, v in variants.items()},
         "failed_variants": failed,
         # the metric's own ratio: plain FP64 time over the learned + FP32-refined time

@@ -379,6 +379,36 @@ struct fetidp_solver {
       cg_bwd_colptr_d, cg_inv_items_d;
   DBuf<int64_t> cg_xoff_d;
   DBuf<double> cgX, cg_pf, cg_pb, cg_yv, cg_xv, cg_tv, cg_px;
+  void upload_coarse_groups() {
+    cg_start_d.upload(plans.cg_start);
+    cg_group_of_d.upload(plans.cg_group_of);
+    cg_fwd_tiles_d.upload(plans.cg_fwd_tiles);
+    cg_fwd_rowptr_d.upload(plans.cg_fwd_rowptr);
+    cg_bwd_tiles_d.upload(plans.cg_bwd_tiles);
+    cg_bwd_colptr_d.upload(plans.cg_bwd_colptr);
+    cg_inv_items_d.upload(plans.cg_inv_items);
+    cg_xoff_d.upload(plans.cg_xoff);
+    cgX.alloc(size_t(plans.cg_x_elems));
+    cg_pf.alloc(size_t(std::max<int64_t>(plans.cg_fwd_off.back(), 1)) * kTile);
+    cg_pb.alloc(size_t(std::max<int64_t>(plans.cg_bwd_off.back(), 1)) * kTile);
+    cg_yv.alloc(size_t(plans.coarse_ncp));
+    cg_xv.alloc(size_t(plans.coarse_ncp));
+    cg_tv.alloc(size_t(plans.coarse_ncp));
+    cg_px.alloc(size_t(plans.coarse_group) * size_t(plans.coarse_group + 1) / 2 * kTile);
+  }
+  // Another group width for a grouped coarse solve already on the device: only the group
+  // lists change (pool, schedule and factor do not depend on the width); the next factor
+  // forms the group inverses.
+  void set_coarse_group(int G) {
+    if (!plans.coarse_grouped || G == plans.coarse_group) return;
+    build_coarse_groups(plans, G);
+    coarse_group_used = G;
+    cfg.coarse_group = G;
+    if (device_ready) upload_coarse_groups();
+    factored = false;
+    if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));
+    pcpg_exec = nullptr;
+  }
 
   bool fell_back = false;  // FP32 requested but the factor fell back to FP64
   bool stalled = false;    // the last FP32 outer refinement stalled (solve hands over to FP64)
@@ -569,21 +599,7 @@ struct fetidp_solver {
     cinv.alloc(plans.coarse_inv_elems);
     coarse_tn = int(plans.coarse_ncp / kTile);
     if (plans.coarse_grouped) {
-      cg_start_d.upload(plans.cg_start);
-      cg_group_of_d.upload(plans.cg_group_of);
-      cg_fwd_tiles_d.upload(plans.cg_fwd_tiles);
-      cg_fwd_rowptr_d.upload(plans.cg_fwd_rowptr);
-      cg_bwd_tiles_d.upload(plans.cg_bwd_tiles);
-      cg_bwd_colptr_d.upload(plans.cg_bwd_colptr);
-      cg_inv_items_d.upload(plans.cg_inv_items);
-      cg_xoff_d.upload(plans.cg_xoff);
-      cgX.alloc(size_t(plans.cg_x_elems));
-      cg_pf.alloc(size_t(std::max<int64_t>(plans.cg_fwd_off.back(), 1)) * kTile);
-      cg_pb.alloc(size_t(std::max<int64_t>(plans.cg_bwd_off.back(), 1)) * kTile);
-      cg_yv.alloc(size_t(plans.coarse_ncp));
-      cg_xv.alloc(size_t(plans.coarse_ncp));
-      cg_tv.alloc(size_t(plans.coarse_ncp));
-      cg_px.alloc(size_t(plans.coarse_group) * size_t(plans.coarse_group + 1) / 2 * kTile);
+      upload_coarse_groups();
       ctiles.release();
       cpart.release();
     } else {
@@ -1460,14 +1476,16 @@ struct fetidp_solver {
   // leave the arithmetic unchanged (where the diagonal factorizations run: same
   // stream, second stream, or at the front of the column-step launch; PCPG graph
   // replay; throughput tile kernels capped at 128 registers or allocated freely),
-  // check that every combination returns bitwise the same solution, keep the fastest.
-  // out[3k..3k+2] = (side_stream, graphs, tile_waves), out[... ] times in milliseconds:
-  // per variant 4 values {side_stream, graphs, tile_waves, ms}.
+  // check that every combination returns bitwise the same solution, keep the fastest;
+  // then, for a grouped coarse solve, time the group widths with that launch choice.
+  // The width changes the order of the coarse sums, so those solutions are checked to
+  // 1e-9 (relative 2-norm) instead of bitwise.  Per variant 5 values in out:
+  // {side_stream, graphs, tile_waves, coarse group width (0 = dense inverse), ms}.
   int autotune(int reps, double* out, int cap) {
     if (!factored && !rhs_set) throw InputError("autotune needs a right-hand side (set_rhs_*) first");
     reps = std::max(reps, 1);
     const fetidp_config base = cfg;
-    std::vector<double> ref;
+    std::vector<double> ref, xh(size_t(topo.n_free));
     double best = 1e300;
     int best_v = -1, k = 0, nv = 0;
     auto pick = [&](int v) {
@@ -1475,8 +1493,7 @@ struct fetidp_solver {
       cfg.graphs = (v / 3) & 1;
       cfg.tile_waves = (v / 6) & 1 ? 0 : 4;
     };
-    for (int v = 0; v < 12; ++v) {
-      pick(v);
+    auto time_it = [&]() {
       fetidp_stats S{};
       factor();
       solve(nullptr, nullptr, S);  // warm-up (graph capture, allocations)
@@ -1487,20 +1504,28 @@ struct fetidp_solver {
         solve(nullptr, nullptr, S);
       }
       const double ms = (now_ms() - t0) / reps;
-      std::vector<double> xh(size_t(topo.n_free));
       HIPCHECK(hipMemcpy(xh.data(), xtot.p, sizeof(double) * xh.size(), hipMemcpyDeviceToHost));
+      return ms;
+    };
+    auto record = [&](double ms) {
+      if (k + 5 <= cap && out) {
+        out[k++] = cfg.side_stream;
+        out[k++] = cfg.graphs;
+        out[k++] = cfg.tile_waves;
+        out[k++] = plans.coarse_grouped ? plans.coarse_group : 0;
+        out[k++] = ms;
+      }
+      ++nv;
+    };
+    for (int v = 0; v < 12; ++v) {
+      pick(v);
+      const double ms = time_it();
       if (ref.empty()) ref = xh;
       else if (std::memcmp(ref.data(), xh.data(), sizeof(double) * xh.size()) != 0) {
         cfg = base;
         throw NumericError("autotune: launch variant changed the solution");
       }
-      if (k + 4 <= cap && out) {
-        out[k++] = cfg.side_stream;
-        out[k++] = cfg.graphs;
-        out[k++] = cfg.tile_waves;
-        out[k++] = ms;
-      }
-      ++nv;
+      record(ms);
       if (ms < best) {
         best = ms;
         best_v = v;
@@ -1508,6 +1533,35 @@ struct fetidp_solver {
     }
     cfg = base;
     pick(best_v);
+    if (plans.coarse_grouped) {
+      const int tn = int(plans.coarse_ncp / kTile);
+      const int g0 = plans.coarse_group;
+      double rn = 0.0;
+      for (double v : ref) rn += v * v;
+      int best_g = g0;
+      double best_ms = best;
+      std::vector<int> widths{g0};
+      for (int G : {8, 16, 24, 32, 48, 64})
+        if (G != g0 && G <= std::max(tn, 1)) widths.push_back(G);
+      for (size_t w = 1; w < widths.size(); ++w) {
+        set_coarse_group(widths[w]);
+        const double ms = time_it();
+        double dn = 0.0;
+        for (size_t i = 0; i < xh.size(); ++i) dn += (xh[i] - ref[i]) * (xh[i] - ref[i]);
+        if (!(dn <= 1e-18 * rn)) {
+          set_coarse_group(g0);
+          cfg.coarse_group = base.coarse_group;
+          factor();
+          throw NumericError("autotune: a coarse group width changed the solution beyond 1e-9");
+        }
+        record(ms);
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_g = widths[w];
+        }
+      }
+      set_coarse_group(best_g);
+    }
     factor();
     return nv;
   }

@@ -285,7 +285,7 @@ void build_global_plan(const Topology& T, CsrPlan& csr, NzPlan& rhs) {
 // tile columns g = [a, b): t_g = v_g - L_{g, <a} y (panel products), y_g = X_g t_g with
 // X_g = (L_gg)^-1 explicit; L^-T y by the backward sweep with the panels below the groups
 // and X_g^T.
-static void build_coarse_groups(Plans& P, int G) {
+void build_coarse_groups(Plans& P, int G) {
   const int tn = int(P.coarse_ncp / kTile);
   const int32_t* ft = P.coarse_ftile.data();
   P.coarse_grouped = true;

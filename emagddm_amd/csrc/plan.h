@@ -134,6 +134,9 @@ int packed_index(int a, int b);
 // that many tile columns (build_coarse_plan).
 void build_plans(const Topology& T, Plans& P, int coarse_group_tiles = 0);
 void build_coarse_plan(const Topology& T, Plans& P, int coarse_group_tiles);
+// Group lists of a grouped coarse plan for another group width (the pool, its schedule and
+// its factor do not depend on the width).
+void build_coarse_groups(Plans& P, int group_tiles);
 // Only the global free-dof CSR and RHS plans (also part of build_plans).
 void build_global_plan(const Topology& T, CsrPlan& csr, NzPlan& rhs);
 

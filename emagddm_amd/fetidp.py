@@ -99,14 +99,15 @@ class FETIDPSolver:
 
     def autotune(self, repetitions=2):
         """Time the launch variants (where the diagonal factorizations run, graph replay,
-        tile-kernel waves) on the current problem, keep the fastest; returns
-        [(side_stream, graphs, waves, ms)]."""
-        buf = np.zeros(64)
-        n = self._lib.fetidp_autotune(self._h, repetitions, _dp(buf), 64)
+        tile-kernel waves) and, for a grouped coarse solve, the coarse group widths on the
+        current problem, keep the fastest; returns [(side_stream, graphs, waves,
+        coarse_group, ms)] (coarse_group 0: dense coarse inverse)."""
+        buf = np.zeros(5 * 32)
+        n = self._lib.fetidp_autotune(self._h, repetitions, _dp(buf), buf.size)
         if n < 0:
             raise RuntimeError(f"fetidp_autotune: {SolverStatus(-n).name}: {self.last_error()}")
-        rows = buf[:4 * n].reshape(n, 4)
-        return [(int(a), int(b), int(c), float(d)) for a, b, c, d in rows]
+        rows = buf[:5 * n].reshape(n, 5)
+        return [(int(a), int(b), int(c), int(g), float(d)) for a, b, c, g, d in rows]
 
     def set_config_values(self, **fields):
         """Change configuration fields (names of fetidp_config) on the live handle."""

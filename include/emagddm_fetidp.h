@@ -221,8 +221,12 @@ int32_t fetidp_get_lambda_features(fetidp_handle h, float* features, double* sca
 /* Autotuner: times `repetitions` factor + solve runs of the current problem for every
  * combination of the launch choices that keep the arithmetic identical (side_stream,
  * graphs, tile_waves), checks that all return bitwise the same solution, and keeps
- * the fastest in the handle's configuration (then refactors).  timings[4k..4k+3] =
- * {side_stream, graphs, tile_waves, ms per factor + solve} for variant k (up to cap
+ * the fastest in the handle's configuration.  When the coarse solve is grouped it then
+ * times, with that launch choice, the group widths 8, 16, 24, 32, 48 and 64 tile columns
+ * that fit the coarse matrix (a tile shape: the width changes the order of the coarse
+ * sums, so those solutions must agree with the first to 1e-9 in the 2-norm) and keeps the
+ * fastest (then refactors).  timings[5k..5k+4] = {side_stream, graphs, tile_waves, coarse
+ * group width (0: dense coarse inverse), ms per factor + solve} for variant k (up to cap
  * values).  Returns the number of variants, or -status on error.  Needs a RHS.
  * No reference counterpart. */
 int32_t fetidp_autotune(fetidp_handle h, int32_t repetitions, double* timings, int32_t cap);

@@ -31,6 +31,8 @@ def _host_residual(s, x):
     return float(np.linalg.norm(f - A @ x) / np.linalg.norm(f))
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "cube48j_llt.npz")),
+                    reason="cube48j_llt.npz not generated (tests/golden/make_golden.py cube48j)")
 def test_cube48j_matches_reference_llt(gpu_available):
     g = np.load(os.path.join(GOLDEN, "cube48j_llt.npz"))
     p = W.make_workload("cube48j")
