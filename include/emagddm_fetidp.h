@@ -224,8 +224,8 @@ int32_t fetidp_get_lambda_features(fetidp_handle h, float* features, double* sca
  * the fastest in the handle's configuration.  When the coarse solve is grouped it then
  * times, with that launch choice, the group widths 8, 16, 24, 32, 48 and 64 tile columns
  * that fit the coarse matrix (a tile shape: the width changes the order of the coarse
- * sums, so those solutions must agree with the first to 1e-9 in the 2-norm) and keeps the
- * fastest (then refactors).  timings[5k..5k+4] = {side_stream, graphs, tile_waves, coarse
+ * sums and so the PCPG trajectory, hence a width is kept only if its solve succeeds with
+ * the residual within the tolerance) and keeps the fastest (then refactors).  timings[5k..5k+4] = {side_stream, graphs, tile_waves, coarse
  * group width (0: dense coarse inverse), ms per factor + solve} for variant k (up to cap
  * values).  Returns the number of variants, or -status on error.  Needs a RHS.
  * No reference counterpart. */
