@@ -1182,7 +1182,8 @@ struct fetidp_solver {
     }
     if (head_queued) throw std::logic_error("graph iterations form their own W p");
     const void* key[3] = {W.p, p64 ? (const void*)Pd.p : (const void*)Pf.p, cpool.p};
-    const int key_t = int(sizeof(T)) | (deluxe ? 256 : 0);
+    // the precision flags too: a released buffer's address can come back for the other one
+    const int key_t = int(sizeof(T)) | (deluxe ? 256 : 0) | (p64 ? 512 : 0);
     if (!pcpg_exec || pcpg_key_t != key_t || std::memcmp(key, pcpg_key, sizeof(key))) {
       if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));
       pcpg_exec = nullptr;
