@@ -170,21 +170,32 @@ struct fetidp_solver {
                          cpart.p, y, st);
       return;
     }
-    // S~^-1 x = L^-T L^-1 x: forward sweep over the groups, then backward
+    // S~^-1 x = L^-T L^-1 x: forward sweep over the groups, then backward; step A of a
+    // group also writes the previous group's output blocks
     const std::vector<int32_t>& gs = plans.cg_start;
     const int ng = int(gs.size()) - 1;
     for (int g = 0; g < ng; ++g) {
       const int64_t o = plans.cg_fwd_off[g];
-      launch_cg_forward(gs[g], gs[g + 1] - gs[g], cg_fwd_tiles_d.p + 2 * o, plans.cg_fwd_off[g + 1] - o,
-                        cpool.p, coarse_rowabs.p, cgX.p + plans.cg_xoff[g], x, topo.n_c, cg_fwd_rowptr_d.p,
-                        cg_pf.p, o, cg_tv.p, cg_px.p, cg_yv.p, st);
+      const int pa = g > 0 ? gs[g - 1] : 0, pw = g > 0 ? gs[g] - gs[g - 1] : 0;
+      launch_cg_step_a(false, pa, pw, cg_px.p, cg_yv.p, nullptr, 0, cg_fwd_tiles_d.p + 2 * o,
+                       plans.cg_fwd_off[g + 1] - o, cpool.p, coarse_rowabs.p, cg_yv.p, cg_pf.p + o * kTile, st);
+      launch_cg_step_b(false, gs[g], gs[g + 1] - gs[g], cgX.p + plans.cg_xoff[g], x, topo.n_c,
+                       cg_fwd_rowptr_d.p, cg_pf.p, cg_px.p, st);
     }
+    if (ng > 0)
+      launch_cg_step_a(false, gs[ng - 1], gs[ng] - gs[ng - 1], cg_px.p, cg_yv.p, nullptr, 0, nullptr, 0,
+                       cpool.p, coarse_rowabs.p, nullptr, nullptr, st);
     for (int g = ng - 1; g >= 0; --g) {
       const int64_t o = plans.cg_bwd_off[g];
-      launch_cg_backward(gs[g], gs[g + 1] - gs[g], cg_bwd_tiles_d.p + 2 * o, plans.cg_bwd_off[g + 1] - o,
-                         cpool.p, coarse_rowabs.p, cgX.p + plans.cg_xoff[g], cg_yv.p, plans.coarse_ncp,
-                         cg_bwd_colptr_d.p, cg_pb.p, o, cg_tv.p, cg_px.p, cg_xv.p, y, topo.n_c, st);
+      const int pa = g + 1 < ng ? gs[g + 1] : 0, pw = g + 1 < ng ? gs[g + 2] - gs[g + 1] : 0;
+      launch_cg_step_a(true, pa, pw, cg_px.p, cg_xv.p, y, topo.n_c, cg_bwd_tiles_d.p + 2 * o,
+                       plans.cg_bwd_off[g + 1] - o, cpool.p, coarse_rowabs.p, cg_xv.p, cg_pb.p + o * kTile, st);
+      launch_cg_step_b(true, gs[g], gs[g + 1] - gs[g], cgX.p + plans.cg_xoff[g], cg_yv.p, plans.coarse_ncp,
+                       cg_bwd_colptr_d.p, cg_pb.p, cg_px.p, st);
     }
+    if (ng > 0)
+      launch_cg_step_a(true, gs[0], gs[1] - gs[0], cg_px.p, cg_xv.p, y, topo.n_c, nullptr, 0, cpool.p,
+                       coarse_rowabs.p, nullptr, nullptr, st);
   }
   DBuf<int> cfail;
   DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc, yvec, tbuf;
@@ -378,7 +389,7 @@ struct fetidp_solver {
   DBuf<int32_t> cg_start_d, cg_group_of_d, cg_fwd_tiles_d, cg_fwd_rowptr_d, cg_bwd_tiles_d,
       cg_bwd_colptr_d, cg_inv_items_d;
   DBuf<int64_t> cg_xoff_d;
-  DBuf<double> cgX, cg_pf, cg_pb, cg_yv, cg_xv, cg_tv, cg_px;
+  DBuf<double> cgX, cg_pf, cg_pb, cg_yv, cg_xv, cg_px;
   void upload_coarse_groups() {
     cg_start_d.upload(plans.cg_start);
     cg_group_of_d.upload(plans.cg_group_of);
@@ -393,7 +404,6 @@ struct fetidp_solver {
     cg_pb.alloc(size_t(std::max<int64_t>(plans.cg_bwd_off.back(), 1)) * kTile);
     cg_yv.alloc(size_t(plans.coarse_ncp));
     cg_xv.alloc(size_t(plans.coarse_ncp));
-    cg_tv.alloc(size_t(plans.coarse_ncp));
     cg_px.alloc(size_t(plans.coarse_group) * size_t(plans.coarse_group + 1) / 2 * kTile);
   }
   // Another group width for a grouped coarse solve already on the device: only the group

@@ -72,24 +72,26 @@ void launch_coarse_symv(int64_t nc, int tn, int64_t ntiles, const int32_t* tiles
 // [a, a + w) with X_g = (L_gg)^-1 dense column-major of order 64 w):
 //  * launch_cg_invert forms every X_g after the factorization (one launch per distance
 //    from the diagonal, max_group launches);
-//  * launch_cg_forward: y_g = X_g (rhs_g - L_{g, <a} y), the panel tiles of group g give
-//    partial slots [slot0, slot0 + npanel) of part, rowptr maps tile rows to their slots;
-//  * launch_cg_backward: x_g = X_g^T (y_g - L_{>g, g}^T x), colptr maps tile columns to
-//    slots; out (length nc, may be null) receives x too.
-// y, x and the group right-hand sides t have ncp entries; rhs has nc.  The X_g products
-// use xpart (w(w+1)/2 partial 64-vectors).
+//  * S~^-1 v = L^-T L^-1 v by a forward sweep (groups in order: y_g = X_g (v_g -
+//    L_{g, <a} y)) and a backward sweep (groups in reverse: x_g = X_g^T (y_g -
+//    L_{>g, g}^T x)), two launches per group:
+//    launch_cg_step_a: the output blocks of the previous group [pa, pa + pw) from xpart
+//      into outv (and out2 for entries below nc; may be null), plus the panel products
+//      of this group (npanel tiles, partial 64-vectors into part; vector blocks from vsrc,
+//      or from xpart when they lie in the previous group);
+//    launch_cg_step_b: the X_g (trans: X_g^T) products of the group [a, a + w), each on
+//      its right-hand side block base - (panel slots [ptr[J], ptr[J+1]) of part) (base
+//      entries past nb read as zero), partials into xpart (w(w+1)/2 64-vectors).
+//    A step A with npanel = 0 writes the last group's output.
 void launch_cg_invert(int max_group, const int64_t* step, const int32_t* items, const double* cpool,
                       const int64_t* cra, const int32_t* cft, const double* cinv,
                       const int32_t* cg_start, const int32_t* cg_group_of, const int64_t* xoff,
                       double* X, hipStream_t st);
-void launch_cg_forward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
-                       const int64_t* cra, const double* Xg, const double* rhs, int64_t nc,
-                       const int32_t* rowptr, double* part, int64_t slot0, double* t, double* xpart,
-                       double* y, hipStream_t st);
-void launch_cg_backward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
-                        const int64_t* cra, const double* Xg, const double* y, int64_t ncp,
-                        const int32_t* colptr, double* part, int64_t slot0, double* t, double* xpart,
-                        double* x, double* out, int64_t nc, hipStream_t st);
+void launch_cg_step_a(bool trans, int pa, int pw, const double* xpart, double* outv, double* out2,
+                      int64_t nc, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
+                      const int64_t* cra, const double* vsrc, double* part, hipStream_t st);
+void launch_cg_step_b(bool trans, int a, int w, const double* Xg, const double* base, int64_t nb,
+                      const int32_t* ptr, const double* part, double* xpart, hipStream_t st);
 
 // Forward solve, one launch per block column: yv_r = L_rr^-1 vec_r; afterwards
 // vec_c = f_c - L_cr y and vec_D' = -L_D'r y (= -E K_rr^-1 f_r).

@@ -1171,84 +1171,26 @@ void launch_cg_invert(int max_group, const int64_t* step, const int32_t* items, 
   }
 }
 
-// Panel products of the sweeps, one work-group per tile (I, J) of L, partial 64-vector
-// slot = work-group index: forward (TRANS = false) part = L_IJ v_J, backward (TRANS =
-// true) part = L_IJ^T v_I.  Lane = tile row, wave w holds columns 16w .. 16w+15 (every
-// load one coalesced 512-byte column); row sums through LDS, column sums by the halving
-// butterfly of k_coarse_symv.
-template <bool TRANS>
-__global__ __launch_bounds__(256) void k_cg_panel(const int32_t* __restrict__ tiles,
-                                                  const double* __restrict__ cpool,
-                                                  const int64_t* __restrict__ cra,
-                                                  const double* __restrict__ v,
-                                                  double* __restrict__ part) {
-  __shared__ double xs[64];
-  __shared__ double colp[4][64];
-  const int I = tiles[2 * blockIdx.x], J = tiles[2 * blockIdx.x + 1];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const double* A = cpool + cra[I] + (int64_t)J * kTileElems;
-  double at[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) at[q] = A[(16 * wv + q) * 64 + lane];
-  double* out = part + (int64_t)blockIdx.x * 64;
-  if constexpr (!TRANS) {
-    if (tid < 64) xs[tid] = v[(int64_t)J * 64 + tid];
-    __syncthreads();
-    double rs = 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) rs = fma(at[q], xs[16 * wv + q], rs);
-    colp[wv][lane] = rs;
-    __syncthreads();
-    if (tid < 64) out[tid] = (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
-  } else {
-    const double xr = v[(int64_t)I * 64 + lane];
-    double rv[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) rv[q] = at[q] * xr;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const bool hi = lane & 32;
-      const double keep = hi ? rv[i + 8] : rv[i], send = hi ? rv[i] : rv[i + 8];
-      rv[i] = keep + __shfl_xor(send, 32);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const bool hi = lane & 16;
-      const double keep = hi ? rv[i + 4] : rv[i], send = hi ? rv[i] : rv[i + 4];
-      rv[i] = keep + __shfl_xor(send, 16);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const bool hi = lane & 8;
-      const double keep = hi ? rv[i + 2] : rv[i], send = hi ? rv[i] : rv[i + 2];
-      rv[i] = keep + __shfl_xor(send, 8);
-    }
-    {
-      const bool hi = lane & 4;
-      const double keep = hi ? rv[1] : rv[0], send = hi ? rv[0] : rv[1];
-      rv[0] = keep + __shfl_xor(send, 4);
-    }
-    double cs = rv[0];
-    cs += __shfl_xor(cs, 2);
-    cs += __shfl_xor(cs, 1);
-    if ((lane & 3) == 0) {
-      const int col = 16 * wv + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
-                      ((lane >> 2) & 1);
-      out[col] = cs;
-    }
-  }
-}
+// Grouped sweeps, two launches per group (the fixed-order sums are recomputed where they
+// are consumed instead of being launches of their own; every work-group that forms one of
+// them does the same additions in the same order, so all copies agree bitwise):
+//  * step A of group g: the output blocks of group g-1 (sum of its X-product slots,
+//    written to outv / out2) and the panel products of group g (one work-group per tile
+//    (I, J) of L, partial 64-vector slot = its index: forward L_IJ v_J, backward
+//    L_IJ^T v_I; a block v of group g-1 is summed from the X-product slots in place);
+//  * step B of group g: one work-group per lower tile (i, j) of X_g, which first forms its
+//    right-hand side block t = base - (sum of that block's panel slots) and then writes
+//    the partial X_ij t (forward) or X_ij^T t (backward) to slot i(i+1)/2 + j.
 
-// Right-hand side of one group step: for tile row (column) J = a + blockIdx.x of the
-// group, t_J = base_J - sum of its partial slots [ptr[J], ptr[J+1]) (base entries past
-// nb read as zero).  Four threads per entry each sum every fourth slot; the four sums are
-// added in a fixed order, so the result does not depend on scheduling.
-__global__ __launch_bounds__(256) void k_cg_reduce(int a, const double* __restrict__ base, int64_t nb,
-                                                   const int32_t* __restrict__ ptr,
-                                                   const double* __restrict__ part,
-                                                   double* __restrict__ t) {
-  __shared__ double red[4][64];
-  const int J = a + blockIdx.x, e = threadIdx.x & 63, q = threadIdx.x >> 6;
+// The right-hand side block J of a group step: base_J - the sum of its panel slots
+// [ptr[J], ptr[J+1]) (base entries past nb read as zero).  Four threads per entry each
+// sum every fourth slot; the four sums are added in a fixed order.  256 threads; the
+// result is in tout[0..63] for every thread after the call.
+__device__ __forceinline__ void cg_rhs_block(int J, const double* __restrict__ base, int64_t nb,
+                                             const int32_t* __restrict__ ptr,
+                                             const double* __restrict__ part, double (*red)[64],
+                                             double* tout) {
+  const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
   double s0 = 0.0, s1 = 0.0;
   int k = ptr[J] + q;
   const int k1 = ptr[J + 1];
@@ -1262,42 +1204,45 @@ __global__ __launch_bounds__(256) void k_cg_reduce(int a, const double* __restri
   if (q == 0) {
     const int64_t gi = int64_t(J) * 64 + e;
     const double b = gi < nb ? base[gi] : 0.0;
-    t[gi] = b - ((red[0][e] + red[1][e]) + (red[2][e] + red[3][e]));
+    tout[e] = b - ((red[0][e] + red[1][e]) + (red[2][e] + red[3][e]));
   }
+  __syncthreads();
 }
 
-// Group product with X_g, one work-group per lower tile (i, j) of X_g (row-major
-// enumeration T = i(i+1)/2 + j), each writing one partial 64-vector slot T:
-//   forward  (TRANS = false): y_{a+i} = sum_{j <= i} X_ij t_{a+j}
-//   backward (TRANS = true):  x_{a+j} = sum_{i >= j} X_ij^T t_{a+i}
-// k_cg_xreduce then adds each output block's slots in a fixed order.
+// Output block b of a group of width w from its X-product slots: forward y_{a+b} = sum over
+// j <= b of slot b(b+1)/2 + j, backward x_{a+b} = sum over i >= b of slot i(i+1)/2 + b.
+// Same thread layout and order as cg_rhs_block; result in vout[0..63] after the call.
 template <bool TRANS>
-__global__ __launch_bounds__(256) void k_cg_xgemv(int a, int64_t n, const double* __restrict__ Xg,
-                                                  const double* __restrict__ tv,
-                                                  double* __restrict__ part) {
-  __shared__ double xs[64];
-  __shared__ double colp[4][64];
-  const int T = blockIdx.x;
-  int i = int((sqrt(8.0 * double(T) + 1.0) - 1.0) * 0.5);
-  while ((i + 1) * (i + 2) / 2 <= T) ++i;
-  while (i * (i + 1) / 2 > T) --i;
-  const int j = T - i * (i + 1) / 2;
+__device__ __forceinline__ void cg_out_block(int b, int w, const double* __restrict__ xpart,
+                                             double (*red)[64], double* vout) {
+  const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int k0 = TRANS ? b : 0, k1 = TRANS ? w : b + 1;
+  double s0 = 0.0;
+  for (int k = k0 + q; k < k1; k += 4) {
+    const int T = TRANS ? k * (k + 1) / 2 + b : b * (b + 1) / 2 + k;
+    s0 += xpart[(int64_t)T * 64 + e];
+  }
+  red[q][e] = s0;
+  __syncthreads();
+  if (q == 0) vout[e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+  __syncthreads();
+}
+
+// 64x64 tile (lane = row, wave w holds columns 16w .. 16w+15 in at[]) times a 64-vector:
+// non-transposed: out[r] = sum_c A[r][c] xs[c] (column sums through LDS); transposed:
+// out[c] = sum_r A[r][c] xr (xr = the lane's entry; halving butterfly over the lanes).
+template <bool TRANS>
+__device__ __forceinline__ void cg_tile_product(const double (&at)[16], const double* xs, double xr,
+                                                double (*colp)[64], double* __restrict__ out) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  double at[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) at[q] = Xg[(int64_t)(j * 64 + 16 * wv + q) * n + i * 64 + lane];
-  double* pout = part + (int64_t)T * 64;
   if constexpr (!TRANS) {
-    if (tid < 64) xs[tid] = tv[(int64_t)(a + j) * 64 + tid];
-    __syncthreads();
     double rs = 0.0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) rs = fma(at[q], xs[16 * wv + q], rs);
     colp[wv][lane] = rs;
     __syncthreads();
-    if (tid < 64) pout[tid] = (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
+    if (tid < 64) out[tid] = (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
   } else {
-    const double xr = tv[(int64_t)(a + i) * 64 + lane];
     double rv[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) rv[q] = at[q] * xr;
@@ -1330,59 +1275,101 @@ __global__ __launch_bounds__(256) void k_cg_xgemv(int a, int64_t n, const double
     if ((lane & 3) == 0) {
       const int col = 16 * wv + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
                       ((lane >> 2) & 1);
-      pout[col] = cs;
+      out[col] = cs;
     }
   }
 }
 
-// y_{a+b} (forward) or x_{a+b} (backward) = the fixed-order sum of that block's slots of
-// k_cg_xgemv; one work-group per output block, four threads per entry.
+// Step A.  Work-groups [0, pw): output block b of the previous group [pa, pa + pw) into
+// outv (and out2 where the entry is below nc).  Work-groups [pw, pw + npanel): panel tile
+// tiles[k] = (I, J); its vector block (J forward, I backward) comes from the previous
+// group's X-product slots when it lies in that group, else from vsrc.
 template <bool TRANS>
-__global__ __launch_bounds__(256) void k_cg_xreduce(int a, int w, const double* __restrict__ part,
-                                                    double* __restrict__ outv, double* __restrict__ out2,
-                                                    int64_t nc) {
+__global__ __launch_bounds__(256) void k_cg_step_a(int pa, int pw, const double* __restrict__ xpart,
+                                                   double* __restrict__ outv, double* __restrict__ out2,
+                                                   int64_t nc, const int32_t* __restrict__ tiles,
+                                                   const double* __restrict__ cpool,
+                                                   const int64_t* __restrict__ cra,
+                                                   const double* __restrict__ vsrc,
+                                                   double* __restrict__ part) {
   __shared__ double red[4][64];
-  const int b = blockIdx.x, e = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int k0 = TRANS ? b : 0, k1 = TRANS ? w : b + 1;  // slots of the block: i (bwd) or j (fwd)
-  double s0 = 0.0;
-  for (int k = k0 + q; k < k1; k += 4) {
-    const int T = TRANS ? k * (k + 1) / 2 + b : b * (b + 1) / 2 + k;
-    s0 += part[(int64_t)T * 64 + e];
+  __shared__ double xs[64];
+  __shared__ double colp[4][64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if ((int)blockIdx.x < pw) {
+    const int b = blockIdx.x;
+    cg_out_block<TRANS>(b, pw, xpart, red, xs);
+    if (tid < 64) {
+      const int64_t gi = int64_t(pa + b) * 64 + tid;
+      outv[gi] = xs[tid];
+      if (out2 && gi < nc) out2[gi] = xs[tid];
+    }
+    return;
   }
-  red[q][e] = s0;
-  __syncthreads();
-  if (q == 0) {
-    const double v = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
-    const int64_t gi = int64_t(a + b) * 64 + e;
-    outv[gi] = v;
-    if (out2 && gi < nc) out2[gi] = v;
+  const int k = blockIdx.x - pw;
+  const int I = tiles[2 * k], J = tiles[2 * k + 1];
+  const double* A = cpool + cra[I] + (int64_t)J * kTileElems;
+  double at[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) at[q] = A[(16 * wv + q) * 64 + lane];
+  const int vb = TRANS ? I : J;  // the vector block this tile multiplies
+  if (vb >= pa && vb < pa + pw) {
+    cg_out_block<TRANS>(vb - pa, pw, xpart, red, xs);
+  } else {
+    if (tid < 64) xs[tid] = vsrc[(int64_t)vb * 64 + tid];
+    __syncthreads();
   }
+  cg_tile_product<TRANS>(at, xs, xs[lane], colp, part + (int64_t)k * 64);
 }
 
-void launch_cg_forward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
-                       const int64_t* cra, const double* Xg, const double* rhs, int64_t nc,
-                       const int32_t* rowptr, double* part, int64_t slot0, double* t, double* xpart,
-                       double* y, hipStream_t st) {
-  if (npanel > 0)
-    hipLaunchKernelGGL(k_cg_panel<false>, dim3(npanel), dim3(256), 0, st, panel_tiles, cpool, cra, y,
-                       part + slot0 * 64);
-  hipLaunchKernelGGL(k_cg_reduce, dim3(w), dim3(256), 0, st, a, rhs, nc, rowptr, part, t);
-  hipLaunchKernelGGL(k_cg_xgemv<false>, dim3(w * (w + 1) / 2), dim3(256), 0, st, a, int64_t(64) * w, Xg, t,
-                     xpart);
-  hipLaunchKernelGGL(k_cg_xreduce<false>, dim3(w), dim3(256), 0, st, a, w, xpart, y, nullptr, int64_t(0));
+// Step B of the group [a, a + w), one work-group per lower tile (i, j) of X_g (row-major
+// enumeration T = i(i+1)/2 + j): t = base - panel slots of block a + j (forward) or
+// a + i (backward), then slot T = X_ij t (forward) or X_ij^T t (backward).
+template <bool TRANS>
+__global__ __launch_bounds__(256) void k_cg_step_b(int a, int64_t n, const double* __restrict__ Xg,
+                                                   const double* __restrict__ base, int64_t nb,
+                                                   const int32_t* __restrict__ ptr,
+                                                   const double* __restrict__ part,
+                                                   double* __restrict__ xpart) {
+  __shared__ double red[4][64];
+  __shared__ double xs[64];
+  __shared__ double colp[4][64];
+  const int T = blockIdx.x;
+  int i = int((sqrt(8.0 * double(T) + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= T) ++i;
+  while (i * (i + 1) / 2 > T) --i;
+  const int j = T - i * (i + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double at[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) at[q] = Xg[(int64_t)(j * 64 + 16 * wv + q) * n + i * 64 + lane];
+  cg_rhs_block(a + (TRANS ? i : j), base, nb, ptr, part, red, xs);
+  cg_tile_product<TRANS>(at, xs, xs[lane], colp, xpart + (int64_t)T * 64);
 }
 
-void launch_cg_backward(int a, int w, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
-                        const int64_t* cra, const double* Xg, const double* y, int64_t ncp,
-                        const int32_t* colptr, double* part, int64_t slot0, double* t, double* xpart,
-                        double* x, double* out, int64_t nc, hipStream_t st) {
-  if (npanel > 0)
-    hipLaunchKernelGGL(k_cg_panel<true>, dim3(npanel), dim3(256), 0, st, panel_tiles, cpool, cra, x,
-                       part + slot0 * 64);
-  hipLaunchKernelGGL(k_cg_reduce, dim3(w), dim3(256), 0, st, a, y, ncp, colptr, part, t);
-  hipLaunchKernelGGL(k_cg_xgemv<true>, dim3(w * (w + 1) / 2), dim3(256), 0, st, a, int64_t(64) * w, Xg, t,
-                     xpart);
-  hipLaunchKernelGGL(k_cg_xreduce<true>, dim3(w), dim3(256), 0, st, a, w, xpart, x, out, nc);
+void launch_cg_step_a(bool trans, int pa, int pw, const double* xpart, double* outv, double* out2,
+                      int64_t nc, const int32_t* panel_tiles, int64_t npanel, const double* cpool,
+                      const int64_t* cra, const double* vsrc, double* part, hipStream_t st) {
+  const int64_t nb = int64_t(pw) + npanel;
+  if (nb <= 0) return;
+  if (trans)
+    hipLaunchKernelGGL(k_cg_step_a<true>, dim3(nb), dim3(256), 0, st, pa, pw, xpart, outv, out2, nc,
+                       panel_tiles, cpool, cra, vsrc, part);
+  else
+    hipLaunchKernelGGL(k_cg_step_a<false>, dim3(nb), dim3(256), 0, st, pa, pw, xpart, outv, out2, nc,
+                       panel_tiles, cpool, cra, vsrc, part);
+}
+
+void launch_cg_step_b(bool trans, int a, int w, const double* Xg, const double* base, int64_t nb,
+                      const int32_t* ptr, const double* part, double* xpart, hipStream_t st) {
+  if (w <= 0) return;
+  const int64_t n = int64_t(64) * w;
+  if (trans)
+    hipLaunchKernelGGL(k_cg_step_b<true>, dim3(w * (w + 1) / 2), dim3(256), 0, st, a, n, Xg, base, nb, ptr,
+                       part, xpart);
+  else
+    hipLaunchKernelGGL(k_cg_step_b<false>, dim3(w * (w + 1) / 2), dim3(256), 0, st, a, n, Xg, base, nb, ptr,
+                       part, xpart);
 }
 
 // ---------------------------------------------------------------------------
