@@ -121,13 +121,16 @@ struct fetidp_solver {
   // deluxe scaling (interface faces)
   bool deluxe = false;
   DBuf<FaceDev> face_dev;
-  DBuf<DevSub> face_sub;
   DBuf<int32_t> face_map_lam, face_map_self, d_self;
   DBuf<int64_t> face_slot0, face_slot1;
-  DBuf<SymTile> face_tiles;
   DBuf<TileWork> face_items;
   DBuf<int64_t> face_rowbase, face_rowabs, face_inv_base;
-  DBuf<double> fpool, finv, fops, fpart, fv1, fv2, xslot;
+  DBuf<double> fpool, finv, fops, fv1, fv2, xslot;
+  // D_F of every face: FP32 (the default; M^-1 = sum R^T S R keeps its symmetric
+  // positive semidefinite form whatever R's rounding) or FP64 (precond_precision 2)
+  DBuf<float> fD32;
+  DBuf<double> fD64;
+  bool d64 = false;
   DBuf<int> ffail;
   DBuf<int32_t> vec_dof;
   DBuf<double> vec_w;
@@ -547,12 +550,10 @@ struct fetidp_solver {
     {
       const FacePlan& F = plans.faces;
       face_dev.upload(F.dev);
-      face_sub.upload(F.sub);
       face_map_lam.upload(F.map_lam);
       face_map_self.upload(F.map_self);
       face_slot0.upload(F.slot0);
       face_slot1.upload(F.slot1);
-      face_tiles.upload(F.tiles);
       face_items.upload(F.sched.items);
       face_rowbase.upload(F.rowbase);
       face_rowabs.upload(F.rowabs);
@@ -977,6 +978,7 @@ struct fetidp_solver {
     fell_back = false;
     deluxe = use_deluxe() && plans.faces.n > 0;
     p64 = precond_fp64(deluxe);
+    d64 = cfg.precond_precision == 2;
     if (fp32() && fp32_estimate() > kFp32Limit) {
       // the FP32 solve could not reach the FP64 tolerance in a useful number of outer
       // steps: factor in FP64 from the start
@@ -1034,13 +1036,12 @@ struct fetidp_solver {
     launch_lambda_scatter(topo.n_lam, lam_pos.p, d_sgn.p, dbuf.p, qout, st);
   }
   // Per face: [[A_F, I], [I, 0]] from the FP64 Dirichlet blocks, eliminate A_F with the
-  // batched tile kernels, keep A_F^-1 and B_F as lower tiles.
+  // batched tile kernels, keep A_F^-1 and B_F as lower tiles and form D_F = A_F^-1 B_F.
   void deluxe_setup() {
     const FacePlan& F = plans.faces;
     fpool.alloc(size_t(F.pool_elems));
     finv.alloc(size_t(std::max<int64_t>(F.inv_elems, 1)));
     fops.alloc(size_t(F.op_elems));
-    fpart.alloc(size_t(std::max<int64_t>(F.sp_elems, 1)));
     fv1.alloc(size_t(F.vec_elems));
     fv2.alloc(size_t(F.vec_elems));
     ffail.alloc(size_t(F.n));
@@ -1053,26 +1054,32 @@ struct fetidp_solver {
     run_schedule<double>(F.sched, face_items, face_rowbase, face_rowabs, face_inv_base, fpool.p, finv.p,
                          ffail.p, false);
     launch_face_extract(F.n, F.max_T, face_dev.p, devsub.p, Pd.p, face_slot1.p, fpool.p, fops.p, st);
+    if (d64) {
+      fD32.release();
+      fD64.alloc(size_t(std::max<int64_t>(F.d_elems, 1)));
+      launch_face_D<double>(F.n, F.max_T, face_dev.p, fops.p, fD64.p, st);
+    } else {
+      fD64.release();
+      fD32.alloc(size_t(std::max<int64_t>(F.d_elems, 1)));
+      launch_face_D<float>(F.n, F.max_T, face_dev.p, fops.p, fD32.p, st);
+    }
   }
 
   // z = M^-1 r with deluxe weights: x_s0 = D_F r, x_s1 = D_F r - r (D_F = A_F^-1 B_F),
   // y = S_DD x per subdomain, z = D_F^T (y_s0 + y_s1) - y_s1 on every face.
   void apply_M_deluxe(const double* rin, double* zout) {
     const FacePlan& F = plans.faces;
-    const int64_t nt = int64_t(F.tiles.size());
-    launch_symv<double>(nt, face_tiles.p, int(F.n), F.max_T, face_sub.p, fops.p, 0, face_map_lam.p, nullptr,
-                        rin, fpart.p, fv1.p, st);                                  // B r
-    launch_symv<double>(nt, face_tiles.p, int(F.n), F.max_T, face_sub.p, fops.p, 1, face_map_self.p, nullptr,
-                        fv1.p, fpart.p, fv2.p, st);                                // A^-1 B r
+    auto dmv = [&](bool trans, const int32_t* map, const double* src, double* out) {
+      if (d64) launch_face_dmv<double>(trans, F.n, F.max_T, face_dev.p, fD64.p, map, src, out, st);
+      else launch_face_dmv<float>(trans, F.n, F.max_T, face_dev.p, fD32.p, map, src, out, st);
+    };
+    dmv(false, face_map_lam.p, rin, fv2.p);                                         // D r
     launch_deluxe_split(F.vec_elems, face_map_lam.p, face_slot0.p, face_slot1.p, fv2.p, rin, xslot.p, st);
     launch_symv<double>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, Pd.p, 1,
                         d_self.p, nullptr, xslot.p, sympart.p, dbuf.p, st);       // y = S x
     launch_deluxe_sum(F.vec_elems, face_map_lam.p, face_slot0.p, face_slot1.p, dbuf.p, fv1.p, st);
-    launch_symv<double>(nt, face_tiles.p, int(F.n), F.max_T, face_sub.p, fops.p, 1, face_map_self.p, nullptr,
-                        fv1.p, fpart.p, fv2.p, st);                                // A^-1 u
-    launch_symv<double>(nt, face_tiles.p, int(F.n), F.max_T, face_sub.p, fops.p, 0, face_map_self.p, nullptr,
-                        fv2.p, fpart.p, fv1.p, st);                                // B A^-1 u
-    launch_deluxe_out(F.vec_elems, face_map_lam.p, face_slot1.p, fv1.p, dbuf.p, zout, st);
+    dmv(true, face_map_self.p, fv1.p, fv2.p);                                       // D^T u
+    launch_deluxe_out(F.vec_elems, face_map_lam.p, face_slot1.p, fv2.p, dbuf.p, zout, st);
   }
 
   // W x with x = B^T p gathered per subdomain: Z x as per-tile partials in sympart (summed
@@ -1193,7 +1200,7 @@ struct fetidp_solver {
     if (head_queued) throw std::logic_error("graph iterations form their own W p");
     const void* key[3] = {W.p, p64 ? (const void*)Pd.p : (const void*)Pf.p, cpool.p};
     // the precision flags too: a released buffer's address can come back for the other one
-    const int key_t = int(sizeof(T)) | (deluxe ? 256 : 0) | (p64 ? 512 : 0);
+    const int key_t = int(sizeof(T)) | (deluxe ? 256 : 0) | (p64 ? 512 : 0) | (deluxe && d64 ? 1024 : 0);
     if (!pcpg_exec || pcpg_key_t != key_t || std::memcmp(key, pcpg_key, sizeof(key))) {
       if (pcpg_exec) HIPCHECK(hipGraphExecDestroy(pcpg_exec));
       pcpg_exec = nullptr;

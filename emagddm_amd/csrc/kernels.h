@@ -163,6 +163,15 @@ void launch_face_setup(int64_t nfaces, int max_tf, const FaceDev* faces, const D
 void launch_face_extract(int64_t nfaces, int max_tf, const FaceDev* faces, const DevSub* subs,
                          const double* P, const int64_t* slot1, const double* fpool, double* ops,
                          hipStream_t st);
+// D_F = A_F^-1 B_F of every face from the lower-tile stores (FP64 product, rounded to T).
+template <typename T>
+void launch_face_D(int64_t nfaces, int max_tf, const FaceDev* faces, const double* ops, T* D,
+                   hipStream_t st);
+// out[vec_off + k] = (D_F x)_k, or (D_F^T x)_k when trans, with x_k = src[map[vec_off + k]]
+// (0 where map < 0), on every face.
+template <typename T>
+void launch_face_dmv(bool trans, int64_t nfaces, int max_tf, const FaceDev* faces, const T* D,
+                     const int32_t* map, const double* src, double* out, hipStream_t st);
 void launch_deluxe_split(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
                          const double* g, const double* r, double* xs, hipStream_t st);
 void launch_deluxe_sum(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,

@@ -2447,6 +2447,164 @@ void launch_face_extract(int64_t nfaces, int max_tf, const FaceDev* faces, const
                      P, slot1, fpool, ops);
 }
 
+// Element (r, c) of a symmetric operator stored as lower tiles, row-major inside a tile.
+__device__ __forceinline__ double sym_get(const double* __restrict__ M, int r, int c) {
+  if (r < c) {
+    const int t = r;
+    r = c;
+    c = t;
+  }
+  const int64_t I = r >> 6, J = c >> 6;
+  return M[(I * (I + 1) / 2 + J) * kTileElems + (r & 63) * 64 + (c & 63)];
+}
+
+// D_F = A_F^-1 B_F of every face as full tiles (row-major inside a tile), one work-group
+// per output tile (i, j) of a face: D_ij = sum_k (A^-1)_ik B_kj in FP64, the two input
+// tiles staged in LDS, then rounded to T.  Thread: column = lane, rows 16 w .. 16 w + 15.
+template <typename T>
+__global__ __launch_bounds__(256) void k_face_D(int max_tf, const FaceDev* __restrict__ faces,
+                                                const double* __restrict__ ops, T* __restrict__ D) {
+  const FaceDev F = faces[blockIdx.y];
+  const int tf = F.tf;
+  const int i = int(blockIdx.x) / max_tf, j = int(blockIdx.x) % max_tf;  // grid.x = max_tf^2
+  if (i >= tf || j >= tf) return;
+  __shared__ double sA[64 * 64], sB[64 * 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* Ai = ops + F.ainv_off;
+  const double* Bm = ops + F.b_off;
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+  for (int k = 0; k < tf; ++k) {
+    for (int e = tid; e < 4096; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      sA[e] = sym_get(Ai, 64 * i + r, 64 * k + c);
+      sB[e] = sym_get(Bm, 64 * k + r, 64 * j + c);
+    }
+    __syncthreads();
+    for (int m = 0; m < 64; ++m) {
+      const double b = sB[m * 64 + lane];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc[q] = fma(sA[(16 * wv + q) * 64 + m], b, acc[q]);
+    }
+    __syncthreads();
+  }
+  T* out = D + F.d_off + ((int64_t)i * tf + j) * kTileElems;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) out[(16 * wv + q) * 64 + lane] = (T)acc[q];
+}
+
+template <typename T>
+void launch_face_D(int64_t nfaces, int max_tf, const FaceDev* faces, const double* ops, T* D,
+                   hipStream_t st) {
+  if (nfaces <= 0 || max_tf <= 0) return;
+  hipLaunchKernelGGL(k_face_D<T>, dim3(max_tf * max_tf, nfaces), dim3(256), 0, st, max_tf, faces, ops, D);
+}
+template void launch_face_D<float>(int64_t, int, const FaceDev*, const double*, float*, hipStream_t);
+template void launch_face_D<double>(int64_t, int, const FaceDev*, const double*, double*, hipStream_t);
+
+// out = D_F x (TRANS: D_F^T x) on every face, one work-group per (output block, face):
+// x_k = src[map[vec_off + k]] (0 where map < 0), out[vec_off + k].  Thread: column = lane,
+// rows 16 w .. 16 w + 15 of each tile; FP64 sums over the tiles of the block row (column)
+// in a fixed order, then over the lanes (D x: halving butterfly) or the waves (D^T x).
+template <typename T, bool TRANS>
+__global__ __launch_bounds__(256) void k_face_dmv(const FaceDev* __restrict__ faces,
+                                                  const T* __restrict__ D,
+                                                  const int32_t* __restrict__ map,
+                                                  const double* __restrict__ src,
+                                                  double* __restrict__ out) {
+  __shared__ double xs[64];
+  __shared__ double colp[4][64];
+  const FaceDev F = faces[blockIdx.y];
+  const int tf = F.tf, ob = blockIdx.x;
+  if (ob >= tf) return;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const T* Db = D + F.d_off;
+  const int32_t* mp = map + F.vec_off;
+  if constexpr (!TRANS) {
+    // row r = 16 w + q of block row ob: sum over column blocks jb and columns c = lane
+    double rv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) rv[q] = 0.0;
+    for (int jb = 0; jb < tf; ++jb) {
+      const T* A = Db + ((int64_t)ob * tf + jb) * kTileElems;
+      T at[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) at[q] = A[(16 * wv + q) * 64 + lane];
+      const int32_t m = mp[jb * 64 + lane];
+      const double xc = m >= 0 ? src[m] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) rv[q] = fma((double)at[q], xc, rv[q]);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const bool hi = lane & 32;
+      const double keep = hi ? rv[u + 8] : rv[u], send = hi ? rv[u] : rv[u + 8];
+      rv[u] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool hi = lane & 16;
+      const double keep = hi ? rv[u + 4] : rv[u], send = hi ? rv[u] : rv[u + 4];
+      rv[u] = keep + __shfl_xor(send, 16);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const bool hi = lane & 8;
+      const double keep = hi ? rv[u + 2] : rv[u], send = hi ? rv[u] : rv[u + 2];
+      rv[u] = keep + __shfl_xor(send, 8);
+    }
+    {
+      const bool hi = lane & 4;
+      const double keep = hi ? rv[1] : rv[0], send = hi ? rv[0] : rv[1];
+      rv[0] = keep + __shfl_xor(send, 4);
+    }
+    double rs = rv[0];
+    rs += __shfl_xor(rs, 2);
+    rs += __shfl_xor(rs, 1);
+    if ((lane & 3) == 0) {
+      const int row = 16 * wv + ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                      ((lane >> 2) & 1);
+      out[F.vec_off + ob * 64 + row] = rs;
+    }
+  } else {
+    // column c = lane of block column ob: sum over row blocks ib and rows 16 w + q
+    double cs = 0.0;
+    for (int ib = 0; ib < tf; ++ib) {
+      const T* A = Db + ((int64_t)ib * tf + ob) * kTileElems;
+      T at[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) at[q] = A[(16 * wv + q) * 64 + lane];
+      if (tid < 64) {
+        const int32_t m = mp[ib * 64 + tid];
+        xs[tid] = m >= 0 ? src[m] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) cs = fma((double)at[q], xs[16 * wv + q], cs);
+      __syncthreads();
+    }
+    colp[wv][lane] = cs;
+    __syncthreads();
+    if (tid < 64)
+      out[F.vec_off + ob * 64 + tid] = (colp[0][tid] + colp[1][tid]) + (colp[2][tid] + colp[3][tid]);
+  }
+}
+
+template <typename T>
+void launch_face_dmv(bool trans, int64_t nfaces, int max_tf, const FaceDev* faces, const T* D,
+                     const int32_t* map, const double* src, double* out, hipStream_t st) {
+  if (nfaces <= 0 || max_tf <= 0) return;
+  if (trans)
+    hipLaunchKernelGGL((k_face_dmv<T, true>), dim3(max_tf, nfaces), dim3(256), 0, st, faces, D, map, src, out);
+  else
+    hipLaunchKernelGGL((k_face_dmv<T, false>), dim3(max_tf, nfaces), dim3(256), 0, st, faces, D, map, src, out);
+}
+template void launch_face_dmv<float>(bool, int64_t, int, const FaceDev*, const float*, const int32_t*,
+                                     const double*, double*, hipStream_t);
+template void launch_face_dmv<double>(bool, int64_t, int, const FaceDev*, const double*, const int32_t*,
+                                      const double*, double*, hipStream_t);
+
 // x (D slots) from g = D_F r on every face: side s0 takes g, side s1 takes g - r
 __global__ __launch_bounds__(256) void k_deluxe_split(int64_t nv, const int32_t* __restrict__ map_lam,
                                                       const int64_t* __restrict__ slot0,

@@ -178,7 +178,7 @@ void build_faces(const Topology& T, Plans& P) {
     key[l] = {{slot_sub[T.lam_pos[2 * l]], slot_sub[T.lam_pos[2 * l + 1]]}, int32_t(l)};
   std::sort(key.begin(), key.end());
   std::vector<int> tp_list;
-  int64_t vec = 0, op = 0, sp = 0, pool = 0, inv = 0;
+  int64_t vec = 0, op = 0, pool = 0, inv = 0, dd = 0;
   for (int64_t a = 0; a < nl;) {
     int64_t b = a;
     while (b < nl && key[b].first == key[a].first) ++b;
@@ -192,15 +192,7 @@ void build_faces(const Topology& T, Plans& P) {
     d.pool_base = pool;
     d.ainv_off = op;
     d.b_off = op + sym_tile_elems(nfp);
-    DevSub u{};
-    u.ndp = nfp;
-    u.nd = d.nf;
-    u.dvec_off = vec;
-    u.y_off = vec;
-    u.w_off = d.b_off;
-    u.p_off = d.ainv_off;
-    u.sp_off = sp;
-    const int f = int(F.dev.size());
+    d.d_off = dd;
     for (int i = 0; i < nfp; ++i) {
       const bool real = i < d.nf;
       const int32_t l = real ? key[a + i].second : -1;
@@ -209,8 +201,6 @@ void build_faces(const Topology& T, Plans& P) {
       F.slot0.push_back(real ? T.lam_pos[2 * l] : -1);
       F.slot1.push_back(real ? T.lam_pos[2 * l + 1] : -1);
     }
-    for (int I = 0; I < d.tf; ++I)
-      for (int J = 0; J <= I; ++J) F.tiles.push_back(SymTile{f, I, J});
     // [[A, I], [I, 0]]: 2 tf tile rows, dense lower tiles, tile (I, J) at base + (I(I+1)/2 + J) 4096
     const int ta = 2 * d.tf;
     F.rowbase.push_back(int64_t(F.rowabs.size()));
@@ -218,21 +208,20 @@ void build_faces(const Topology& T, Plans& P) {
     F.inv_off.push_back(inv);
     vec += nfp;
     op += 2 * sym_tile_elems(nfp);
-    sp += int64_t(d.tf) * d.tf * kTile;
     pool += int64_t(ta) * (ta + 1) / 2 * kTileElems;
     inv += int64_t(d.tf) * kTileElems;
+    dd += int64_t(d.tf) * d.tf * kTileElems;
     F.max_T = std::max(F.max_T, d.tf);
     tp_list.push_back(d.tf);
     F.dev.push_back(d);
-    F.sub.push_back(u);
     a = b;
   }
   F.n = int64_t(F.dev.size());
   F.vec_elems = vec;
   F.op_elems = op;
-  F.sp_elems = sp;
   F.pool_elems = pool;
   F.inv_elems = inv;
+  F.d_elems = dd;
   F.sched = batched_dense_schedule(tp_list);
 }
 }  // namespace
@@ -869,7 +858,12 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
       const int ta = 2 * F.dev[w.mat].tf;
       check(w.J <= w.I && w.I < ta && w.k1 <= w.J && w.k0 >= 0, "face tile");
     }
-    for (const SymTile& t : F.tiles) check(t.s >= 0 && t.s < F.n && t.J <= t.I && t.I < F.dev[t.s].tf, "face sym tile");
+    int64_t dd = 0;
+    for (const FaceDev& d : F.dev) {
+      check(d.tf >= 1 && d.nf <= d.tf * kTile && d.d_off == dd, "face D_F offset");
+      dd += int64_t(d.tf) * d.tf * kTileElems;
+    }
+    check(dd == F.d_elems, "face D_F size");
     int64_t nreal = 0;
     for (int32_t l : F.map_lam) nreal += l >= 0;
     check(nreal == T.n_lam, "faces cover every multiplier once");

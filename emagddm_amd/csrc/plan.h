@@ -49,19 +49,16 @@ struct TileSchedule {
 // Interface faces for deluxe scaling: the multipliers shared by one subdomain pair
 // (s0 < s1).  Per face F the solver forms A_F = S_s0,FF + S_s1,FF from the two
 // Dirichlet Schur blocks, inverts it by a partial Cholesky of [[A_F, I], [I, 0]]
-// (trailing block -A_F^-1) and keeps B_F = S_s1,FF; the scaled jump operator then
-// applies D_F = A_F^-1 B_F and its transpose through symmetric tile products.
+// (trailing block -A_F^-1), keeps B_F = S_s1,FF and forms D_F = A_F^-1 B_F explicitly;
+// the scaled jump operator applies D_F and its transpose.
 struct FacePlan {
   int64_t n = 0;
   std::vector<FaceDev> dev;
-  std::vector<DevSub> sub;            // per face, for the symmetric-product kernels:
-                                      //   ndp = padded order, dvec_off = y_off = vec_off,
-                                      //   w_off = b_off, p_off = ainv_off, sp_off
   std::vector<int32_t> map_lam;       // face-vector slot -> multiplier (-1 padding)
   std::vector<int32_t> map_self;      // face-vector slot -> itself (-1 padding)
   std::vector<int64_t> slot0, slot1;  // face-vector slot -> D slot in s0 / s1 (-1 padding)
-  std::vector<SymTile> tiles;
-  int64_t vec_elems = 0, op_elems = 0, sp_elems = 0, pool_elems = 0, inv_elems = 0;
+  int64_t vec_elems = 0, op_elems = 0, pool_elems = 0, inv_elems = 0;
+  int64_t d_elems = 0;                // explicit D_F of every face (tf^2 tiles each)
   int max_T = 0;                      // max padded order / 64
   std::vector<int64_t> rowabs, rowbase, inv_off;
   TileSchedule sched;                 // eliminates the A_F block of every face

@@ -39,6 +39,7 @@ struct FaceDev {
   int32_t s0, s1, nf, tf;       // subdomains, multipliers, padded order / 64
   int64_t pool_base, vec_off;   // factor pool (dense lower tiles), face-vector offset
   int64_t ainv_off, b_off;      // lower-tile stores of A_F^-1 and B_F
+  int64_t d_off;                // D_F = A_F^-1 B_F, tf x tf full tiles (row-major tiles)
 };
 
 // One lower tile (I, J) of subdomain s's symmetric dual operators (Z and P).

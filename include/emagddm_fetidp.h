@@ -50,7 +50,10 @@ typedef struct fetidp_config {
   int32_t graphs;           /* 1: replay each PCPG iteration as one captured HIP graph
                                (ignored while event profiling is on) */
   int32_t precond_precision; /* Dirichlet blocks: 0 auto (FP64 when max nu / (min kappa
-                               h_min^2) > 1e6, else FP32), 1 FP32, 2 FP64 */
+                               h_min^2) > 1e6, else FP32), 1 FP32, 2 FP64.  With deluxe
+                               scaling the blocks are FP64 and this selects the face
+                               operators D_F = A_F^-1 B_F: FP32 (0, 1; formed in FP64) or
+                               FP64 (2) */
   int32_t scaling;          /* interface scaling of the preconditioner: 0 auto (deluxe when
                                kappa/nu varies by more than 1e4 between materials or when
                                neighbouring subdomains' bounding boxes overlap by more
