@@ -936,6 +936,14 @@ struct fetidp_solver {
       HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
       launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st3);
       launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st3);
+      if (deluxe) {
+        // the face factorizations only read the Dirichlet blocks: on st3 too, beside the
+        // coarse chain (whose dependent single-matrix launches leave most of the GPU idle)
+        const hipStream_t main = st;
+        st = st3;
+        deluxe_setup();
+        st = main;
+      }
       HIPCHECK(hipEventRecord(ev_extract, st3));
     }
     // coarse problem in FP64
@@ -949,7 +957,7 @@ struct fetidp_solver {
         launch_cg_invert(plans.coarse_group, plans.cg_inv_step.data(), cg_inv_items_d.p, cpool.p, coarse_rowabs.p, coarse_ftile.p, cinv.p, cg_start_d.p,
                          cg_group_of_d.p, cg_xoff_d.p, cgX.p, st);
       });
-    if (deluxe) deluxe_setup();
+    if (deluxe && profiling()) deluxe_setup();
     if (!profiling()) HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
     HIPCHECK(hipStreamSynchronize(st));
     prof_collect();

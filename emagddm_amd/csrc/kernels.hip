@@ -2447,51 +2447,57 @@ void launch_face_extract(int64_t nfaces, int max_tf, const FaceDev* faces, const
                      P, slot1, fpool, ops);
 }
 
-// Element (r, c) of a symmetric operator stored as lower tiles, row-major inside a tile.
-__device__ __forceinline__ double sym_get(const double* __restrict__ M, int r, int c) {
-  if (r < c) {
-    const int t = r;
-    r = c;
-    c = t;
+// Tile (i, k) of a symmetric operator stored as lower tiles (row-major inside a tile) into
+// LDS (leading dimension kLd), read coalesced: stored tile (i, k) as is when i >= k, else
+// stored tile (k, i) transposed.
+__device__ __forceinline__ void stage_sym_tile(const double* __restrict__ M, int i, int k, double* sX) {
+  const bool tr = i < k;
+  const int64_t I = tr ? k : i, K = tr ? i : k;
+  const double* src = M + (I * (I + 1) / 2 + K) * kTileElems;
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int a = e >> 6, b = e & 63;  // stored (row a, column b)
+    const double v = src[e];
+    if (tr) sX[b * kLd + a] = v;
+    else sX[a * kLd + b] = v;
   }
-  const int64_t I = r >> 6, J = c >> 6;
-  return M[(I * (I + 1) / 2 + J) * kTileElems + (r & 63) * 64 + (c & 63)];
 }
 
 // D_F = A_F^-1 B_F of every face as full tiles (row-major inside a tile), one work-group
-// per output tile (i, j) of a face: D_ij = sum_k (A^-1)_ik B_kj in FP64, the two input
-// tiles staged in LDS, then rounded to T.  Thread: column = lane, rows 16 w .. 16 w + 15.
+// per output tile (i, j) of a face: D_ij = sum_k (A^-1)_ik B_kj on FP64 MFMA (wave w
+// holds output rows 16 w .. 16 w + 15, four 16x16 blocks), the two input tiles of each k
+// staged in LDS; then rounded to T.
 template <typename T>
 __global__ __launch_bounds__(256) void k_face_D(int max_tf, const FaceDev* __restrict__ faces,
                                                 const double* __restrict__ ops, T* __restrict__ D) {
+  using acc_t = Mfma<double>::acc_t;
   const FaceDev F = faces[blockIdx.y];
   const int tf = F.tf;
   const int i = int(blockIdx.x) / max_tf, j = int(blockIdx.x) % max_tf;  // grid.x = max_tf^2
   if (i >= tf || j >= tf) return;
-  __shared__ double sA[64 * 64], sB[64 * 64];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const double* Ai = ops + F.ainv_off;
-  const double* Bm = ops + F.b_off;
-  double acc[16];
+  __shared__ double sA[64 * kLd], sB[64 * kLd];
+  const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6;
+  const int lr = l & 15, lk = l >> 4;
+  acc_t acc[4];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+  for (int cb = 0; cb < 4; ++cb) acc[cb] = acc_t{0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < tf; ++k) {
-    for (int e = tid; e < 4096; e += 256) {
-      const int r = e >> 6, c = e & 63;
-      sA[e] = sym_get(Ai, 64 * i + r, 64 * k + c);
-      sB[e] = sym_get(Bm, 64 * k + r, 64 * j + c);
-    }
+    stage_sym_tile(ops + F.ainv_off, i, k, sA);
+    stage_sym_tile(ops + F.b_off, k, j, sB);
     __syncthreads();
-    for (int m = 0; m < 64; ++m) {
-      const double b = sB[m * 64 + lane];
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const double a = sA[(16 * wv + lr) * kLd + kk + lk];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) acc[q] = fma(sA[(16 * wv + q) * 64 + m], b, acc[q]);
+      for (int cb = 0; cb < 4; ++cb) acc[cb] = Mfma<double>::op(a, sB[(kk + lk) * kLd + 16 * cb + lr], acc[cb]);
     }
     __syncthreads();
   }
   T* out = D + F.d_off + ((int64_t)i * tf + j) * kTileElems;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) out[(16 * wv + q) * 64 + lane] = (T)acc[q];
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      out[(16 * wv + Mfma<double>::row(lk, r)) * 64 + 16 * cb + lr] = (T)acc[cb][r];
 }
 
 template <typename T>
