@@ -1,4 +1,4 @@
-// Learned lambda0 network: gfx950 kernels (bf16 MFMA 16x16x16, FP32 accumulate)
+// Learned lambda0 network: gfx950 kernels (bf16 MFMA 16x16x32, FP32 accumulate)
 // and the C ABI of include/emagddm_mpnn.h.
 //
 // Every reduction is a fixed-order sum (per-block partials, then a fixed-order
@@ -27,7 +27,8 @@ namespace {
       throw std::runtime_error(std::string(#x) + " failed: " + hipGetErrorString(e_));    \
   } while (0)
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bfloat16, round to nearest even (NaN kept quiet)
@@ -39,22 +40,24 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
-// D[4(l/16)+j][l%16] += sum_k A[l%16][k] B[k][l%16] with lane l holding
-// A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16] (checked by emag_mpnn_mfma_probe)
-__device__ __forceinline__ f32x4 mfma_bf16(bf16x4 a, bf16x4 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+// D[4(l/16)+j][l%16] += sum_k A[l%16][k] B[k][l%16] over k < 32 with lane l holding
+// A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16], j < 8 (checked by emag_mpnn_mfma_probe)
+__device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a),
+                                                 __builtin_bit_cast(bf16v8, b), c, 0, 0, 0);
 }
 
-__device__ __forceinline__ bf16x4 pack4(float x, float y, float z, float w) {
-  return bf16x4{(short)f2bf(x), (short)f2bf(y), (short)f2bf(z), (short)f2bf(w)};
+__device__ __forceinline__ bf16x8 pack8(float4 x, float4 y) {
+  return bf16x8{(short)f2bf(x.x), (short)f2bf(x.y), (short)f2bf(x.z), (short)f2bf(x.w),
+                (short)f2bf(y.x), (short)f2bf(y.y), (short)f2bf(y.z), (short)f2bf(y.w)};
 }
 
 // ---------------------------------------------------------------------------
 // Row GEMM: out = [A0 | A1] (N x Kt, FP32, rounded to bf16 on load) times B
-// (Kt x NC, given as Bt[n][k] in bf16).  One workgroup = 4 waves x 32 rows.
-// Bt is staged once per workgroup in LDS with the 8-byte k-chunks of row n
-// XOR-swizzled by n, so the 16 lanes that read one k-chunk of 16 rows hit
-// distinct banks.  LDS = NC * Kt * 2 bytes <= 64 KiB.
+// (Kt x NC, given as Bt[n][k] in bf16).  One workgroup = 4 waves x 32 rows; K steps of
+// 32 (k >= Kt read as zero; K0 and Kt are multiples of 8).  Bt is staged once per
+// workgroup in LDS with the 16-byte k-chunks of row n XOR-swizzled by n, so the 16 lanes
+// that read one k-chunk of 16 rows hit distinct banks.  LDS = NC * Kt * 2 bytes <= 64 KiB.
 // ---------------------------------------------------------------------------
 enum { kEpiStore = 0, kEpiTanhRes = 1, kEpiBwd = 2 };
 constexpr int kMmWaveRows = 32, kMmRows = 128;
@@ -68,11 +71,11 @@ __global__ __launch_bounds__(256) void k_mm(int64_t N, const float* __restrict__
                                             float* __restrict__ out0, float* __restrict__ out1,
                                             const float* __restrict__ aux) {
   extern __shared__ unsigned short sB[];
-  const int Kt = K0 + K1, nq = Kt / 4, qmask = (nq < 16 ? nq : 16) - 1;
+  const int Kt = K0 + K1, nq = Kt / 8, qmask = (nq < 16 ? nq : 16) - 1;
   for (int e = threadIdx.x; e < NC * nq; e += 256) {
     const int n = e / nq, q = e % nq;
-    const uint2 v = *reinterpret_cast<const uint2*>(Bt + (int64_t)n * Kt + 4 * q);
-    *reinterpret_cast<uint2*>(sB + n * Kt + 4 * swz(n, q, qmask)) = v;
+    const uint4 v = *reinterpret_cast<const uint4*>(Bt + (int64_t)n * Kt + 8 * q);
+    *reinterpret_cast<uint4*>(sB + n * Kt + 8 * swz(n, q, qmask)) = v;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -85,23 +88,27 @@ __global__ __launch_bounds__(256) void k_mm(int64_t N, const float* __restrict__
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int c = 0; c < NT; ++c) acc[h][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < Kt; k0 += 16) {
-    const int k = k0 + 4 * lk;
-    bf16x4 a[2];
+  for (int k0 = 0; k0 < Kt; k0 += 32) {
+    const int k = k0 + 8 * lk;
+    const bool kin = k < Kt;
+    bf16x8 a[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int64_t row = row0 + 16 * h + lr;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (row < N)
-        v = k < K0 ? *reinterpret_cast<const float4*>(A0 + row * K0 + k)
-                   : *reinterpret_cast<const float4*>(A1 + row * K1 + (k - K0));
-      a[h] = pack4(v.x, v.y, v.z, v.w);
+      float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+      if (row < N && kin) {
+        const float* src = k < K0 ? A0 + row * K0 + k : A1 + row * K1 + (k - K0);
+        v0 = *reinterpret_cast<const float4*>(src);
+        v1 = *reinterpret_cast<const float4*>(src + 4);
+      }
+      a[h] = pack8(v0, v1);
     }
-    const int q = k >> 2;
+    const int q = k >> 3;
 #pragma unroll
     for (int c = 0; c < NT; ++c) {
       const int n = 16 * c + lr;
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(sB + n * Kt + 4 * swz(n, q, qmask));
+      bf16x8 b = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (kin) b = *reinterpret_cast<const bf16x8*>(sB + n * Kt + 8 * swz(n, q, qmask));
       acc[0][c] = mfma_bf16(a[0], b, acc[0][c]);
       acc[1][c] = mfma_bf16(a[1], b, acc[1][c]);
     }
@@ -276,12 +283,12 @@ __global__ __launch_bounds__(256) void k_wgrad(int64_t N, int64_t rows_per_blk,
     }
     __syncthreads();
 #pragma unroll
-    for (int ii = 0; ii < kWgChunk; ii += 16) {
+    for (int ii = 0; ii < kWgChunk; ii += 32) {
 #pragma unroll
       for (int t = 0; t < TW; ++t) {
         const int tile = wv * TW + t, kt = tile / 8, nt = tile % 8;
-        const bf16x4 a = *reinterpret_cast<const bf16x4*>(sA + (kt * 16 + lr) * kWgLd + ii + 4 * lk);
-        const bf16x4 b = *reinterpret_cast<const bf16x4*>(sG + (nt * 16 + lr) * kWgLd + ii + 4 * lk);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(sA + (kt * 16 + lr) * kWgLd + ii + 8 * lk);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(sG + (nt * 16 + lr) * kWgLd + ii + 8 * lk);
         acc[t] = mfma_bf16(a, b, acc[t]);
       }
     }
@@ -353,11 +360,13 @@ __global__ __launch_bounds__(256) void k_refresh_bf(const float* __restrict__ w,
   bf[e] = f2bf(v);
 }
 
-// probe: lane l feeds a_frag/b_frag[4l..4l+3] and returns its accumulators
+// probe: lane l feeds a_frag/b_frag[8l..8l+7] and returns its accumulators
 __global__ void k_mpnn_probe(const float* a, const float* b, float* d) {
   const int l = threadIdx.x;
-  const bf16x4 av = pack4(a[4 * l], a[4 * l + 1], a[4 * l + 2], a[4 * l + 3]);
-  const bf16x4 bv = pack4(b[4 * l], b[4 * l + 1], b[4 * l + 2], b[4 * l + 3]);
+  const bf16x8 av = pack8(make_float4(a[8 * l], a[8 * l + 1], a[8 * l + 2], a[8 * l + 3]),
+                          make_float4(a[8 * l + 4], a[8 * l + 5], a[8 * l + 6], a[8 * l + 7]));
+  const bf16x8 bv = pack8(make_float4(b[8 * l], b[8 * l + 1], b[8 * l + 2], b[8 * l + 3]),
+                          make_float4(b[8 * l + 4], b[8 * l + 5], b[8 * l + 6], b[8 * l + 7]));
   const f32x4 r = mfma_bf16(av, bv, f32x4{0.f, 0.f, 0.f, 0.f});
   for (int j = 0; j < 4; ++j) d[4 * l + j] = r[j];
 }
@@ -767,11 +776,11 @@ int32_t emag_mpnn_mfma_probe(const float* a_frag, const float* b_frag, float* d)
   float *da = nullptr, *db = nullptr, *dd = nullptr;
   int32_t rc = 0;
   try {
-    MPCHECK(hipMalloc(&da, 256 * sizeof(float)));
-    MPCHECK(hipMalloc(&db, 256 * sizeof(float)));
+    MPCHECK(hipMalloc(&da, 512 * sizeof(float)));
+    MPCHECK(hipMalloc(&db, 512 * sizeof(float)));
     MPCHECK(hipMalloc(&dd, 256 * sizeof(float)));
-    MPCHECK(hipMemcpy(da, a_frag, 256 * sizeof(float), hipMemcpyHostToDevice));
-    MPCHECK(hipMemcpy(db, b_frag, 256 * sizeof(float), hipMemcpyHostToDevice));
+    MPCHECK(hipMemcpy(da, a_frag, 512 * sizeof(float), hipMemcpyHostToDevice));
+    MPCHECK(hipMemcpy(db, b_frag, 512 * sizeof(float), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_mpnn_probe, dim3(1), dim3(64), 0, nullptr, da, db, dd);
     MPCHECK(hipDeviceSynchronize());
     MPCHECK(hipMemcpy(d, dd, 256 * sizeof(float), hipMemcpyDeviceToHost));

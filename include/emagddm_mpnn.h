@@ -60,9 +60,9 @@ int32_t emag_mpnn_train_step(emag_mpnn_handle m, int64_t batch, const float* x_d
                              const float* target_dev, const float* omega_dev, float lr,
                              double* loss, void* stream);
 
-/* Lane-layout probe of the 16x16x16 bfloat16 MFMA used by the network: lane l
- * passes a_frag[4l..4l+3] and b_frag[4l..4l+3] (values exact in bfloat16) and
- * returns its four accumulators in d[4l..4l+3] (host arrays of 256 floats). */
+/* Lane-layout probe of the 16x16x32 bfloat16 MFMA used by the network: lane l
+ * passes a_frag[8l..8l+7] and b_frag[8l..8l+7] (values exact in bfloat16; host arrays
+ * of 512 floats) and returns its four accumulators in d[4l..4l+3] (256 floats). */
 int32_t emag_mpnn_mfma_probe(const float* a_frag, const float* b_frag, float* d);
 
 #ifdef __cplusplus

@@ -22,17 +22,17 @@ def _fp(a):
 
 
 def test_bf16_mfma_lane_layout(gpu_available):
-    """Lane l holds A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16]; accumulator j of lane l
-    is D[4(l/16)+j][l%16]."""
+    """16x16x32: lane l holds A[l%16][8(l/16)+j] and B[8(l/16)+j][l%16], j < 8;
+    accumulator j of lane l is D[4(l/16)+j][l%16]."""
     rng = np.random.default_rng(0)
-    A = rng.integers(-8, 9, size=(16, 16)).astype(np.float32)
-    B = rng.integers(-8, 9, size=(16, 16)).astype(np.float32)
-    af = np.zeros(256, np.float32)
-    bf = np.zeros(256, np.float32)
+    A = rng.integers(-8, 9, size=(16, 32)).astype(np.float32)
+    B = rng.integers(-8, 9, size=(32, 16)).astype(np.float32)
+    af = np.zeros(512, np.float32)
+    bf = np.zeros(512, np.float32)
     for l in range(64):
-        for j in range(4):
-            af[4 * l + j] = A[l % 16, 4 * (l // 16) + j]
-            bf[4 * l + j] = B[4 * (l // 16) + j, l % 16]
+        for j in range(8):
+            af[8 * l + j] = A[l % 16, 8 * (l // 16) + j]
+            bf[8 * l + j] = B[8 * (l // 16) + j, l % 16]
     d = np.zeros(256, np.float32)
     assert _lib.load().emag_mpnn_mfma_probe(_fp(af), _fp(bf), _fp(d)) == 0
     D = A.astype(np.float64) @ B.astype(np.float64)
