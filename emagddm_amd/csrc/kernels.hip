@@ -1065,7 +1065,7 @@ void launch_coarse_symv(int64_t nc, int tn, int64_t ntiles, const int32_t* tiles
 
 // X_g tile (i, j), i >= j: X_ii = L_ii^-1, X_ij = -L_ii^-1 sum_{k = max(j, f_i)}^{i-1} L_ik X_kj.
 // One work-group per tile; a launch holds the tiles of one distance d = i - j, so every
-// X_kj it reads (k - j < d) is complete.  Each thread owns a 4x4 block of the output.
+// X_kj it reads (k - j < d) is complete.
 __global__ __launch_bounds__(256) void k_cg_inv_step(const int32_t* __restrict__ items,
                                                      const double* __restrict__ cpool,
                                                      const int64_t* __restrict__ cra,
@@ -1075,88 +1075,63 @@ __global__ __launch_bounds__(256) void k_cg_inv_step(const int32_t* __restrict__
                                                      const int32_t* __restrict__ cg_group_of,
                                                      const int64_t* __restrict__ xoff,
                                                      double* __restrict__ X) {
-  // products in two halves of the inner dimension: sA holds 64 x 32 of the left factor,
-  // sB 32 x 64 of the right one (or all of M = the sum, 64 x 64, for the last product)
-  constexpr int kH = 33;
-  __shared__ double sA[64 * kH], sB[64 * kLd];
+  // FP64 MFMA: wave w holds rows 16 w .. 16 w + 15 of the 64x64 result (four 16x16
+  // blocks); each product stages its left factor as sA[row][inner] and its right factor
+  // as sB[inner][column] (leading dimension kLd)
+  using acc_t = Mfma<double>::acc_t;
+  __shared__ double sA[64 * kLd], sB[64 * kLd];
   const int i = items[2 * blockIdx.x], j = items[2 * blockIdx.x + 1];
   const int g = cg_group_of[i], a = cg_start[g];
   const int64_t n = int64_t(cg_start[g + 1] - a) * 64;
   double* Xg = X + xoff[g];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6, lr = l & 15, lk = l >> 4;
   const double* Li = cinv + (int64_t)i * kTileElems;
   double* Xij = Xg + (int64_t)(j - a) * 64 * n + (int64_t)(i - a) * 64;
   if (i == j) {
     for (int e = tid; e < kTileElems; e += 256) Xij[(int64_t)(e >> 6) * n + (e & 63)] = Li[e];
     return;
   }
-  const int r0 = (tid & 15) * 4, c0 = (tid >> 4) * 4;
-  double acc[4][4];
+  auto mma = [&](acc_t (&acc)[4]) {
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const double av = sA[(16 * wv + lr) * kLd + kk + lk];
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+      for (int cb = 0; cb < 4; ++cb) acc[cb] = Mfma<double>::op(av, sB[(kk + lk) * kLd + 16 * cb + lr], acc[cb]);
+    }
+  };
+  acc_t m[4];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) acc[q][p] = 0.0;
+  for (int cb = 0; cb < 4; ++cb) m[cb] = acc_t{0.0, 0.0, 0.0, 0.0};
+  // M = sum_k L_ik X_kj over the envelope of row i
   const int k0 = max(j, cft[i]);
   for (int k = k0; k < i; ++k) {
-    const double* Lk = cpool + cra[i] + (int64_t)k * kTileElems;
+    const double* Lk = cpool + cra[i] + (int64_t)k * kTileElems;       // column-major tile
     const double* Xk = Xg + (int64_t)(j - a) * 64 * n + (int64_t)(k - a) * 64;
-    for (int h = 0; h < 2; ++h) {
-      for (int e = tid; e < 64 * 32; e += 256) {
-        const int r = e & 63, c = e >> 6;  // row r, column 32h + c of L_ik
-        sA[r * kH + c] = Lk[(32 * h + c) * 64 + r];
-        const int t = e & 31, col = e >> 5;  // row 32h + t, column col of X_kj
-        sB[t * kLd + col] = Xk[(int64_t)col * n + 32 * h + t];
-      }
-      __syncthreads();
-      for (int t = 0; t < 32; ++t) {
-        double av[4], bv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          av[q] = sA[(r0 + q) * kH + t];
-          bv[q] = sB[t * kLd + c0 + q];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int p = 0; p < 4; ++p) acc[q][p] = fma(av[q], bv[p], acc[q][p]);
-      }
-      __syncthreads();
-    }
-  }
-  // X_ij = -L_ii^-1 M with M = acc (all of M in sB, L_ii^-1 in two halves in sA)
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int p = 0; p < 4; ++p) sB[(r0 + q) * kLd + c0 + p] = acc[q][p];
-  double out[4][4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int p = 0; p < 4; ++p) out[q][p] = 0.0;
-  for (int h = 0; h < 2; ++h) {
-    for (int e = tid; e < 64 * 32; e += 256) {
-      const int r = e & 63, c = e >> 6;
-      sA[r * kH + c] = Li[(32 * h + c) * 64 + r];
+    for (int e = tid; e < kTileElems; e += 256) {
+      const int hi = e >> 6, lo = e & 63;
+      sA[lo * kLd + hi] = Lk[e];                       // L_ik(row lo, inner hi)
+      sB[lo * kLd + hi] = Xk[(int64_t)hi * n + lo];    // X_kj(inner lo, column hi)
     }
     __syncthreads();
-    for (int t = 0; t < 32; ++t) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        av[q] = sA[(r0 + q) * kH + t];
-        bv[q] = sB[(32 * h + t) * kLd + c0 + q];
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int p = 0; p < 4; ++p) out[q][p] = fma(av[q], bv[p], out[q][p]);
-    }
+    mma(m);
     __syncthreads();
   }
+  // X_ij = -L_ii^-1 M
 #pragma unroll
-  for (int p = 0; p < 4; ++p)
+  for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) Xij[(int64_t)(c0 + p) * n + r0 + q] = -out[q][p];
+    for (int r = 0; r < 4; ++r) sB[(16 * wv + Mfma<double>::row(lk, r)) * kLd + 16 * cb + lr] = m[cb][r];
+  for (int e = tid; e < kTileElems; e += 256) sA[(e & 63) * kLd + (e >> 6)] = Li[e];
+  __syncthreads();
+  acc_t o[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) o[cb] = acc_t{0.0, 0.0, 0.0, 0.0};
+  mma(o);
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      Xij[(int64_t)(16 * cb + lr) * n + 16 * wv + Mfma<double>::row(lk, r)] = -o[cb][r];
 }
 
 void launch_cg_invert(int max_group, const int64_t* step, const int32_t* items, const double* cpool,
