@@ -1140,7 +1140,18 @@ struct fetidp_solver {
     launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
     coarse_apply(gc.p, uc.p);
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
-    launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
+    if (deluxe) {
+      // deluxe average ubar = (S0 + S1)^-1 (S0 u0 + S1 u1) = u0 + D_F (u1 - u0) per face:
+      // each side weighted by its own Dirichlet block, so a stiff side's value is not
+      // spoiled by the soft side's (the interface residual then carries no contrast factor)
+      const FacePlan& F = plans.faces;
+      launch_deluxe_jump(F.vec_elems, face_map_lam.p, face_slot0.p, face_slot1.p, dbuf.p, fv1.p, st);
+      if (d64) launch_face_dmv<double>(false, F.n, F.max_T, face_dev.p, fD64.p, face_map_self.p, fv1.p, fv2.p, st);
+      else launch_face_dmv<float>(false, F.n, F.max_T, face_dev.p, fD32.p, face_map_self.p, fv1.p, fv2.p, st);
+      launch_deluxe_avg(F.vec_elems, face_map_lam.p, face_slot0.p, dbuf.p, fv2.p, ubar.p, st);
+    } else {
+      launch_lambda_scatter(topo.n_lam, lam_pos.p, d_half.p, dbuf.p, ubar.p, st);
+    }
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
     timed(13, [&] {
       launch_interior<T>(topo.n_sub, plans.max_Ti, devsub.p, pool.p, rowabs.p, inv.p, local_inv_base.p,

@@ -172,6 +172,12 @@ void launch_face_D(int64_t nfaces, int max_tf, const FaceDev* faces, const doubl
 template <typename T>
 void launch_face_dmv(bool trans, int64_t nfaces, int max_tf, const FaceDev* faces, const T* D,
                      const int32_t* map, const double* src, double* out, hipStream_t st);
+// Deluxe average of the two sides' values u (D-slot space) on every face:
+// jmp = u_s1 - u_s0 (face-vector space, 0 on padding); ubar[l] = u_s0 + g with g = D_F jmp.
+void launch_deluxe_jump(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
+                        const double* u, double* jmp, hipStream_t st);
+void launch_deluxe_avg(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const double* u,
+                       const double* g, double* ubar, hipStream_t st);
 void launch_deluxe_split(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
                          const double* g, const double* r, double* xs, hipStream_t st);
 void launch_deluxe_sum(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,

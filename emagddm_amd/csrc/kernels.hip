@@ -2624,6 +2624,40 @@ __global__ __launch_bounds__(256) void k_deluxe_out(int64_t nv, const int32_t* _
   if (l >= 0) z[l] = h[k] - y[slot1[k]];
 }
 
+// Deluxe average of the two sides' interface values (recovery): jump j = u_s1 - u_s0 per
+// face-vector slot (0 on padding), then ubar = u_s0 + D_F j, scattered to the multipliers.
+__global__ __launch_bounds__(256) void k_deluxe_jump(int64_t nv, const int32_t* __restrict__ map_lam,
+                                                     const int64_t* __restrict__ slot0,
+                                                     const int64_t* __restrict__ slot1,
+                                                     const double* __restrict__ u,
+                                                     double* __restrict__ jmp) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= nv) return;
+  jmp[k] = map_lam[k] >= 0 ? u[slot1[k]] - u[slot0[k]] : 0.0;
+}
+__global__ __launch_bounds__(256) void k_deluxe_avg(int64_t nv, const int32_t* __restrict__ map_lam,
+                                                    const int64_t* __restrict__ slot0,
+                                                    const double* __restrict__ u,
+                                                    const double* __restrict__ g,
+                                                    double* __restrict__ ubar) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= nv) return;
+  const int32_t l = map_lam[k];
+  if (l >= 0) ubar[l] = u[slot0[k]] + g[k];
+}
+void launch_deluxe_jump(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
+                        const double* u, double* jmp, hipStream_t st) {
+  if (nv > 0)
+    hipLaunchKernelGGL(k_deluxe_jump, dim3((nv + 255) / 256), dim3(256), 0, st, nv, map_lam, slot0, slot1,
+                       u, jmp);
+}
+void launch_deluxe_avg(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const double* u,
+                       const double* g, double* ubar, hipStream_t st) {
+  if (nv > 0)
+    hipLaunchKernelGGL(k_deluxe_avg, dim3((nv + 255) / 256), dim3(256), 0, st, nv, map_lam, slot0, u, g,
+                       ubar);
+}
+
 void launch_deluxe_split(int64_t nv, const int32_t* map_lam, const int64_t* slot0, const int64_t* slot1,
                          const double* g, const double* r, double* xs, hipStream_t st) {
   if (nv > 0)
