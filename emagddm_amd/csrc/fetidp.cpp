@@ -857,6 +857,10 @@ struct fetidp_solver {
   // dual tolerance); measured ratios: 474 on cube48j (not a floor), 5e3 on cube64c and
   // 2.6e4 on cube96j (floors, removed by one refinement step)
   static constexpr double kFloorRatio = 1e3;
+  // Assumed amplification of a first refinement correction when the recovery takes the
+  // deluxe average of the interface values; measured 1.4 (96^3 contrast cube) and 0.4
+  // (64^3 contrast cube), against 470 with the plain mean of the two sides
+  static constexpr double kCorrAmp = 10.0;
 
   bool precond_fp64(bool dl) const {
     if (dl) return true;  // the face blocks are formed and inverted in FP64
@@ -1386,9 +1390,9 @@ struct fetidp_solver {
       int k = 0;
       // Amplification of a correction's dual residual into the recovered residual: a
       // first solve that ended on a floor measured the floor, not the amplification, so
-      // the correction assumes kFloorRatio until one correction has measured its own
-      // (measured: 2.6e4 floor ratio but 470 per correction on cube96j)
-      double amp_ref = std::min(amp, kFloorRatio);
+      // the first correction assumes kCorrAmp (deluxe average in the recovery) or
+      // kFloorRatio, and later ones what the previous correction measured
+      double amp_ref = std::min(amp, deluxe ? kCorrAmp : kFloorRatio);
       while (status == FETIDP_SUCCESS && res > cfg.tolerance && k < cfg.max_refinements) {
         condense<T>(pool, inv, W, rglob.p, false);
         lam.zero(st);

@@ -26,8 +26,9 @@ solver implements, in the plainest dense form, for small meshes:
   (src/numeric/iterative_solvers.cpp:449-491).
 * Recovery: u_c = S~^-1 (f~_c + sum R^T Phi^T B^T lambda),
   u_D,s = a_s - Z_s B^T lambda - Phi_s R u_c on each side; the global vector
-  takes u_c on primal edges and the mean of the two sides on each dual edge,
-  then every interior block is re-solved against those interface values,
+  takes u_c on primal edges and on each dual edge the mean of the two sides
+  (stiffness scaling) or their deluxe average u_s0 + D_F (u_s1 - u_s0) per face
+  (deluxe scaling), then every interior block is re-solved against those values,
   u_i = K_ii^-1 (f_i - K_ib u_b), so that only interface rows keep a residual.
 """
 from __future__ import annotations
@@ -255,7 +256,8 @@ class DenseFETIDP:
         return lam, status, min(it, max_iter), hist
 
     def recover(self, lam):
-        """Dirichlet recovery: mean interface values, then local interior solves."""
+        """Dirichlet recovery: averaged interface values (mean, or deluxe average with
+        deluxe scaling), then local interior solves."""
         g = self.fc.copy()
         for S in self.subs:
             x = S["sgn"] * lam[S["lam"]]
@@ -263,13 +265,21 @@ class DenseFETIDP:
         uc = self._coarse_solve(g)
         xg = np.zeros(self.n_free)
         cnt = np.zeros(self.n_free)
+        uds = []
         for S in self.subs:
             ni, nd = S["ni"], S["nd"]
             ud = S["a"] - S["Z"] @ (S["sgn"] * lam[S["lam"]]) - S["Phi"] @ uc[S["cidx"]]
+            uds.append(ud)
             o = S["order"]
             xg[o[ni:ni + nd]] += ud
             cnt[o[ni:ni + nd]] += 1
         xg = np.where(cnt > 0, xg / np.maximum(cnt, 1), xg)
+        if self.scaling == "deluxe":
+            # (S0 + S1)^-1 (S0 u0 + S1 u1) on every face
+            for (s0, s1, lams, k0, k1, D0) in self.faces:
+                S = self.subs[s0]
+                u0, u1 = uds[s0][k0], uds[s1][k1]
+                xg[S["order"][S["ni"] + k0]] = u0 + D0 @ (u1 - u0)
         prim = np.flatnonzero(self.c_of >= 0)
         xg[prim] = uc[self.c_of[prim]]
         for S in self.subs:

@@ -154,6 +154,38 @@ def test_deluxe_preconditioner_matches_oracle(gpu_available, n, parts, prec):
     assert abs(r1 @ z2 - r2 @ z1) <= 1e-12 * np.linalg.norm(r1) * np.linalg.norm(z2)
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-8), ("auto", 1e-6)])
+def test_deluxe_recovery_matches_oracle(gpu_available, prec, tol):
+    """Recovery at a given lambda (no PCPG iteration) with deluxe scaling: the interface
+    values are the deluxe average u_s0 + D_F (u_s1 - u_s0) of the two sides, as in the
+    oracle, to the rounding of the interior re-solves of this contrast problem (a few
+    1e-9); the mean of the two sides (stiffness scaling's recovery) differs from it by
+    more than 1e3 times that at a lambda that leaves jumps."""
+    from oracle import fetidp as ofetidp
+    p = W.make_problem(12, 3, None, "contrast", seed=7, name="cd")
+    print(f"[deluxe] recovery 12^3 3^3 {prec}", flush=True)
+    F = ofetidp.DenseFETIDP(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.tet_subdomain,
+                            p.pec_pairs, p.jq, scaling="deluxe")
+    lam_star = F.solve(tol=1e-12, max_iter=500)["lam"]
+    rng = np.random.default_rng(5)
+    lam = lam_star + 1e-3 * np.linalg.norm(lam_star) / np.sqrt(lam_star.size) * rng.normal(size=lam_star.size)
+    s = FETIDPSolver(scaling="deluxe", precond_precision=prec, tolerance=1e30, dual_tolerance=1e30)
+    s.setup_problem(p)
+    s.set_rhs_tet_J(p.jq)
+    s.factor()
+    r = s.solve(lambda0=lam)
+    assert r.iterations == 0 and r.refinements == 0
+    xo = F.recover(lam)
+    err = np.linalg.norm(r.x - xo) / np.linalg.norm(xo)
+    # the plain mean of the two sides, for scale
+    F.scaling = "stiffness"
+    xm = F.recover(lam)
+    F.scaling = "deluxe"
+    dm = np.linalg.norm(xm - xo) / np.linalg.norm(xo)
+    print(f"[deluxe] recovery rel err {err:.2e} (mean vs deluxe average {dm:.2e})", flush=True)
+    assert err <= tol and dm > 1e3 * err
+
+
 def test_deluxe_contrast_solve(gpu_available):
     """Automatic deluxe scaling (kappa/nu spans > 1e4): PCPG count of the oracle's
     deluxe iteration, 1e-8 residual, direct answer."""

@@ -88,10 +88,12 @@ def test_nonfinite_source_is_invalid_input(gpu_available):
 
 def test_fp32_gate_and_stall_fallback(gpu_available):
     """FP32_REFINE either converges or hands over to FP64 (fell_back) with the FP64
-    answer: on a 24^3 cube (FP32 accuracy estimate 3.4e-3, under the gate) and on a
-    contrast problem whose estimate is far above it."""
+    answer: on a 24^3 cube (FP32 accuracy estimate 3.4e-3, under the gate), on an 8^3
+    contrast problem (estimate 4.1e-3, under the gate) and on a 16^3 contrast problem
+    whose estimate (6.6e-2) is far above it, which must factor in FP64 from the start."""
     for p in (W.make_problem(24, 2, seed=2, name="fp32_24"),
-              W.make_problem(8, 2, materials="contrast", seed=5, name="fp32_c")):
+              W.make_problem(8, 2, materials="contrast", seed=5, name="fp32_c8"),
+              W.make_problem(16, 2, materials="contrast", seed=5, name="fp32_c")):
         print(f"[status] fp32 {p.name}", flush=True)
         s = FETIDPSolver(precision="fp32").setup_problem(p)
         s.set_rhs_tet_J(p.jq)
