@@ -93,6 +93,7 @@ struct fetidp_solver {
   hipStream_t st3 = nullptr;  // operator extraction beside the coarse factorization
   hipEvent_t ev_main = nullptr, ev_side = nullptr, ev_extract = nullptr, ev_zero = nullptr,
              ev_early = nullptr;
+  hipEvent_t ev_phi = nullptr, ev_coarse = nullptr;  // PCPG: coarse solve beside the Z product
   int32_t n_mat = 0;
   double symbolic_ms = 0.0, last_factor_ms = 0.0, upload_ms = 0.0;
   bool device_ready = false;
@@ -166,11 +167,12 @@ struct fetidp_solver {
   DBuf<double> cpart;        // partial sums of the coarse S~^-1 products (tn * tn * 64)
   DBuf<int32_t> ctiles;      // (I, J <= I) tile pairs of the -S~^-1 block of the coarse pool
   int coarse_tn = 0;         // its tile order, coarse_ncp / 64
-  // uc = S~^-1 x from the factored coarse pool
-  void coarse_apply(const double* x, double* y) {
+  // uc = S~^-1 x from the factored coarse pool, on stream cs (default: st)
+  void coarse_apply(const double* x, double* y, hipStream_t cs = nullptr) {
+    if (!cs) cs = st;
     if (!plans.coarse_grouped) {
       launch_coarse_symv(topo.n_c, coarse_tn, int64_t(ctiles.n / 2), ctiles.p, cpool.p, coarse_rowabs.p, x,
-                         cpart.p, y, st);
+                         cpart.p, y, cs);
       return;
     }
     // S~^-1 x = L^-T L^-1 x: forward sweep over the groups, then backward; step A of a
@@ -181,24 +183,24 @@ struct fetidp_solver {
       const int64_t o = plans.cg_fwd_off[g];
       const int pa = g > 0 ? gs[g - 1] : 0, pw = g > 0 ? gs[g] - gs[g - 1] : 0;
       launch_cg_step_a(false, pa, pw, cg_px.p, cg_yv.p, nullptr, 0, cg_fwd_tiles_d.p + 2 * o,
-                       plans.cg_fwd_off[g + 1] - o, cpool.p, coarse_rowabs.p, cg_yv.p, cg_pf.p + o * kTile, st);
+                       plans.cg_fwd_off[g + 1] - o, cpool.p, coarse_rowabs.p, cg_yv.p, cg_pf.p + o * kTile, cs);
       launch_cg_step_b(false, gs[g], gs[g + 1] - gs[g], cgX.p + plans.cg_xoff[g], x, topo.n_c,
-                       cg_fwd_rowptr_d.p, cg_pf.p, cg_px.p, st);
+                       cg_fwd_rowptr_d.p, cg_pf.p, cg_px.p, cs);
     }
     if (ng > 0)
       launch_cg_step_a(false, gs[ng - 1], gs[ng] - gs[ng - 1], cg_px.p, cg_yv.p, nullptr, 0, nullptr, 0,
-                       cpool.p, coarse_rowabs.p, nullptr, nullptr, st);
+                       cpool.p, coarse_rowabs.p, nullptr, nullptr, cs);
     for (int g = ng - 1; g >= 0; --g) {
       const int64_t o = plans.cg_bwd_off[g];
       const int pa = g + 1 < ng ? gs[g + 1] : 0, pw = g + 1 < ng ? gs[g + 2] - gs[g + 1] : 0;
       launch_cg_step_a(true, pa, pw, cg_px.p, cg_xv.p, y, topo.n_c, cg_bwd_tiles_d.p + 2 * o,
-                       plans.cg_bwd_off[g + 1] - o, cpool.p, coarse_rowabs.p, cg_xv.p, cg_pb.p + o * kTile, st);
+                       plans.cg_bwd_off[g + 1] - o, cpool.p, coarse_rowabs.p, cg_xv.p, cg_pb.p + o * kTile, cs);
       launch_cg_step_b(true, gs[g], gs[g + 1] - gs[g], cgX.p + plans.cg_xoff[g], cg_yv.p, plans.coarse_ncp,
-                       cg_bwd_colptr_d.p, cg_pb.p, cg_px.p, st);
+                       cg_bwd_colptr_d.p, cg_pb.p, cg_px.p, cs);
     }
     if (ng > 0)
       launch_cg_step_a(true, gs[0], gs[1] - gs[0], cg_px.p, cg_xv.p, y, topo.n_c, nullptr, 0, cpool.p,
-                       coarse_rowabs.p, nullptr, nullptr, st);
+                       coarse_rowabs.p, nullptr, nullptr, cs);
   }
   DBuf<int> cfail;
   DBuf<double> f_glob, f_cur, vec, vout, U, ybuf, dbuf, fc, uc, gc, yvec, tbuf;
@@ -270,7 +272,7 @@ struct fetidp_solver {
     for (hipEvent_t e : guess_ev)
       if (e) (void)hipEventDestroy(e);
     if (hsc) (void)hipHostFree(hsc);
-    for (hipEvent_t e : {ev_main, ev_side, ev_extract, ev_zero, ev_early})
+    for (hipEvent_t e : {ev_main, ev_side, ev_extract, ev_zero, ev_early, ev_phi, ev_coarse})
       if (e) (void)hipEventDestroy(e);
     if (st3) (void)hipStreamDestroy(st3);
     if (st2) (void)hipStreamDestroy(st2);
@@ -491,6 +493,8 @@ struct fetidp_solver {
       HIPCHECK(hipEventCreateWithFlags(&ev_extract, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_zero, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_early, hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&ev_phi, hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&ev_coarse, hipEventDisableTiming));
       HIPCHECK(hipEventRecord(ev_side, st2));  // a recorded, complete event to start from
     }
     xyz.upload(h_xyz_);
@@ -1101,6 +1105,15 @@ struct fetidp_solver {
     launch_apply_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp,
                       devsub.p, W.p, d_lam.p, d_sgn.p, pin, sympart.p, ybuf.p, st);
   }
+  // The same W x as two launches, Phi^T x first (ev_phi recorded after it), so that the
+  // coarse solve it feeds can run on st2 beside the Z product (PCPG iterations)
+  template <typename T>
+  void apply_W_split(const DBuf<T>& W, const double* pin) {
+    launch_gemv_phi<T>(topo.n_sub, plans.max_ncp, devsub.p, W.p, d_lam.p, d_sgn.p, pin, ybuf.p, st, plans.max_ndp);
+    HIPCHECK(hipEventRecord(ev_phi, st));
+    launch_symv<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_Td, devsub.p, W.p, 0,
+                   d_lam.p, d_sgn.p, pin, sympart.p, nullptr, st);
+  }
   void apply_M(const double* rin, double* zout) {
     if (deluxe) {
       timed(11, [&] { apply_M_deluxe(rin, zout); });
@@ -1182,10 +1195,25 @@ struct fetidp_solver {
   template <typename T>
   void pcpg_iteration_kernels(const DBuf<T>& W, bool head_queued = false) {
     const int64_t nl = topo.n_lam;
-    if (!head_queued) timed(9, [&] { apply_W<T>(W, p.p); });
+    // outside event profiling the coarse solve (a chain of small launches) runs on st2
+    // beside the Z product; profiled runs keep every launch on st
+    const bool split = !profiling();
+    if (!head_queued) {
+      if (split) apply_W_split<T>(W, p.p);
+      else timed(9, [&] { apply_W<T>(W, p.p); });
+    }
+    if (split) {
+      HIPCHECK(hipStreamWaitEvent(st2, ev_phi, 0));
+      launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st2);
+      coarse_apply(gc.p, uc.p, st2);
+      HIPCHECK(hipEventRecord(ev_coarse, st2));
+      HIPCHECK(hipStreamWaitEvent(st, ev_coarse, 0));
+    }
     timed(10, [&] {
-      launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
-      coarse_apply(gc.p, uc.p);
+      if (!split) {
+        launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, nullptr, gc.p, st);
+        coarse_apply(gc.p, uc.p);
+      }
       launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, 1.0, 1.0, 0.0,
                              dbuf.p, st);
       const double* a1[1] = {p.p};
@@ -1291,7 +1319,7 @@ struct fetidp_solver {
       const double seq = hs[kScSeq] + 1.0;
       pcpg_iteration<T>(W, head);
       head = queue_next && k + 1 < max_it && last_dual > 100.0 * tol;
-      if (head) apply_W<T>(W, p.p);
+      if (head) apply_W_split<T>(W, p.p);
       wait_scalars(seq);
       if (!(std::fabs(hs[kScPQ]) >= 1e-300)) return FETIDP_NUMERICAL_ERROR;
       ++its;
