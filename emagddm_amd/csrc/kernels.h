@@ -98,7 +98,16 @@ void launch_cg_step_b(bool trans, int a, int w, const double* Xg, const double* 
 template <typename T>
 void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool,
                     const int64_t* rowabs, const T* inv, const int64_t* inv_base,
-                    const int32_t* ftile, double* vec, double* yv, hipStream_t st);
+                    const int32_t* ftile, double* vec, double* yv, hipStream_t st, int skip_int = 0,
+                    int j0 = 0);
+// skip_int: the interior blocks of vec are zero and those of yv already hold zeros, so every
+// subdomain skips its own interior columns and the sweep starts at column j0 (at most the
+// smallest interior tile count).
+// Zero the interior blocks (tile rows < Ti) of a local vector.
+void launch_zero_interior(int ns, int max_Ti, const DevSub* subs, double* v, hipStream_t st);
+// kDotBlocks partial sums of x_d^2 over the free dofs d of kind `want` (0 interior).
+void launch_kind_sq(int64_t n, const int8_t* kind, int want, const double* x, double* partial,
+                    hipStream_t st);
 // Interior solve: xout_i = L_ii^-T (yv_i - L_{D,c; i}^T U); tbuf is scratch (vec space).
 template <typename T>
 void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const int64_t* rowabs,

@@ -1404,6 +1404,8 @@ __device__ __forceinline__ void colsum4(const T* L, const double* u, int tid, do
 
 // Forward step J: y_J = L_JJ^-1 b_J (written to yv by the row-J workgroup);
 // rows I > J: b_I -= L_IJ y_J.  D' rows are structurally zero while J < Ti.
+// skip_int: the interior right-hand side is zero (and yv's interior blocks already are),
+// so the interior columns change nothing and are skipped.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
@@ -1411,9 +1413,10 @@ __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ sub
                                                   const T* __restrict__ inv,
                                                   const int64_t* __restrict__ inv_base,
                                                   const int32_t* __restrict__ ftile, int J,
-                                                  double* __restrict__ vec, double* __restrict__ yv) {
+                                                  int skip_int, double* __restrict__ vec,
+                                                  double* __restrict__ yv) {
   const DevSub S = subs[blockIdx.y];
-  if (J >= S.Tr) return;
+  if (J >= S.Tr || (skip_int && J < S.Ti)) return;
   const int I = J + blockIdx.x;
   if (I >= S.T || ftile[S.ft_off + I] > J) return;  // tile (I, J) outside the envelope
   __shared__ double part[4][64];
@@ -1435,18 +1438,53 @@ __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ sub
 template <typename T>
 void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool,
                     const int64_t* rowabs, const T* inv, const int64_t* inv_base,
-                    const int32_t* ftile, double* vec, double* yv, hipStream_t st) {
+                    const int32_t* ftile, double* vec, double* yv, hipStream_t st, int skip_int,
+                    int j0) {
   if (ns <= 0) return;
-  for (int J = 0; J < max_Tr; ++J)
+  for (int J = skip_int ? j0 : 0; J < max_Tr; ++J)
     hipLaunchKernelGGL(k_fwd_step<T>, dim3(max_T - J, ns), dim3(256), 0, st, subs, pool, rowabs, inv,
-                       inv_base, ftile, J, vec, yv);
+                       inv_base, ftile, J, skip_int, vec, yv);
 }
 template void launch_forward<double>(int, int, int, const DevSub*, const double*, const int64_t*,
                                      const double*, const int64_t*, const int32_t*, double*,
-                                     double*, hipStream_t);
+                                     double*, hipStream_t, int, int);
 template void launch_forward<float>(int, int, int, const DevSub*, const float*, const int64_t*,
                                     const float*, const int64_t*, const int32_t*, double*, double*,
-                                    hipStream_t);
+                                    hipStream_t, int, int);
+
+// Zero the interior blocks (tile rows < Ti) of a local vector, one work-group per
+// (interior tile, subdomain).
+__global__ __launch_bounds__(64) void k_zero_interior(const DevSub* __restrict__ subs,
+                                                      double* __restrict__ v) {
+  const DevSub S = subs[blockIdx.y];
+  if ((int)blockIdx.x >= S.Ti) return;
+  v[S.vec_off + (int64_t)blockIdx.x * 64 + threadIdx.x] = 0.0;
+}
+void launch_zero_interior(int ns, int max_Ti, const DevSub* subs, double* v, hipStream_t st) {
+  if (ns <= 0 || max_Ti <= 0) return;
+  hipLaunchKernelGGL(k_zero_interior, dim3(max_Ti, ns), dim3(64), 0, st, subs, v);
+}
+
+// Block partial sums of x_d^2 over the free dofs of one kind (0 interior, 1 dual, 2 primal),
+// kDotBlocks partials (summed on the host in a fixed order).
+__global__ __launch_bounds__(256) void k_kind_sq(int64_t n, const int8_t* __restrict__ kind, int want,
+                                                 const double* __restrict__ x, double* __restrict__ partial) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    if (kind[i] == want) s += x[i] * x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+void launch_kind_sq(int64_t n, const int8_t* kind, int want, const double* x, double* partial,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_kind_sq, dim3(kDotBlocks), dim3(256), 0, st, n, kind, want, x, partial);
+}
 
 // Interface part of the interior solve, for every interior tile column K:
 // t_K = y_K - sum_{I in [Ti, Tc)} L_IK^T U_I.  1024 threads per (K, subdomain): quarter q
