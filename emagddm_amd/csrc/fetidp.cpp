@@ -1160,9 +1160,18 @@ struct fetidp_solver {
   // recover the global free-dof solution from lambda into xout
   template <typename T>
   void recover(const DBuf<T>& pool, const DBuf<T>& inv, const DBuf<T>& W, double* xout) {
-    apply_W<T>(W, lam.p);
-    launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
-    coarse_apply(gc.p, uc.p);
+    if (profiling()) {
+      apply_W<T>(W, lam.p);
+      launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st);
+      coarse_apply(gc.p, uc.p);
+    } else {  // the coarse solve beside the Z product, as in the PCPG iterations
+      apply_W_split<T>(W, lam.p);
+      HIPCHECK(hipStreamWaitEvent(st2, ev_phi, 0));
+      launch_coarse_gather(topo.n_c, c_ptr.p, c_ypos.p, ybuf.p, fc.p, gc.p, st2);
+      coarse_apply(gc.p, uc.p, st2);
+      HIPCHECK(hipEventRecord(ev_coarse, st2));
+      HIPCHECK(hipStreamWaitEvent(st, ev_coarse, 0));
+    }
     launch_dual_correct<T>(topo.n_sub, plans.max_ndp, plans.max_ncp, devsub.p, W.p, sympart.p, c_loc.p, uc.p, vec.p, -1.0, -1.0, -1.0, dbuf.p, st);
     if (deluxe) {
       // deluxe average ubar = (S0 + S1)^-1 (S0 u0 + S1 u1) = u0 + D_F (u1 - u0) per face:
