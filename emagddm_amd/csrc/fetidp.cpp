@@ -1301,8 +1301,10 @@ struct fetidp_solver {
   // `fresh` recomputes r, z, p from lambda; otherwise continues the recurrence
   // (r, z, p and the device scalar r.z persist between calls).
   template <typename T>
+  // lam_zero: the start is lam = 0, so F lam = 0 and the first residual is d itself (the
+  // F product of the start is skipped; bitwise the same r)
   int pcpg(const DBuf<T>& W, double tol, int max_it, bool fresh, int& its,
-           double& dn) {
+           double& dn, bool lam_zero = false) {
     const int64_t nl = topo.n_lam;
     volatile const double* hs = hsc;
     double o[3];
@@ -1313,8 +1315,8 @@ struct fetidp_solver {
         last_dual = init_dual = 0.0;
         return FETIDP_SUCCESS;
       }
-      apply_F<T>(W, lam.p, q.p);
-      if (last_learned) {
+      if (!lam_zero) apply_F<T>(W, lam.p, q.p);
+      if (last_learned && !lam_zero) {
         // scale the learned start by its F-norm optimum alpha = (lam.d)/(lam.F lam),
         // so ||lam0 - lam*||_F <= ||lam*||_F whatever the network predicted
         const double* a2[2] = {lam.p, lam.p};
@@ -1324,7 +1326,7 @@ struct fetidp_solver {
         launch_scale2(nl, alpha, lam.p, q.p, st);
       }
       HIPCHECK(hipMemcpyAsync(r.p, dv.p, sizeof(double) * nl, hipMemcpyDeviceToDevice, st));
-      launch_axpy(nl, -1.0, q.p, r.p, st);
+      if (!lam_zero) launch_axpy(nl, -1.0, q.p, r.p, st);
       apply_M(r.p, z.p);
       const double* a[2] = {r.p, r.p};
       const double* b[2] = {r.p, z.p};
@@ -1420,10 +1422,11 @@ struct fetidp_solver {
       if (h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
       else if (lam_model) learned_guess();
       else lam.zero(st);
+      const bool zero_start = !h_lam0 && !lam_model;
       double tol = cfg.dual_tolerance, prev = 1e300, amp = 1.0;
       bool fresh = true;
       for (int round = 0; round < 6; ++round) {
-        status = pcpg<T>(W, tol, cfg.max_iterations - its, fresh, its, dn);
+        status = pcpg<T>(W, tol, cfg.max_iterations - its, fresh, its, dn, fresh && zero_start);
         fresh = false;
         recover<T>(pool, inv, W, x.p);
         res = global_residual(x.p, f_glob.p);
@@ -1460,7 +1463,7 @@ struct fetidp_solver {
         // the correction needs only enough dual accuracy to bring ||f - K x|| under the
         // tolerance with a 4x margin
         const double rtol = std::min(1e-2, std::max(cfg.dual_tolerance, 0.25 * cfg.tolerance / (res * amp_ref)));
-        status = pcpg<T>(W, rtol, cfg.max_iterations - its, true, its, dn);
+        status = pcpg<T>(W, rtol, cfg.max_iterations - its, true, its, dn, true);
         if (status != FETIDP_SUCCESS) break;
         recover<T>(pool, inv, W, x.p);
         launch_axpy(nf, 1.0, x.p, xtot.p, st);
@@ -1488,7 +1491,8 @@ struct fetidp_solver {
         if (k == 0 && h_lam0) HIPCHECK(hipMemcpyAsync(lam.p, h_lam0, sizeof(double) * nl, hipMemcpyHostToDevice, st));
         else if (k == 0 && lam_model) learned_guess();
         else lam.zero(st);
-        status = pcpg<T>(W, cfg.inner_tolerance, cfg.max_iterations - its, true, its, dn);
+        status = pcpg<T>(W, cfg.inner_tolerance, cfg.max_iterations - its, true, its, dn,
+                         k > 0 || (!h_lam0 && !lam_model));
         if (status == FETIDP_NUMERICAL_ERROR || status == FETIDP_DIVERGED) break;
         recover<T>(pool, inv, W, x.p);                      // z = M32^-1 r
         if (k == 0) {
