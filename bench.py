@@ -186,6 +186,21 @@ def roofline(v, precision, workload):
     if intensity is not None:
         out.update(flop_per_byte=round(intensity, 2), ridge_flop_per_byte=round(ridge, 2),
                    hbm_gbs=round(traffic / (avg_ms * 1e-3) / 1e9, 1))
+    # the same kernel split by the matrices its launches work on: the batched subdomain
+    # launches and the coarse matrix's dependent single-matrix chain
+    split = {}
+    for cls, m in KERNEL_OF_CLASS.items():
+        if m != mode:
+            continue
+        ms_c, n_c = prof[cls]
+        if n_c == 0 or ms_c == 0:
+            continue
+        fl = prof["flops"].get(cls, 0.0) * PROFILE_STEPS
+        split[cls] = dict(launches_per_step=n_c // PROFILE_STEPS, avg_launch_ms=round(ms_c / n_c, 4),
+                          achieved=round(fl / (ms_c * 1e-3) / 1e12, 2),
+                          frac=round(fl / (ms_c * 1e-3) / 1e12 / peak, 4))
+    if split:
+        out["by_matrix_class"] = split
     return out
 
 
