@@ -961,7 +961,10 @@ template void launch_extract_W<float>(int64_t, const SymTile*, int, int, int, co
 // k_coarse_reduce sums each row of slots (in a fixed order) and negates.  Lane = tile row and
 // wave w holds columns 16w .. 16w+15, so every load is one coalesced 512-byte column; the
 // column sums use the same halving butterfly as k_symv_tile, the row sums a 4-way LDS
-// reduction.  Entries of x past nc read as zero (padding rows of the block are zero).
+// reduction.  Diagonal tiles are read through their lower triangle only (the upper entry
+// (r, c) as (c, r)), like every other reader of the pool, so the operator is symmetric by
+// construction whatever the factorization left above the diagonal.  Entries of x past nc
+// read as zero (padding rows of the block are zero).
 __global__ __launch_bounds__(256) void k_coarse_symv(int64_t nc, int tn, const int32_t* __restrict__ tiles,
                                                      const double* __restrict__ cpool,
                                                      const int64_t* __restrict__ cra,
@@ -973,8 +976,12 @@ __global__ __launch_bounds__(256) void k_coarse_symv(int64_t nc, int tn, const i
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const double* A = cpool + cra[tn + I] + (int64_t)(tn + J) * kTileElems;
   double at[16];
+  const bool diag = I == J;  // uniform per work-group
 #pragma unroll
-  for (int q = 0; q < 16; ++q) at[q] = A[(16 * wv + q) * 64 + lane];
+  for (int q = 0; q < 16; ++q) {
+    const int col = 16 * wv + q;
+    at[q] = (diag && col > lane) ? A[lane * 64 + col] : A[col * 64 + lane];
+  }
   if (tid < 128) {
     const int64_t idx = int64_t((tid >> 6) ? I : J) * 64 + lane;
     const double xv = idx < nc ? x[idx] : 0.0;

@@ -65,3 +65,26 @@ def test_grouped_coarse_config_switch_and_graphs(gpu_available):
     assert r_eager.status == SolverStatus.SUCCESS
     assert np.array_equal(r_eager.x, r_graph.x) and np.array_equal(r_graph.x, r_graph2.x)
     assert np.linalg.norm(r_eager.x - xd) <= 1e-10 * np.linalg.norm(xd)
+
+
+@pytest.fixture(scope="module")
+def c16x4_oracle():
+    p = W.make_problem(16, 4, seed=3, name="c16x4")
+    return p, ofetidp.DenseFETIDP(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa,
+                                  p.tet_subdomain, p.pec_pairs, p.jq, scaling="stiffness")
+
+
+@pytest.mark.parametrize("coarse", ["dense", "grouped"])
+def test_coarse_solve_matches_oracle_several_tiles(gpu_available, c16x4_oracle, coarse):
+    """n_c = 432 (7 coarse tile columns): F through the dense -S~^-1 tiles (diagonal tiles
+    read through their lower triangle) or the grouped sweeps equals the oracle's F, whose
+    coarse solve is a dense LAPACK solve with S~."""
+    p, o = c16x4_oracle
+    print(f"[coarse] oracle F, {coarse}", flush=True)
+    s = _solver(p, coarse_solve=coarse, coarse_group=2, scaling="stiffness")
+    assert s.sizes["n_primal"] == 432
+    rng = np.random.default_rng(11)
+    for _ in range(2):
+        v = rng.standard_normal(s.sizes["n_lambda"])
+        q, qo = s.apply_dual_operator(v), o.apply_F(v)
+        assert np.linalg.norm(q - qo) <= 1e-10 * np.linalg.norm(qo)
