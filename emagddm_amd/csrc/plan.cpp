@@ -65,6 +65,72 @@ void longest_first(std::vector<TileWork>& list) {
   });
 }
 
+// Work-groups of a launch go to the GPU's eight XCDs in turn (work-group b to XCD b % 8),
+// and each XCD has its own L2.  Items that read the same panels (one matrix's tiles of
+// one column J, or one matrix's Schur tiles, key = (mat, J) or mat) are therefore placed
+// at positions of one residue mod 8, consecutively, so they run on one XCD at about the
+// same time and the shared panel is fetched from HBM once per group instead of once per
+// item.  Groups are dealt to the eight residues longest-total-work first, each to the
+// residue with the least work so far; within a residue the groups keep longest-first
+// order (by their longest item), and so do the items of a group.  Positions past the
+// shortest residue's length hold the longer residues' leftovers in the same round-robin
+// order.  Only the order changes: the arithmetic of every item is that of any other order.
+void xcd_grouped(std::vector<TileWork>& list, bool by_matrix) {
+  if (list.size() < 16) {
+    longest_first(list);
+    return;
+  }
+  longest_first(list);
+  std::vector<int64_t> key(list.size());
+  for (size_t i = 0; i < list.size(); ++i)
+    key[i] = by_matrix ? int64_t(list[i].mat) : (int64_t(list[i].mat) << 20) | list[i].J;
+  // groups in order of their longest item (the list is longest first)
+  std::vector<int64_t> order(list.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = int64_t(i);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  struct Group {
+    std::vector<int32_t> idx;
+    int first;      // position of its longest item in the longest-first list
+    double work;
+  };
+  std::vector<Group> groups;
+  for (size_t i = 0; i < order.size(); ++i) {
+    const int64_t it = order[i];
+    if (i == 0 || key[it] != key[order[i - 1]]) groups.push_back(Group{{}, int(it), 0.0});
+    Group& g = groups.back();
+    g.idx.push_back(int32_t(it));
+    g.first = std::min(g.first, int(it));
+    g.work += double(list[it].k1 - list[it].k0) + 2.0;
+  }
+  for (Group& g : groups) std::sort(g.idx.begin(), g.idx.end());  // longest first inside
+  std::vector<int32_t> by_work(groups.size());
+  for (size_t g = 0; g < groups.size(); ++g) by_work[g] = int32_t(g);
+  std::stable_sort(by_work.begin(), by_work.end(),
+                   [&](int32_t a, int32_t b) { return groups[a].work > groups[b].work; });
+  constexpr int kXcd = 8;
+  std::vector<double> load(kXcd, 0.0);
+  std::vector<std::vector<int32_t>> res_groups(kXcd);
+  for (int32_t g : by_work) {
+    const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += groups[g].work;
+    res_groups[x].push_back(g);
+  }
+  std::vector<std::vector<int32_t>> queue(kXcd);
+  for (int x = 0; x < kXcd; ++x) {
+    std::sort(res_groups[x].begin(), res_groups[x].end(),
+              [&](int32_t a, int32_t b) { return groups[a].first < groups[b].first; });
+    for (int32_t g : res_groups[x])
+      queue[x].insert(queue[x].end(), groups[g].idx.begin(), groups[g].idx.end());
+  }
+  std::vector<TileWork> out;
+  out.reserve(list.size());
+  std::vector<size_t> pos(kXcd, 0);
+  while (out.size() < list.size())
+    for (int x = 0; x < kXcd; ++x)
+      if (pos[x] < queue[x].size()) out.push_back(list[size_t(queue[x][pos[x]++])]);
+  list.swap(out);
+}
+
 }  // namespace
 
 // Incremental diagonal (inc = true): the column-step workgroups (mode 1, flags & 1)
@@ -634,7 +700,7 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
             schur1_of(s, early);
             schur1_done[size_t(s)] = 1;
           }
-        longest_first(early);
+        xcd_grouped(early, true);
         add_tiles(S, 4, early);
         // their S_DD snapshot can follow at once (the later launches do not touch them)
         const int64_t off = int64_t(P.snap_tiles.size());
@@ -645,13 +711,13 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
-      longest_first(below);
+      xcd_grouped(below, false);
       add_tiles(S, 1, below);
     }
     std::vector<TileWork> schur1;
     for (int64_t s = 0; s < ns; ++s)
       if (!schur1_done[size_t(s)]) schur1_of(s, schur1);
-    longest_first(schur1);
+    xcd_grouped(schur1, true);
     add_tiles(S, 0, schur1);
     // snapshot S_DD for the preconditioner (the rest of the subdomains; all of them in the
     // left-looking variant, over the same list)
@@ -710,18 +776,18 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
             schur2_of(s, early);
             schur2_done[size_t(s)] = 1;
           }
-        longest_first(early);
+        xcd_grouped(early, true);
         add_tiles(S, 4, early);
       }
       add_tiles(S, 2, diag);
       add_tiles(S, 3, next);
-      longest_first(below);
+      xcd_grouped(below, false);
       add_tiles(S, 1, below);
     }
     std::vector<TileWork> schur2;
     for (int64_t s = 0; s < ns; ++s)
       if (!schur2_done[size_t(s)]) schur2_of(s, schur2);
-    longest_first(schur2);
+    xcd_grouped(schur2, true);
     add_tiles(S, 0, schur2);
   }
 
