@@ -2049,7 +2049,8 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
               const TileWork& w = sch[q]->items[L.off + i];
               const bool trsm = (L.mode == 1 || L.mode == 3) && !(w.flags & 2);  // not a lookahead
               if (!useful) {
-                fl += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + (trsm ? 1.0 + (w.flags & 1) : 0.0));
+                fl += 2.0 * kTile * kTile * kTile *
+                      (double(w.k1 - w.k0) + (trsm ? 1.0 + (w.flags & 1) + ((w.flags & 64) ? 1.0 : 0.0) : 0.0));
                 continue;
               }
               const double rI = real_rows(q, w.mat, w.I), rJ = real_rows(q, w.mat, w.J);
@@ -2057,6 +2058,7 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
               for (int K = w.k0; K < w.k1; ++K) kr += real_rows(q, w.mat, K);
               fl += 2.0 * rI * rJ * kr;
               if (trsm) fl += rI * rJ * rJ + ((w.flags & 1) ? rI * rI * rJ : 0.0);
+              if (trsm && (w.flags & 64)) fl += rI * rI * real_rows(q, w.mat, w.I - 2);
             }
         if (k < cap) out[k++] = fl;
       }

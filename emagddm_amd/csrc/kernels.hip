@@ -576,14 +576,17 @@ __device__ __forceinline__ double block_row_dot64(const T* M, int ld, const doub
 // MODE 1: same, then X = C L_JJ^-T with the stored inverse diagonal tile; with
 //         flags & 1 the diagonal tile of row I also takes its share C_II -= X X^T
 //         (column step, I > J; every WG of a launch owns a distinct row I, apart
-//         from a lookahead item, flags & 2, which only updates the next column's
-//         tile (J + 2, J + 1)).  Items with flags & 16 do the MODE 3 work; the
+//         from the lookahead items, flags & 2, which only update the next column's
+//         tile (J + 2, J + 1) or the diagonal tile (J + 2, J + 2) from K < J; in the
+//         FP64 schedule only MODE 3 items take a share).  Items with flags & 16 do the
+//         MODE 3 work; the
 //         schedule puts them at the front of the launch so they are dispatched first,
 //         and their waves run at raised priority (they are on the critical path).
 // MODE 2: same (empty K range when the diagonal was updated incrementally), then
 //         Cholesky of the diagonal tile and its inverse
-// MODE 3: MODE 1 for the item I = J + 1, whose share completes C_II, then the
-//         Cholesky and inverse of C_II for the next column step
+// MODE 3: MODE 1 for the item I = J + 1, whose share completes C_II (with flags & 64
+//         first the K = I - 2 term from the stored tile L_{I,I-2}), then the Cholesky
+//         and inverse of C_II for the next column step
 // The inverse diagonal tile of column J of matrix m lives at inv + inv_base[m] + J*4096.
 template <typename T, int MODE>
 __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
@@ -693,6 +696,12 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       auto shared_d = [&](int x, int y, int r, int j, int i) {
         return (from_c || dfresh) ? a3[x][y][r] : Cd[j * 64 + i] + a3[x][y][r];
       };
+      if (fac && (w.flags & 64)) {
+        // the diagonal's K = I - 2 term, C_II -= L_{I,I-2} L_{I,I-2}^T (tile final since
+        // the previous column launch; the earlier terms came from a diagonal lookahead)
+        const T* Tk = pool + ra[w.I] + (int64_t)(w.I - 2) * kTileElems;
+        tile_mma<T, 64, 64, true>(a3, Tk, Tk, pb0, qb0, lr, lk);
+      }
       // X[i][j] = sum_k C'[i][k] Linv[j][k]:  D2[p=j][q=i], A op = Linv (p,k), B op = C' (q,k)
       const T* Li = inv + inv_base[w.mat] + (int64_t)w.J * kTileElems;
       acc_t a2[2][2];

@@ -49,10 +49,13 @@ void add_tiles(TileSchedule& S, int mode, const std::vector<TileWork>& list) {
   // column-step launch that follows it
   if (mode == 3)
     for (size_t i = size_t(L.off); i < S.items.size(); ++i) S.items[i].flags |= 16;
-  // modes 1 and 3 add the TRSM product (and the diagonal-tile share when flagged);
-  // lookahead items (flags & 2) are update-only
+  // modes 1 and 3 add the TRSM product (and the diagonal-tile share when flagged, and
+  // with flags & 64 the diagonal's K = I - 2 term); lookahead items (flags & 2) are
+  // update-only
   for (const TileWork& w : list) {
-    const double extra = (mode == 1 || mode == 3) && !(w.flags & 2) ? 1.0 + (w.flags & 1) : 0.0;
+    const double extra = (mode == 1 || mode == 3) && !(w.flags & 2)
+                             ? 1.0 + (w.flags & 1) + ((w.flags & 64) ? 1.0 : 0.0)
+                             : 0.0;
     S.flops += 2.0 * kTile * kTile * kTile * (double(w.k1 - w.k0) + extra);
   }
 }
@@ -642,7 +645,6 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
   for (int variant = 0; variant < 2; ++variant) {
     const bool inc = variant == 0;
     TileSchedule& S = inc ? P.local_sched : P.local_sched_ll;
-    const int fl = inc ? 1 : 0;
     int maxTi = 0, maxTd = 0;
     for (const SubLayout& L : T.sub) {
       maxTi = std::max(maxTi, L.Ti);
@@ -652,6 +654,11 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
     // Lookahead (incremental variant): the mode-1 launch of column j also applies
     // K in [.., j) to the next column's mode-3 tile (j + 2, j + 1) (flags 2, update
     // only), so the side-stream item keeps one product and its diagonal factorization.
+    // Diagonal tiles (FP64 variant): the same launch applies K in [f, j) to the diagonal
+    // tile (j + 2, j + 2) (update only, left-looking over its row panel); the mode-3 item
+    // of column j + 1 adds the K = j term (flags 64) and its own X X^T, so no other
+    // column-step item touches a diagonal tile, and trailing diagonal tiles take their
+    // sums in the Schur lists.
     // Schur updates of a subdomain need only its own interior columns; once the column
     // launches no longer fill the GPU (fewer items than kSideFill), the Schur updates of
     // the subdomains already done ride beside them (mode 4: a mode-0 launch that may run
@@ -662,7 +669,7 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
       if (L.Ti == 0) return;
       const int32_t* ft = &P.ftile[P.ftile_off[s]];
       for (int J = L.Ti; J < L.Tc; ++J)
-        for (int I = inc ? J + 1 : J; I < L.Tc; ++I) {
+        for (int I = J; I < L.Tc; ++I) {
           const int k0 = std::max(ft[I], ft[J]);
           if (k0 < L.Ti) out.push_back(TileWork{int32_t(s), I, J, k0, L.Ti, 0});
         }
@@ -682,14 +689,19 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
           diag.push_back(TileWork{int32_t(s), j, j, inc ? 0 : ft[j], inc ? 0 : j, 0});
         for (int I = j + 1; I < L.Tc; ++I)
           if (ft[I] <= j) {
-            TileWork w{int32_t(s), I, j, std::max(ft[I], ft[j]), j, fl};
+            TileWork w{int32_t(s), I, j, std::max(ft[I], ft[j]), j, 0};
             const bool is_next = inc && I == j + 1 && I < L.Ti;
-            if (is_next && j >= 1 && w.k0 < j - 1) w.k0 = j - 1;  // rest applied by lookahead
+            if (is_next) {
+              // diagonal share X X^T, and the K = j - 1 term of the diagonal when stored
+              w.flags = 1 | (j >= 1 && ft[I] <= j - 1 ? 64 : 0);
+              if (j >= 1 && w.k0 < j - 1) w.k0 = j - 1;  // rest applied by lookahead
+            }
             (is_next ? next : below).push_back(w);
           }
         if (inc && j + 2 < L.Ti) {
           const int k0 = std::max(ft[j + 2], ft[j + 1]);
           if (k0 < j) below.push_back(TileWork{int32_t(s), j + 2, j + 1, k0, j, 2});
+          if (ft[j + 2] < j) below.push_back(TileWork{int32_t(s), j + 2, j + 2, ft[j + 2], j, 2});
         }
       }
       if (inc && int64_t(diag.size() + next.size() + below.size()) < kSideFill &&
@@ -738,7 +750,7 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
       if (L.Tr == L.Ti) return;
       const int32_t* ft = &P.ftile[P.ftile_off[s]];
       for (int J = L.Tr; J < L.T; ++J)
-        for (int I = inc ? J + 1 : J; I < L.T; ++I) {
+        for (int I = J; I < L.T; ++I) {
           const int k0 = std::max(L.Ti, std::max(ft[I], ft[J]));
           if (k0 < L.Tr) out.push_back(TileWork{int32_t(s), I, J, k0, L.Tr, 0});
         }
@@ -758,14 +770,19 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
           diag.push_back(TileWork{int32_t(s), col, col, inc ? 0 : std::max(L.Ti, ft[col]), inc ? 0 : col, 0});
         for (int I = col + 1; I < L.T; ++I)
           if (ft[I] <= col) {
-            TileWork w{int32_t(s), I, col, std::max(L.Ti, std::max(ft[I], ft[col])), col, fl};
+            TileWork w{int32_t(s), I, col, std::max(L.Ti, std::max(ft[I], ft[col])), col, 0};
             const bool is_next = inc && I == col + 1 && I < L.Tr;
-            if (is_next && col >= 1 && w.k0 < col - 1) w.k0 = col - 1;  // rest applied by lookahead
+            if (is_next) {
+              w.flags = 1 | (col - 1 >= std::max(L.Ti, ft[I]) ? 64 : 0);
+              if (col >= 1 && w.k0 < col - 1) w.k0 = col - 1;  // rest applied by lookahead
+            }
             (is_next ? next : below).push_back(w);
           }
         if (inc && col + 2 < L.Tr) {
           const int k0 = std::max(L.Ti, std::max(ft[col + 2], ft[col + 1]));
           if (k0 < col) below.push_back(TileWork{int32_t(s), col + 2, col + 1, k0, col, 2});
+          const int kd = std::max(L.Ti, ft[col + 2]);
+          if (kd < col) below.push_back(TileWork{int32_t(s), col + 2, col + 2, kd, col, 2});
         }
       }
       if (inc && !early2 && int64_t(diag.size() + next.size() + below.size()) < kSideFill) {
@@ -856,6 +873,7 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
       if (L.mode != 1 && L.mode != 3) continue;
       for (int64_t i = L.off; i < L.off + L.n; ++i) {
         const TileWork& w = S.items[size_t(i)];
+        if (w.I == w.J && (w.flags & 2)) continue;  // a diagonal lookahead (checked below)
         const SubLayout& Ls = T.sub[w.mat];
         check(w.J < Ls.Tr && w.I > w.J, "local column-step item");
         const int32_t* ft = &P.ftile[P.ftile_off[w.mat]];
@@ -875,6 +893,51 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
           const int32_t nk = next_k[size_t(toff[s] + int64_t(I) * Ls.Tr + J)];
           check(nk == J, "local K ranges complete (and every tile has its column step)");
         }
+    }
+  }
+  // every diagonal tile receives its K range exactly once, in launch order: from the
+  // start of its row's envelope (Schur items and diagonal lookaheads apply ranges, a
+  // mode-3 item the K = I - 2 term when flagged and its own K = I - 1 share) to its own
+  // column (eliminated rows) or to the end of the eliminated columns (trailing rows)
+  {
+    const TileSchedule& S = P.local_sched;
+    std::vector<int64_t> roff(size_t(ns) + 1, 0);
+    for (int64_t s = 0; s < ns; ++s) roff[size_t(s) + 1] = roff[size_t(s)] + T.sub[s].T;
+    std::vector<int32_t> next_d(size_t(roff[size_t(ns)]), -1);
+    auto start_of = [&](int64_t s, int I) {
+      const SubLayout& Ls = T.sub[s];
+      const int f = P.ftile[P.ftile_off[s] + I];
+      return I >= Ls.Tc ? std::max(Ls.Ti, f) : f;  // D' rows are zero in interior columns
+    };
+    auto apply = [&](int64_t s, int I, int k0, int k1) {
+      if (k1 <= k0) return;
+      int32_t& nk = next_d[size_t(roff[s] + I)];
+      if (nk < 0) nk = start_of(s, I);
+      check(k0 == nk, "local diagonal K ranges contiguous");
+      nk = k1;
+    };
+    for (const TileLaunch& L : S.launches) {
+      if (L.mode < 0) continue;
+      for (int64_t i = L.off; i < L.off + L.n; ++i) {
+        const TileWork& w = S.items[size_t(i)];
+        if (w.I == w.J) {
+          apply(w.mat, w.I, w.k0, w.k1);
+        } else if (L.mode == 3 || (w.flags & 16)) {
+          if (w.flags & 64) apply(w.mat, w.I, w.I - 2, w.I - 1);
+          apply(w.mat, w.I, w.J, w.J + 1);
+        } else {
+          check(!(w.flags & (1 | 64)), "diagonal share outside a mode-3 item");
+        }
+      }
+    }
+    for (int64_t s = 0; s < ns; ++s) {
+      const SubLayout& Ls = T.sub[s];
+      for (int I = 0; I < Ls.T; ++I) {
+        const int end = std::min(I, Ls.Tr), st = start_of(s, I);
+        if (st >= end) continue;
+        const int32_t nk = next_d[size_t(roff[s] + I)];
+        check(nk == end, "local diagonal K ranges complete");
+      }
     }
   }
   // a mode-3 item completes and factors the diagonal tile of the next column
