@@ -467,7 +467,8 @@ __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
                                           int ktot, T* sA, T* sB, int tid, int pb0, int qb0,
                                           int lr, int lk) {
   // DEPTH chunks are in flight in registers (2: one more chunk's loads overlap a chunk's
-  // products, for the long K loops of the Schur updates)
+  // products; every mode uses 2, which fits the column step's 128-register budget since
+  // only the factoring items hold a diagonal accumulator)
   constexpr int KC = 16, CH = 64 * KC;
   const int nst = ktot / KC;
   if (nst == 0) return;
@@ -646,7 +647,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       if (wave == 0) bpre = rhs.b[o + (int64_t)w.I * 64 + lane];
     }
   }
-  panel_mma<T, MODE == 0 ? 2 : 1>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
+  panel_mma<T, 2>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
                pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, lds_panel,
                lds_panel + kStage, tid, pb0, qb0, lr, lk);
 
