@@ -208,8 +208,9 @@ def cpu_baseline(prob, system, sample_iters):
     """Reference CGSolver (Jacobi, tol 1e-8, x0 = 0) from oracle/_ref on the same free-dof
     system (the GPU-assembled CSR, bit-identical in structure and equal to 1e-13 to the
     oracle's), timed on a bounded prefix of its iterations.  When the full-run iteration
-    count was measured (tests/golden/<workload>_cg.json) the time is scaled to it;
-    otherwise only the per-iteration cost is reported."""
+    count was measured (tests/golden/<workload>_cg.json) the time is scaled to it; when
+    that run stopped short of 1e-8 the scaled time is reported as a lower bound; with no
+    record only the per-iteration cost is reported."""
     import ctypes as C
     import numpy as np
     gpath = os.path.join(ROOT, "tests", "golden", f"{prob.name}_cg.json")
@@ -243,8 +244,18 @@ def cpu_baseline(prob, system, sample_iters):
                     sample=f"{its} of {full} CG iterations on the same {n}-dof system "
                            f"({ms / 1e3:.2f} s measured, scaled by the full-run count "
                            f"measured in the build container)")
-    note = ("full run did not reach 1e-8 within " + str(gold["iterations"]) + " iterations"
-            if gold is not None else "full-run iteration count not measured")
+    if gold is not None:
+        # the reference path did not reach 1e-8 within the recorded count: its time to
+        # 1e-8 is at least that count times the measured cost of one iteration
+        lim = int(gold["iterations"])
+        return dict(value=round(per_it * lim, 3),
+                    unit="s to 1e-8, lower bound (Jacobi-PCG, reference CGSolver)",
+                    cores=1, kind=kind,
+                    sample=f"{its} CG iterations on the same {n}-dof system ({ms / 1e3:.2f} s, "
+                           f"{per_it * 1e3:.1f} ms each) times {lim}: the full run had not reached "
+                           f"1e-8 after {lim} iterations in the build container (relative "
+                           f"residual {float(gold['residual']):.2e} then)")
+    note = "full-run iteration count not measured"
     return dict(value=round(per_it, 5), unit="s per Jacobi-PCG iteration (reference CGSolver)",
                 cores=1, kind=kind,
                 sample=f"{its} CG iterations on the same {n}-dof system ({ms / 1e3:.2f} s, "

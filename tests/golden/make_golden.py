@@ -6,6 +6,7 @@ seeded problem's input checksums and the reference solvers' outputs.
 
     python tests/golden/make_golden.py            # micro4, micro8c, cube16
     python tests/golden/make_golden.py cube48j    # full-size summary (minutes)
+    python tests/golden/make_golden.py cg cube96j 20000   # reference CG iteration count
 """
 import ctypes as C
 import os
@@ -83,6 +84,38 @@ def main_large(name="cube48j"):
     print(name, n, res, fm.value, sm.value)
 
 
+def main_cg(name, max_iterations):
+    """The reference CGSolver (Jacobi, tol 1e-8, x0 = 0) on the full free-dof system of a
+    workload, as <name>_cg.json: its status, iteration count and final relative residual
+    (bench.py times a bounded prefix of these iterations on the GPU host)."""
+    import json
+    import time
+    lib = ref_lib()
+    p = W.make_workload(name)
+    asm = assembly.assemble(p.xyz, p.tet2node, p.tet_material, p.nu, p.kappa, p.pec_pairs, p.jq)
+    A, f = asm["A"], asm["f"]
+    del asm
+    n = A.shape[0]
+    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    del A
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    x = np.zeros(n)
+    it, res, sm = C.c_int(), C.c_double(), C.c_double()
+    t0 = time.perf_counter()
+    st = lib.ref_solve_cg(n, ip(rp), ip(col), dp(val), dp(f), C.c_double(1e-8), int(max_iterations),
+                          dp(x), C.byref(it), C.byref(res), C.byref(sm))
+    out = {"workload": name, "solver": "reference CGSolver + Jacobi (oracle/_ref), tol 1e-8, x0 = 0",
+           "status": int(st), "iterations": int(it.value), "residual": float(res.value),
+           "solve_ms_container_1core": round(sm.value, 1),
+           "wall_s": round(time.perf_counter() - t0, 1), "n_free": int(n),
+           "note": ("full-run iteration count of the reference path" if st == 0 else
+                    f"the reference path had not reached 1e-8 after {int(it.value)} iterations")}
+    with open(os.path.join(HERE, f"{name}_cg.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(out)
+
+
 def main():
     lib = ref_lib()
     cases = {
@@ -99,7 +132,9 @@ def main():
 
 
 if __name__ == "__main__":
-    if len(sys.argv) > 1:
+    if len(sys.argv) > 3 and sys.argv[1] == "cg":
+        main_cg(sys.argv[2], int(sys.argv[3]))
+    elif len(sys.argv) > 1:
         main_large(sys.argv[1])
     else:
         main()
