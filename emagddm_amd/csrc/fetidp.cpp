@@ -1187,8 +1187,8 @@ struct fetidp_solver {
     }
     launch_fill_U(topo.n_sub, std::max(plans.max_ndp, plans.max_ncp), devsub.p, d_lam.p, ubar.p, c_loc.p, uc.p, U.p, st);
     timed(13, [&] {
-      launch_interior<T>(topo.n_sub, plans.max_Ti, devsub.p, pool.p, rowabs.p, inv.p, local_inv_base.p,
-                         ftile.p, yvec.p, U.p, tbuf.p, vout.p, st);
+      launch_interior<T>(topo.n_sub, plans.max_Ti, plans.bwd_width.data(), devsub.p, pool.p, rowabs.p, inv.p,
+                         local_inv_base.p, ftile.p, yvec.p, U.p, tbuf.p, vout.p, st);
     });
     launch_gather_global(topo.n_free, free_kind.p, free_a.p, vout.p, ubar.p, uc.p, xout, st);
   }

@@ -109,10 +109,13 @@ void launch_zero_interior(int ns, int max_Ti, const DevSub* subs, double* v, hip
 void launch_kind_sq(int64_t n, const int8_t* kind, int want, const double* x, double* partial,
                     hipStream_t st);
 // Interior solve: xout_i = L_ii^-T (yv_i - L_{D,c; i}^T U); tbuf is scratch (vec space).
+// width (host, max_Ti entries): the backward step of row J launches width[J] work-groups
+// per subdomain (rows J - 1, J - 2, ...).
 template <typename T>
-void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const int64_t* rowabs,
-                     const T* inv, const int64_t* inv_base, const int32_t* ftile, const double* yv,
-                     const double* U, double* tbuf, double* xout, hipStream_t st);
+void launch_interior(int ns, int max_Ti, const int32_t* width, const DevSub* subs, const T* pool,
+                     const int64_t* rowabs, const T* inv, const int64_t* inv_base,
+                     const int32_t* ftile, const double* yv, const double* U, double* tbuf,
+                     double* xout, hipStream_t st);
 // rho[slot] = pool diagonal at each D slot; stiffness scaling weights and 0.5 means.
 void launch_scaling(int64_t ndv, int64_t nl, const int64_t* rho_pos, const void* pool, int is_f32,
                     double* rho, const int64_t* lam_pos, const double* sgn, double* sgnD,

@@ -1555,8 +1555,9 @@ __global__ __launch_bounds__(1024) void k_int_prep(const DevSub* __restrict__ su
 }
 
 // Backward step J >= 1 (x_J known): rows I < J of the envelope take t_I -= L_JI^T x_J
-// (work-group I reads tile (J, I) of the contiguous tile row J); the work-group of row
-// J - 1, whose t is then complete, also forms x_{J-1} = L_{J-1,J-1}^-T t_{J-1}.
+// (work-group x reads tile (J, I = J - 1 - x) of the contiguous tile row J); the
+// work-group of row J - 1, whose t is then complete, also forms x_{J-1} =
+// L_{J-1,J-1}^-T t_{J-1}.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
@@ -1568,7 +1569,8 @@ __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ sub
                                                   double* __restrict__ xout) {
   const DevSub S = subs[blockIdx.y];
   if (J >= S.Ti) return;
-  const int I = blockIdx.x;
+  const int I = J - 1 - int(blockIdx.x);
+  if (I < 0) return;
   const bool in_env = I >= ftile[S.ft_off + J];  // tile (J, I) is stored
   if (I >= J || (!in_env && I != J - 1)) return;
   const int tid = threadIdx.x;
@@ -1599,22 +1601,23 @@ __global__ __launch_bounds__(256) void k_bwd_step(const DevSub* __restrict__ sub
 }
 
 template <typename T>
-void launch_interior(int ns, int max_Ti, const DevSub* subs, const T* pool, const int64_t* rowabs,
-                     const T* inv, const int64_t* inv_base, const int32_t* ftile, const double* yv,
-                     const double* U, double* tbuf, double* xout, hipStream_t st) {
+void launch_interior(int ns, int max_Ti, const int32_t* width, const DevSub* subs, const T* pool,
+                     const int64_t* rowabs, const T* inv, const int64_t* inv_base,
+                     const int32_t* ftile, const double* yv, const double* U, double* tbuf,
+                     double* xout, hipStream_t st) {
   if (ns <= 0 || max_Ti <= 0) return;
   hipLaunchKernelGGL(k_int_prep<T>, dim3(max_Ti, ns), dim3(1024), 0, st, subs, pool, rowabs, inv,
                      inv_base, ftile, yv, U, tbuf, xout);
   for (int J = max_Ti - 1; J >= 1; --J)
-    hipLaunchKernelGGL(k_bwd_step<T>, dim3(J, ns), dim3(256), 0, st, subs, pool, rowabs, inv,
-                       inv_base, ftile, J, tbuf, xout);
+    hipLaunchKernelGGL(k_bwd_step<T>, dim3(std::max(1, std::min(J, width[J])), ns), dim3(256), 0, st,
+                       subs, pool, rowabs, inv, inv_base, ftile, J, tbuf, xout);
 }
-template void launch_interior<double>(int, int, const DevSub*, const double*, const int64_t*,
-                                      const double*, const int64_t*, const int32_t*, const double*,
-                                      const double*, double*, double*, hipStream_t);
-template void launch_interior<float>(int, int, const DevSub*, const float*, const int64_t*,
-                                     const float*, const int64_t*, const int32_t*, const double*,
-                                     const double*, double*, double*, hipStream_t);
+template void launch_interior<double>(int, int, const int32_t*, const DevSub*, const double*,
+                                      const int64_t*, const double*, const int64_t*, const int32_t*,
+                                      const double*, const double*, double*, double*, hipStream_t);
+template void launch_interior<float>(int, int, const int32_t*, const DevSub*, const float*,
+                                     const int64_t*, const float*, const int64_t*, const int32_t*,
+                                     const double*, const double*, double*, double*, hipStream_t);
 
 __global__ __launch_bounds__(256) void k_rho(int64_t n, const int64_t* __restrict__ pos,
                                              const void* __restrict__ pool, int is_f32,

@@ -1013,6 +1013,14 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
       P.max_Ti = std::max(P.max_Ti, T.sub[s].Ti);
     }
     P.inv_elems = std::max<int64_t>(off, kTileElems);
+    // backward interior sweep: step J touches rows I in [f_J, J) of every subdomain with
+    // J < Ti (and always I = J - 1); its launch needs the widest such range
+    P.bwd_width.assign(size_t(std::max(P.max_Ti, 1)), 0);
+    for (int64_t s = 0; s < ns; ++s) {
+      const int32_t* ft = &P.ftile[P.ftile_off[s]];
+      for (int J = 1; J < T.sub[s].Ti; ++J)
+        P.bwd_width[size_t(J)] = std::max(P.bwd_width[size_t(J)], J - std::min(ft[J], J - 1));
+    }
     P.coarse_inv_elems = P.coarse_ncp;  // (ncp / 64) tiles of 4096 = ncp * 64 elements
     P.coarse_inv_elems *= kTile;
   }

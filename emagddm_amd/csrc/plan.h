@@ -98,6 +98,8 @@ struct Plans {
   int max_ndp = 0, max_ncp = 0, max_m = 0, max_wrows = 0;
   int max_T = 0, max_Tr = 0, max_Ti = 0, min_Ti = 0;
   std::vector<int64_t> inv_off;  // per subdomain: offset of its inverse diagonal tiles
+  std::vector<int32_t> bwd_width;  // per interior tile row J: max over subdomains of J - f_J
+                                   // (rows a backward interior step touches)
   std::vector<int32_t> ftile;    // per subdomain and tile row: first nonzero tile column
   std::vector<int64_t> ftile_off;     // per subdomain: offset of its tile rows in ftile/rowabs
   std::vector<int64_t> rowabs;        // per (subdomain, tile row I): element offset of tile (I, 0)
