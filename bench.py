@@ -175,8 +175,9 @@ def roofline(v, precision, workload):
     bound = "mfma" if intensity is None or intensity >= ridge else "hbm"
     out = dict(bound=bound, achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                frac=round(achieved / peak, 4), traffic=traffic, kernel=kernel,
-               flop_unit="executed 64x64 tile products (padding, full diagonal tiles and the "
-                         "TRSM as a product by the stored inverse included)",
+               flop_unit="executed 64x64 tile products (padding and full diagonal tiles "
+                         "included; the TRSM as a product by the stored lower-triangular "
+                         "inverse, 160 of its 256 MFMAs issued)",
                launches_per_step=launches // PROFILE_STEPS,
                flops_per_launch=per_launch_flops, avg_launch_ms=avg_ms,
                # the same launches counted in useful flops: real rows only, triangular

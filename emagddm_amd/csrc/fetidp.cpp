@@ -2035,9 +2035,10 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
     }
     return double(std::min<int64_t>(std::max<int64_t>(real, 0), kTile));
   };
-  // executed: 2 * 64^3 per tile product (TRSM as a product with the stored inverse, the
-  // full diagonal-tile share); useful: the same products on the real rows only, the TRSM
-  // as a triangular solve (r_I r_J^2) and the lower half of the diagonal share (r_I^2 r_J)
+  // executed: 2 * 64^3 per tile product as issued (the TRSM as a product with the stored
+  // lower-triangular inverse, 160 of 256 MFMAs; the full diagonal-tile share); useful: the
+  // same products on the real rows only, the TRSM as a triangular solve (r_I r_J^2) and
+  // the lower half of the diagonal share (r_I^2 r_J)
   for (int useful = 0; useful < 2; ++useful)
     for (int q = 0; q < 2; ++q)
       for (int mode = 0; mode < 3; ++mode) {
@@ -2050,7 +2051,7 @@ int32_t fetidp_get_profile(fetidp_handle h, double* out, int32_t cap) {
               const bool trsm = (L.mode == 1 || L.mode == 3) && !(w.flags & 2);  // not a lookahead
               if (!useful) {
                 fl += 2.0 * kTile * kTile * kTile *
-                      (double(w.k1 - w.k0) + (trsm ? 1.0 + (w.flags & 1) + ((w.flags & 64) ? 1.0 : 0.0) : 0.0));
+                      (double(w.k1 - w.k0) + (trsm ? 0.625 + (w.flags & 1) + ((w.flags & 64) ? 1.0 : 0.0) : 0.0));
                 continue;
               }
               const double rI = real_rows(q, w.mat, w.I), rJ = real_rows(q, w.mat, w.J);

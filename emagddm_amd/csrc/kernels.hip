@@ -310,6 +310,47 @@ __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[2][2], c
   }
 }
 
+// The same product when the A tile is lower triangular (A(p, k) = 0 for k > p, as a
+// stored inverse diagonal tile is, with exact zeros): the 16-row block x of this wave
+// (rows pb0 + 16 x ..) needs k < pb0 + 16 (x + 1) only, so the steps past that are left
+// out -- the products they would add are exact zeros.  160 of the 256 MFMAs of the four
+// waves; the same terms in the same order as tile_mma.
+template <typename T, int LDA, int LDB, int PB0>
+__device__ __forceinline__ void tile_mma_lower_a_at(typename Mfma<T>::acc_t (&acc)[2][2],
+                                                    const T* Ap, const T* Bq, int qb0, int lr,
+                                                    int lk) {
+#pragma unroll
+  for (int kk = 0; kk < PB0 + 16; kk += 4) {
+    const int kc = kk + lk;
+    const T a0 = Ap[kc * LDA + PB0 + lr];
+    const T a1 = Ap[kc * LDA + PB0 + 16 + lr];
+    const T b0 = Bq[kc * LDB + qb0 + lr];
+    const T b1 = Bq[kc * LDB + qb0 + 16 + lr];
+    acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+  }
+#pragma unroll
+  for (int kk = PB0 + 16; kk < PB0 + 32; kk += 4) {
+    const int kc = kk + lk;
+    const T a1 = Ap[kc * LDA + PB0 + 16 + lr];
+    const T b0 = Bq[kc * LDB + qb0 + lr];
+    const T b1 = Bq[kc * LDB + qb0 + 16 + lr];
+    acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+  }
+}
+template <typename T, int LDA, int LDB>
+__device__ __forceinline__ void tile_mma_lower_a(typename Mfma<T>::acc_t (&acc)[2][2],
+                                                 const T* Ap, const T* Bq, int pb0, int qb0,
+                                                 int lr, int lk) {
+  if (pb0 == 0)  // uniform per wave
+    tile_mma_lower_a_at<T, LDA, LDB, 0>(acc, Ap, Bq, qb0, lr, lk);
+  else
+    tile_mma_lower_a_at<T, LDA, LDB, 32>(acc, Ap, Bq, qb0, lr, lk);
+}
+
 __device__ __forceinline__ double readlane(double v, int lane) {
   const unsigned long long u = __double_as_longlong(v);
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
@@ -710,7 +751,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) a2[x][y] = acc_t{0, 0, 0, 0};
-      tile_mma<T, 64, kLd>(a2, Li, s, pb0, qb0, lr, lk);
+      tile_mma_lower_a<T, 64, kLd>(a2, Li, s, pb0, qb0, lr, lk);
       __syncthreads();  // every wave is done reading C' from s
 #pragma unroll
       for (int x = 0; x < 2; ++x)
