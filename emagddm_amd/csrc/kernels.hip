@@ -290,17 +290,18 @@ void launch_mfma_probe(int is_f32, double* out) {
     hipLaunchKernelGGL(k_mfma_probe<double>, dim3(1), dim3(64), 0, nullptr, out);
 }
 
-// acc (32x32 per wave, 2x2 MFMA blocks) += A_p,k * B_k,q over one 64-deep tile (-=
-// with NEG), A operand read from Ap (column-major 64x64, element (p,k) at k*lda + p),
-// B operand from Bq (element (q,k) at k*ldb + q).
+// acc (2x2 MFMA blocks per wave: rows p0 + 0..15 and p1 + 0..15, columns qb0 + 0..31)
+// += A_p,k * B_k,q over one 64-deep tile (-= with NEG), A operand read from Ap
+// (column-major 64x64, element (p,k) at k*lda + p), B operand from Bq (element (q,k) at
+// k*ldb + q).
 template <typename T, int LDA, int LDB, bool NEG = false>
 __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[2][2], const T* Ap,
-                                         const T* Bq, int pb0, int qb0, int lr, int lk) {
+                                         const T* Bq, int p0, int p1, int qb0, int lr, int lk) {
 #pragma unroll 4
   for (int kk = 0; kk < 64; kk += 4) {
     const int kc = kk + lk;
-    const T a0 = NEG ? -Ap[kc * LDA + pb0 + lr] : Ap[kc * LDA + pb0 + lr];
-    const T a1 = NEG ? -Ap[kc * LDA + pb0 + 16 + lr] : Ap[kc * LDA + pb0 + 16 + lr];
+    const T a0 = NEG ? -Ap[kc * LDA + p0 + lr] : Ap[kc * LDA + p0 + lr];
+    const T a1 = NEG ? -Ap[kc * LDA + p1 + lr] : Ap[kc * LDA + p1 + lr];
     const T b0 = Bq[kc * LDB + qb0 + lr];
     const T b1 = Bq[kc * LDB + qb0 + 16 + lr];
     acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
@@ -311,19 +312,19 @@ __device__ __forceinline__ void tile_mma(typename Mfma<T>::acc_t (&acc)[2][2], c
 }
 
 // The same product when the A tile is lower triangular (A(p, k) = 0 for k > p, as a
-// stored inverse diagonal tile is, with exact zeros): the 16-row block x of this wave
-// (rows pb0 + 16 x ..) needs k < pb0 + 16 (x + 1) only, so the steps past that are left
-// out -- the products they would add are exact zeros.  160 of the 256 MFMAs of the four
-// waves; the same terms in the same order as tile_mma.
-template <typename T, int LDA, int LDB, int PB0>
+// stored inverse diagonal tile is, with exact zeros): the 16-row block b of a wave needs
+// k < 16 (b + 1) only, so the steps past that are left out -- the products they would
+// add are exact zeros.  The waves hold row blocks {0, 3} or {1, 2} (see tile_update), so
+// each issues 40 of its 64 MFMAs; the same terms in the same order as tile_mma.
+template <typename T, int LDA, int LDB, int B0, int B1>
 __device__ __forceinline__ void tile_mma_lower_a_at(typename Mfma<T>::acc_t (&acc)[2][2],
                                                     const T* Ap, const T* Bq, int qb0, int lr,
                                                     int lk) {
 #pragma unroll
-  for (int kk = 0; kk < PB0 + 16; kk += 4) {
+  for (int kk = 0; kk < 16 * (B0 + 1); kk += 4) {
     const int kc = kk + lk;
-    const T a0 = Ap[kc * LDA + PB0 + lr];
-    const T a1 = Ap[kc * LDA + PB0 + 16 + lr];
+    const T a0 = Ap[kc * LDA + 16 * B0 + lr];
+    const T a1 = Ap[kc * LDA + 16 * B1 + lr];
     const T b0 = Bq[kc * LDB + qb0 + lr];
     const T b1 = Bq[kc * LDB + qb0 + 16 + lr];
     acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
@@ -332,9 +333,9 @@ __device__ __forceinline__ void tile_mma_lower_a_at(typename Mfma<T>::acc_t (&ac
     acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
   }
 #pragma unroll
-  for (int kk = PB0 + 16; kk < PB0 + 32; kk += 4) {
+  for (int kk = 16 * (B0 + 1); kk < 16 * (B1 + 1); kk += 4) {
     const int kc = kk + lk;
-    const T a1 = Ap[kc * LDA + PB0 + 16 + lr];
+    const T a1 = Ap[kc * LDA + 16 * B1 + lr];
     const T b0 = Bq[kc * LDB + qb0 + lr];
     const T b1 = Bq[kc * LDB + qb0 + 16 + lr];
     acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
@@ -343,12 +344,12 @@ __device__ __forceinline__ void tile_mma_lower_a_at(typename Mfma<T>::acc_t (&ac
 }
 template <typename T, int LDA, int LDB>
 __device__ __forceinline__ void tile_mma_lower_a(typename Mfma<T>::acc_t (&acc)[2][2],
-                                                 const T* Ap, const T* Bq, int pb0, int qb0,
+                                                 const T* Ap, const T* Bq, int p0, int qb0,
                                                  int lr, int lk) {
-  if (pb0 == 0)  // uniform per wave
-    tile_mma_lower_a_at<T, LDA, LDB, 0>(acc, Ap, Bq, qb0, lr, lk);
+  if (p0 == 0)  // uniform per wave: row blocks {0, 3}, else {1, 2}
+    tile_mma_lower_a_at<T, LDA, LDB, 0, 3>(acc, Ap, Bq, qb0, lr, lk);
   else
-    tile_mma_lower_a_at<T, LDA, LDB, 32>(acc, Ap, Bq, qb0, lr, lk);
+    tile_mma_lower_a_at<T, LDA, LDB, 1, 2>(acc, Ap, Bq, qb0, lr, lk);
 }
 
 __device__ __forceinline__ double readlane(double v, int lane) {
@@ -505,8 +506,8 @@ __device__ __forceinline__ void tile_trtri(T* s, const T* rdg, T* tmp, int tid) 
 template <typename T, int DEPTH = 1>
 __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
                                           const T* __restrict__ Ap, const T* __restrict__ Bq,
-                                          int ktot, T* sA, T* sB, int tid, int pb0, int qb0,
-                                          int lr, int lk) {
+                                          int ktot, T* sA, T* sB, int tid, int p0, int p1,
+                                          int qb0, int lr, int lk) {
   // DEPTH chunks are in flight in registers (2: one more chunk's loads overlap a chunk's
   // products; every mode uses 2, which fits the column step's 128-register budget since
   // only the factoring items hold a diagonal accumulator)
@@ -546,8 +547,8 @@ __device__ __forceinline__ void panel_mma(typename Mfma<T>::acc_t (&acc)[2][2],
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const int kc = kk + lk;
-      const T a0 = a[kc * 64 + pb0 + lr];
-      const T a1 = a[kc * 64 + pb0 + 16 + lr];
+      const T a0 = a[kc * 64 + p0 + lr];
+      const T a1 = a[kc * 64 + p1 + lr];
       const T b0 = bb[kc * 64 + qb0 + lr];
       const T b1 = bb[kc * 64 + qb0 + 16 + lr];
       acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
@@ -650,7 +651,11 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   // D = C^T: rows p of D are columns j of C, columns q of D are rows i of C.
-  const int pb0 = (wave >> 1) * 32, qb0 = (wave & 1) * 32;
+  // wave w holds D rows (C columns) in the 16-blocks {0, 3} (w < 2) or {1, 2} (w >= 2)
+  // and the 32 D columns from qb0: the two row-block sets cost the same in the
+  // triangular TRSM product (tile_mma_lower_a)
+  const int qb0 = (wave & 1) * 32;
+  const int px[2] = {(wave >> 1) ? 16 : 0, (wave >> 1) ? 32 : 48};
   // FP64: the accumulator starts from C (loaded before the panel loads are issued, so
   // its latency overlaps them) and takes -sum_K L_IK L_JK^T.  FP32 sums the products
   // from zero and adds C once at the end (one rounding against C; accumulating into C
@@ -663,7 +668,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        acc[x][y][r] = (!from_c || fresh) ? T(0) : C[(pb0 + 16 * x + Mfma<T>::row(lk, r)) * 64 + qb0 + 16 * y + lr];
+        acc[x][y][r] = (!from_c || fresh) ? T(0) : C[(px[x] + Mfma<T>::row(lk, r)) * 64 + qb0 + 16 * y + lr];
   // C + acc: the updated tile (element (i, j) at j * 64 + i)
   auto updated = [&](int x, int y, int r, int j, int i) {
     return (from_c || fresh) ? acc[x][y][r] : C[j * 64 + i] + acc[x][y][r];
@@ -690,7 +695,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
   }
   panel_mma<T, 2>(acc, pool + ra[w.J] + (int64_t)w.k0 * kTileElems,
                pool + ra[w.I] + (int64_t)w.k0 * kTileElems, (w.k1 - w.k0) * 64, lds_panel,
-               lds_panel + kStage, tid, pb0, qb0, lr, lk);
+               lds_panel + kStage, tid, px[0], px[1], qb0, lr, lk);
 
   // MODE 0, and lookahead items (flags & 2) of a column-step launch: update only
   if (MODE == 0 || (MODE == 1 && (w.flags & 2))) {  // uniform per workgroup
@@ -700,7 +705,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       for (int y = 0; y < 2; ++y)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
+          const int j = px[x] + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
           C[j * 64 + i] = updated(x, y, r, j, i);
         }
     return;
@@ -716,7 +721,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       for (int y = 0; y < 2; ++y)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
+          const int j = px[x] + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
           s[j * kLd + i] = updated(x, y, r, j, i);
         }
     __syncthreads();
@@ -734,7 +739,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
           for (int r = 0; r < 4; ++r)
             a3[x][y][r] = (!from_c || !share || dfresh)
                               ? T(0)
-                              : Cd[(pb0 + 16 * x + Mfma<T>::row(lk, r)) * 64 + qb0 + 16 * y + lr];
+                              : Cd[(px[x] + Mfma<T>::row(lk, r)) * 64 + qb0 + 16 * y + lr];
       auto shared_d = [&](int x, int y, int r, int j, int i) {
         return (from_c || dfresh) ? a3[x][y][r] : Cd[j * 64 + i] + a3[x][y][r];
       };
@@ -742,7 +747,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
         // the diagonal's K = I - 2 term, C_II -= L_{I,I-2} L_{I,I-2}^T (tile final since
         // the previous column launch; the earlier terms came from a diagonal lookahead)
         const T* Tk = pool + ra[w.I] + (int64_t)(w.I - 2) * kTileElems;
-        tile_mma<T, 64, 64, true>(a3, Tk, Tk, pb0, qb0, lr, lk);
+        tile_mma<T, 64, 64, true>(a3, Tk, Tk, px[0], px[1], qb0, lr, lk);
       }
       // X[i][j] = sum_k C'[i][k] Linv[j][k]:  D2[p=j][q=i], A op = Linv (p,k), B op = C' (q,k)
       const T* Li = inv + inv_base[w.mat] + (int64_t)w.J * kTileElems;
@@ -751,7 +756,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) a2[x][y] = acc_t{0, 0, 0, 0};
-      tile_mma_lower_a<T, 64, kLd>(a2, Li, s, pb0, qb0, lr, lk);
+      tile_mma_lower_a<T, 64, kLd>(a2, Li, s, px[0], qb0, lr, lk);
       __syncthreads();  // every wave is done reading C' from s
 #pragma unroll
       for (int x = 0; x < 2; ++x)
@@ -759,7 +764,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
         for (int y = 0; y < 2; ++y)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
+            const int j = px[x] + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
             C[j * 64 + i] = a2[x][y][r];
             s[j * kLd + i] = a2[x][y][r];
           }
@@ -776,7 +781,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
       if (!fac && !(w.flags & 1)) return;  // uniform per workgroup
       __syncthreads();
       // C_II -= X X^T
-      tile_mma<T, kLd, kLd, true>(a3, s, s, pb0, qb0, lr, lk);
+      tile_mma<T, kLd, kLd, true>(a3, s, s, px[0], px[1], qb0, lr, lk);
       if (!fac) {
 #pragma unroll
         for (int x = 0; x < 2; ++x)
@@ -784,7 +789,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
           for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
+              const int j = px[x] + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
               Cd[j * 64 + i] = shared_d(x, y, r, j, i);
             }
       } else {
@@ -796,7 +801,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
           for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const int j = pb0 + 16 * x + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
+              const int j = px[x] + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
               s[j * kLd + i] = shared_d(x, y, r, j, i);
             }
         __syncthreads();
