@@ -1165,17 +1165,30 @@ __global__ __launch_bounds__(256) void k_cg_inv_step(const int32_t* __restrict__
   acc_t m[4];
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) m[cb] = acc_t{0.0, 0.0, 0.0, 0.0};
-  // M = sum_k L_ik X_kj over the envelope of row i
+  // M = sum_k L_ik X_kj over the envelope of row i; the two tiles of step k + 1 are
+  // loaded into registers while step k's products run
   const int k0 = max(j, cft[i]);
-  for (int k = k0; k < i; ++k) {
+  double ra[16], rb[16];
+  auto fetch = [&](int k) {
     const double* Lk = cpool + cra[i] + (int64_t)k * kTileElems;       // column-major tile
     const double* Xk = Xg + (int64_t)(j - a) * 64 * n + (int64_t)(k - a) * 64;
-    for (int e = tid; e < kTileElems; e += 256) {
-      const int hi = e >> 6, lo = e & 63;
-      sA[lo * kLd + hi] = Lk[e];                       // L_ik(row lo, inner hi)
-      sB[lo * kLd + hi] = Xk[(int64_t)hi * n + lo];    // X_kj(inner lo, column hi)
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = tid + 256 * u;
+      ra[u] = Lk[e];
+      rb[u] = Xk[(int64_t)(e >> 6) * n + (e & 63)];
+    }
+  };
+  if (k0 < i) fetch(k0);
+  for (int k = k0; k < i; ++k) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = tid + 256 * u, hi = e >> 6, lo = e & 63;
+      sA[lo * kLd + hi] = ra[u];  // L_ik(row lo, inner hi)
+      sB[lo * kLd + hi] = rb[u];  // X_kj(inner lo, column hi)
     }
     __syncthreads();
+    if (k + 1 < i) fetch(k + 1);
     mma(m);
     __syncthreads();
   }
