@@ -2539,21 +2539,6 @@ void launch_face_extract(int64_t nfaces, int max_tf, const FaceDev* faces, const
                      P, slot1, fpool, ops);
 }
 
-// Tile (i, k) of a symmetric operator stored as lower tiles (row-major inside a tile) into
-// LDS (leading dimension kLd), read coalesced: stored tile (i, k) as is when i >= k, else
-// stored tile (k, i) transposed.
-__device__ __forceinline__ void stage_sym_tile(const double* __restrict__ M, int i, int k, double* sX) {
-  const bool tr = i < k;
-  const int64_t I = tr ? k : i, K = tr ? i : k;
-  const double* src = M + (I * (I + 1) / 2 + K) * kTileElems;
-  for (int e = threadIdx.x; e < 4096; e += 256) {
-    const int a = e >> 6, b = e & 63;  // stored (row a, column b)
-    const double v = src[e];
-    if (tr) sX[b * kLd + a] = v;
-    else sX[a * kLd + b] = v;
-  }
-}
-
 // D_F = A_F^-1 B_F of every face as full tiles (row-major inside a tile), one work-group
 // per output tile (i, j) of a face: D_ij = sum_k (A^-1)_ik B_kj on FP64 MFMA (wave w
 // holds output rows 16 w .. 16 w + 15, four 16x16 blocks), the two input tiles of each k
@@ -2572,10 +2557,31 @@ __global__ __launch_bounds__(256) void k_face_D(int max_tf, const FaceDev* __res
   acc_t acc[4];
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) acc[cb] = acc_t{0.0, 0.0, 0.0, 0.0};
+  // the two tiles of step k + 1 are loaded into registers while step k's products run;
+  // the operators are stored as lower tiles (row-major inside a tile), so a tile (r, c)
+  // with r < c is staged as the transpose of the stored tile (c, r)
+  double ra[16], rb[16];
+  auto fetch = [&](int k) {
+    const double* srcA = ops + F.ainv_off + (int64_t)(i >= k ? i * (i + 1) / 2 + k : k * (k + 1) / 2 + i) * kTileElems;
+    const double* srcB = ops + F.b_off + (int64_t)(k >= j ? k * (k + 1) / 2 + j : j * (j + 1) / 2 + k) * kTileElems;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      ra[u] = srcA[tid + 256 * u];
+      rb[u] = srcB[tid + 256 * u];
+    }
+  };
+  fetch(0);
   for (int k = 0; k < tf; ++k) {
-    stage_sym_tile(ops + F.ainv_off, i, k, sA);
-    stage_sym_tile(ops + F.b_off, k, j, sB);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = tid + 256 * u, r = e >> 6, c = e & 63;  // stored (row r, column c)
+      if (i < k) sA[c * kLd + r] = ra[u];
+      else sA[r * kLd + c] = ra[u];
+      if (k < j) sB[c * kLd + r] = rb[u];
+      else sB[r * kLd + c] = rb[u];
+    }
     __syncthreads();
+    if (k + 1 < tf) fetch(k + 1);
 #pragma unroll 4
     for (int kk = 0; kk < 64; kk += 4) {
       const double a = sA[(16 * wv + lr) * kLd + kk + lk];
