@@ -32,7 +32,7 @@ COARSE_CLASSES = ("coarse_column", "coarse_schur", "coarse_diag")
 # passes over the same workload, each scaled by what the counter reads for a 512 MiB
 # device-to-device copy (FETCH x2.000, WRITE x1.000); the counters cannot be read live.
 MEASURED_TRAFFIC = {("cube48j", "k_tile_update<double,1>"): 0.3279e9,
-                    ("cube96j", "k_tile_update<double,1>"): 0.7458e9}
+                    ("cube96j", "k_tile_update<double,1>"): 0.7324e9}
 WORKLOAD_TEXT = {
     "cube48j": "48^3-hex Kuhn cube (663,552 tets, 753,552 free edges), 6x6x6 jittered "
                "subdomains 6-10 hexes wide, nu=1, kappa=1e-2, 2 Gaussian current loops, PEC",
