@@ -92,7 +92,7 @@ struct fetidp_solver {
   hipStream_t st2 = nullptr;  // side stream of the factorization (mode-3 launches)
   hipStream_t st3 = nullptr;  // operator extraction beside the coarse factorization
   hipEvent_t ev_main = nullptr, ev_side = nullptr, ev_extract = nullptr, ev_zero = nullptr,
-             ev_early = nullptr;
+             ev_early = nullptr, ev_face = nullptr;
   hipEvent_t ev_phi = nullptr, ev_coarse = nullptr;  // PCPG: coarse solve beside the Z product
   int32_t n_mat = 0;
   double symbolic_ms = 0.0, last_factor_ms = 0.0, upload_ms = 0.0;
@@ -272,7 +272,7 @@ struct fetidp_solver {
     for (hipEvent_t e : guess_ev)
       if (e) (void)hipEventDestroy(e);
     if (hsc) (void)hipHostFree(hsc);
-    for (hipEvent_t e : {ev_main, ev_side, ev_extract, ev_zero, ev_early, ev_phi, ev_coarse})
+    for (hipEvent_t e : {ev_main, ev_side, ev_extract, ev_zero, ev_early, ev_face, ev_phi, ev_coarse})
       if (e) (void)hipEventDestroy(e);
     if (st3) (void)hipStreamDestroy(st3);
     if (st2) (void)hipStreamDestroy(st2);
@@ -493,6 +493,7 @@ struct fetidp_solver {
       HIPCHECK(hipEventCreateWithFlags(&ev_extract, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_zero, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_early, hipEventDisableTiming));
+      HIPCHECK(hipEventCreateWithFlags(&ev_face, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_phi, hipEventDisableTiming));
       HIPCHECK(hipEventCreateWithFlags(&ev_coarse, hipEventDisableTiming));
       HIPCHECK(hipEventRecord(ev_side, st2));  // a recorded, complete event to start from
@@ -704,9 +705,12 @@ struct fetidp_solver {
     const int slot0 = local ? 2 : 6;  // local: slots 2..4 (modes 0..2); coarse: 6..8
     bool after_side = false;      // the previous launch was a mode-3 launch on st2
     bool early = false;           // an early Schur launch (mode 4) is pending on st3
+    bool tail = false;            // ... and only Schur launches (mode 0) follow it
     for (size_t li = 0; li < S.launches.size(); ++li) {
       const TileLaunch& L = S.launches[li];
       if (L.mode == 4) {
+        tail = true;
+        for (size_t k = li + 1; k < S.launches.size(); ++k) tail = tail && S.launches[k].mode == 0;
         // Schur updates of finished subdomains: beside the following column launches on
         // st3 (after everything queued on st so far); st meets them again before the
         // S_DD snapshot and at the end
@@ -781,7 +785,9 @@ struct fetidp_solver {
       });
     }
     HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));
-    if (early) HIPCHECK(hipStreamWaitEvent(st, ev_early, 0));
+    // a trailing early launch (the D' rows of the dual Schur updates) is followed on st3
+    // by the W extraction, which the end of the factorization waits for
+    if (early && !tail) HIPCHECK(hipStreamWaitEvent(st, ev_early, 0));
   }
 
   // FP64 Dirichlet blocks when cond(S_DD) ~ max nu / (min kappa h^2) could exceed
@@ -946,15 +952,19 @@ struct fetidp_solver {
       HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
       launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st3);
       launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st3);
+      HIPCHECK(hipEventRecord(ev_extract, st3));
       if (deluxe) {
-        // the face factorizations only read the Dirichlet blocks: on st3 too, beside the
-        // coarse chain (whose dependent single-matrix launches leave most of the GPU idle)
+        // the face factorizations only read the Dirichlet blocks: beside the coarse chain
+        // (whose dependent single-matrix launches leave most of the GPU idle), on st2
+        // when the coarse schedule leaves it free, else after the extraction on st3
+        const hipStream_t fs = cfg.side_stream == 2 ? st2 : st3;
+        if (fs == st2) HIPCHECK(hipStreamWaitEvent(st2, ev_main, 0));
         const hipStream_t main = st;
-        st = st3;
+        st = fs;
         deluxe_setup();
         st = main;
+        HIPCHECK(hipEventRecord(ev_face, fs));
       }
-      HIPCHECK(hipEventRecord(ev_extract, st3));
     }
     // coarse problem in FP64
     cpool.zero(st);
@@ -968,7 +978,10 @@ struct fetidp_solver {
                          cg_group_of_d.p, cg_xoff_d.p, cgX.p, st);
       });
     if (deluxe && profiling()) deluxe_setup();
-    if (!profiling()) HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
+    if (!profiling()) {
+      HIPCHECK(hipStreamWaitEvent(st, ev_extract, 0));
+      if (deluxe) HIPCHECK(hipStreamWaitEvent(st, ev_face, 0));
+    }
     HIPCHECK(hipStreamSynchronize(st));
     prof_collect();
     std::vector<int> hf(topo.n_sub);

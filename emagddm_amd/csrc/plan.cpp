@@ -665,6 +665,7 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
     // the subdomains already done ride beside them (mode 4: a mode-0 launch that may run
     // on a second stream), the rest follow the last column.
     constexpr int kSideFill = 1024;
+    constexpr bool kSplitSchur2 = true;
     auto schur1_of = [&](int64_t s, std::vector<TileWork>& out) {
       const SubLayout& L = T.sub[s];
       if (L.Ti == 0) return;
@@ -786,7 +787,7 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
           if (kd < col) below.push_back(TileWork{int32_t(s), col + 2, col + 2, kd, col, 2});
         }
       }
-      if (inc && !early2 && int64_t(diag.size() + next.size() + below.size()) < kSideFill) {
+      if (inc && !kSplitSchur2 && !early2 && int64_t(diag.size() + next.size() + below.size()) < kSideFill) {
         early2 = true;
         std::vector<TileWork> early;
         for (int64_t s = 0; s < ns; ++s)
@@ -805,8 +806,19 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
     std::vector<TileWork> schur2;
     for (int64_t s = 0; s < ns; ++s)
       if (!schur2_done[size_t(s)]) schur2_of(s, schur2);
-    xcd_grouped(schur2, true);
-    add_tiles(S, 0, schur2);
+    if (kSplitSchur2) {
+      // the coarse problem needs only the c rows: the D' rows (Z and Phi) go to a mode-4
+      // launch that the coarse factorization does not wait for
+      std::vector<TileWork> crow, drow;
+      for (const TileWork& w : schur2) (w.I >= T.sub[w.mat].Tc ? drow : crow).push_back(w);
+      xcd_grouped(drow, true);
+      add_tiles(S, 4, drow);
+      xcd_grouped(crow, true);
+      add_tiles(S, 0, crow);
+    } else {
+      xcd_grouped(schur2, true);
+      add_tiles(S, 0, schur2);
+    }
   }
 
   // ---- fresh tiles: mark items whose tile (flags 4) or diagonal tile (flags 8) nothing
