@@ -698,6 +698,10 @@ struct fetidp_solver {
     else launch_extract_P<T, float>(L.n, snap_tiles.p + L.off, devsub.p, pool, rowabs.p, ftile.p, snap, Pf.p, s);
   }
 
+  // Dynamic LDS (unused) of the trailing early Schur launch, which runs beside the coarse
+  // chain: two of its work-groups (33.3 KB + 24 KB each) per CU leave a chain work-group
+  // (40 KB) room on every CU of the 160 KB, and fewer Schur waves on the chain's SIMDs
+  static constexpr int kTailLdsPad = 24576;
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& rowbase,
                     const DBuf<int64_t>& rabs, const DBuf<int64_t>& inv_base, T* pool, T* inv,
@@ -724,7 +728,7 @@ struct fetidp_solver {
         HIPCHECK(hipEventRecord(ev_main, st));
         HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
         launch_tile_mode<T>(0, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st3,
-                            cfg.tile_waves == 0 ? 0 : 4, rhs);
+                            cfg.tile_waves == 0 ? 0 : 4, rhs, tail ? kTailLdsPad : 0);
         HIPCHECK(hipEventRecord(ev_early, st3));
         early = true;
         continue;

@@ -46,7 +46,7 @@ struct RhsArgs {
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
                       const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
-                      hipStream_t st, int waves = 4, RhsArgs rhs = RhsArgs());
+                      hipStream_t st, int waves = 4, RhsArgs rhs = RhsArgs(), int lds_pad = 0);
 
 // MFMA lane-layout probe (one wave, 512 doubles of output).
 void launch_mfma_probe(int is_f32, double* out);

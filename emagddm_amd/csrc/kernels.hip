@@ -859,15 +859,15 @@ __global__ __launch_bounds__(256) void k_tile_factor(const TileWork* __restrict_
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
                       const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
-                      hipStream_t st, int waves, RhsArgs rhs) {
+                      hipStream_t st, int waves, RhsArgs rhs, int lds_pad) {
   if (n <= 0) return;
   if (mode == 0) rhs = RhsArgs();  // Schur updates do not touch the right-hand side
   if (mode == 0 && waves == 0)
-    hipLaunchKernelGGL((k_tile_update_free<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+    hipLaunchKernelGGL((k_tile_update_free<T, 0>), dim3(n), dim3(256), lds_pad, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else if (mode == 1 && waves == 0)
     hipLaunchKernelGGL((k_tile_update_free<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else if (mode == 0)
-    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), lds_pad, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else if (mode == 1)
     hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
   else if (mode == 2)
@@ -876,9 +876,9 @@ void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* row
     hipLaunchKernelGGL((k_tile_factor<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
 }
 template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                       const int64_t*, double*, double*, int*, hipStream_t, int, RhsArgs);
+                                       const int64_t*, double*, double*, int*, hipStream_t, int, RhsArgs, int);
 template void launch_tile_mode<float>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                      const int64_t*, float*, float*, int*, hipStream_t, int, RhsArgs);
+                                      const int64_t*, float*, float*, int*, hipStream_t, int, RhsArgs, int);
 
 // ---------------------------------------------------------------------------
 // Operator extraction after the partial factorization
