@@ -1165,7 +1165,7 @@ struct fetidp_solver {
       timed(12, [&] {
         launch_forward<T>(topo.n_sub, plans.max_T, plans.max_Tr, devsub.p, pool.p, rowabs.p, inv.p,
                           local_inv_base.p, ftile.p, vec.p, yvec.p, st, interface_only ? 1 : 0,
-                          plans.min_Ti);
+                          plans.max_dual, plans.max_after);
       });
       fwd_valid = is_f;
     }

@@ -99,10 +99,10 @@ template <typename T>
 void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool,
                     const int64_t* rowabs, const T* inv, const int64_t* inv_base,
                     const int32_t* ftile, double* vec, double* yv, hipStream_t st, int skip_int = 0,
-                    int j0 = 0);
-// skip_int: the interior blocks of vec are zero and those of yv already hold zeros, so every
-// subdomain skips its own interior columns and the sweep starts at column j0 (at most the
-// smallest interior tile count).
+                    int max_dual = 0, int max_after = 0);
+// skip_int: the interior blocks of vec are zero and those of yv already hold zeros, so only
+// the dual columns are swept: step J takes every subdomain's dual column Ti + J at once
+// (max_dual = max (Tr - Ti) steps, max_after = max (T - Ti) rows below the first one).
 // Zero the interior blocks (tile rows < Ti) of a local vector.
 void launch_zero_interior(int ns, int max_Ti, const DevSub* subs, double* v, hipStream_t st);
 // kDotBlocks partial sums of x_d^2 over the free dofs d of kind `want` (0 interior).

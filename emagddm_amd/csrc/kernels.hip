@@ -1478,21 +1478,23 @@ __device__ __forceinline__ void colsum4(const T* L, const double* u, int tid, do
   colsum4_sum(v, u, tid, out);
 }
 
-// Forward step J: y_J = L_JJ^-1 b_J (written to yv by the row-J workgroup);
-// rows I > J: b_I -= L_IJ y_J.  D' rows are structurally zero while J < Ti.
-// skip_int: the interior right-hand side is zero (and yv's interior blocks already are),
-// so the interior columns change nothing and are skipped.
+// Forward step: column J (J counted from the subdomain's first dual column Ti when
+// skip_int) gets y_J = L_JJ^-1 b_J (written to yv by the row-J workgroup); rows I > J:
+// b_I -= L_IJ y_J.  D' rows are structurally zero while J < Ti.  skip_int: the interior
+// right-hand side is zero (and yv's interior blocks already are), so the interior columns
+// change nothing and only the dual ones are swept, every subdomain's from the first step.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_step(const DevSub* __restrict__ subs,
                                                   const T* __restrict__ pool,
                                                   const int64_t* __restrict__ rowabs,
                                                   const T* __restrict__ inv,
                                                   const int64_t* __restrict__ inv_base,
-                                                  const int32_t* __restrict__ ftile, int J,
+                                                  const int32_t* __restrict__ ftile, int Jstep,
                                                   int skip_int, double* __restrict__ vec,
                                                   double* __restrict__ yv) {
   const DevSub S = subs[blockIdx.y];
-  if (J >= S.Tr || (skip_int && J < S.Ti)) return;
+  const int J = skip_int ? S.Ti + Jstep : Jstep;
+  if (J >= S.Tr) return;
   const int I = J + blockIdx.x;
   if (I >= S.T || ftile[S.ft_off + I] > J) return;  // tile (I, J) outside the envelope
   __shared__ double part[4][64];
@@ -1515,18 +1517,19 @@ template <typename T>
 void launch_forward(int ns, int max_T, int max_Tr, const DevSub* subs, const T* pool,
                     const int64_t* rowabs, const T* inv, const int64_t* inv_base,
                     const int32_t* ftile, double* vec, double* yv, hipStream_t st, int skip_int,
-                    int j0) {
+                    int max_dual, int max_after) {
   if (ns <= 0) return;
-  for (int J = skip_int ? j0 : 0; J < max_Tr; ++J)
-    hipLaunchKernelGGL(k_fwd_step<T>, dim3(max_T - J, ns), dim3(256), 0, st, subs, pool, rowabs, inv,
+  const int steps = skip_int ? max_dual : max_Tr, rows = skip_int ? max_after : max_T;
+  for (int J = 0; J < steps; ++J)
+    hipLaunchKernelGGL(k_fwd_step<T>, dim3(rows - J, ns), dim3(256), 0, st, subs, pool, rowabs, inv,
                        inv_base, ftile, J, skip_int, vec, yv);
 }
 template void launch_forward<double>(int, int, int, const DevSub*, const double*, const int64_t*,
                                      const double*, const int64_t*, const int32_t*, double*,
-                                     double*, hipStream_t, int, int);
+                                     double*, hipStream_t, int, int, int);
 template void launch_forward<float>(int, int, int, const DevSub*, const float*, const int64_t*,
                                     const float*, const int64_t*, const int32_t*, double*, double*,
-                                    hipStream_t, int, int);
+                                    hipStream_t, int, int, int);
 
 // Zero the interior blocks (tile rows < Ti) of a local vector, one work-group per
 // (interior tile, subdomain).

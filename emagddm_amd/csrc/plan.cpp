@@ -1016,9 +1016,9 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
   P.inv_off.resize(ns);
   {
     int64_t off = 0;
-    P.min_Ti = ns > 0 ? T.sub[0].Ti : 0;
     for (int64_t s = 0; s < ns; ++s) {
-      P.min_Ti = std::min(P.min_Ti, T.sub[s].Ti);
+      P.max_dual = std::max(P.max_dual, T.sub[s].Tr - T.sub[s].Ti);
+      P.max_after = std::max(P.max_after, T.sub[s].T - T.sub[s].Ti);
       P.inv_off[s] = off;
       off += int64_t(T.sub[s].Tr) * kTileElems;
       P.max_T = std::max(P.max_T, T.sub[s].T);
