@@ -710,6 +710,8 @@ struct fetidp_solver {
     bool after_side = false;      // the previous launch was a mode-3 launch on st2
     bool early = false;           // an early Schur launch (mode 4) is pending on st3
     bool tail = false;            // ... and only Schur launches (mode 0) follow it
+    // the local Schur updates write the S_DD snapshot of their D-block tiles (flags 128)
+    const SnapArgs snap = local ? SnapArgs{devsub.p, p64 ? (void*)Pd.p : (void*)Pf.p, p64 ? 0 : 1} : SnapArgs();
     for (size_t li = 0; li < S.launches.size(); ++li) {
       const TileLaunch& L = S.launches[li];
       if (L.mode == 4) {
@@ -721,14 +723,14 @@ struct fetidp_solver {
         if (profiling()) {
           timed(slot0, [&] {
             launch_tile_mode<T>(0, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st,
-                                cfg.tile_waves == 0 ? 0 : 4, rhs);
+                                cfg.tile_waves == 0 ? 0 : 4, rhs, 0, snap);
           });
           continue;
         }
         HIPCHECK(hipEventRecord(ev_main, st));
         HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
         launch_tile_mode<T>(0, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st3,
-                            cfg.tile_waves == 0 ? 0 : 4, rhs, tail ? kTailLdsPad : 0);
+                            cfg.tile_waves == 0 ? 0 : 4, rhs, tail ? kTailLdsPad : 0, snap);
         HIPCHECK(hipEventRecord(ev_early, st3));
         early = true;
         continue;
@@ -785,7 +787,7 @@ struct fetidp_solver {
       }
       timed(slot0 + L.mode, [&] {
         launch_tile_mode<T>(L.mode, items.p + L.off, L.n, rowbase.p, rabs.p, inv_base.p, pool, inv, failp, st,
-                            cfg.tile_waves == 0 ? 0 : 4, rhs);
+                            cfg.tile_waves == 0 ? 0 : 4, rhs, 0, snap);
       });
     }
     HIPCHECK(hipStreamWaitEvent(st, ev_side, 0));

@@ -43,10 +43,20 @@ struct RhsArgs {
   double* y = nullptr;
   const int64_t* off = nullptr;
 };
+// S_DD snapshot written by the Schur updates themselves: a MODE-0 item with flags & 128
+// (a tile of the D block, after its last interior product) also stores its result as
+// tile (I - o_d/64, J - o_d/64) of the subdomain's lower-tile store P (row-major inside a
+// tile, diagonal tiles from their lower half), FP32 when p32.
+struct SnapArgs {
+  const DevSub* subs = nullptr;
+  void* P = nullptr;
+  int p32 = 0;
+};
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
                       const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
-                      hipStream_t st, int waves = 4, RhsArgs rhs = RhsArgs(), int lds_pad = 0);
+                      hipStream_t st, int waves = 4, RhsArgs rhs = RhsArgs(), int lds_pad = 0,
+                      SnapArgs snap = SnapArgs());
 
 // MFMA lane-layout probe (one wave, 512 doubles of output).
 void launch_mfma_probe(int is_f32, double* out);

@@ -637,7 +637,7 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
                                             const int64_t* __restrict__ rowabs,
                                             const int64_t* __restrict__ inv_base,
                                             T* __restrict__ pool, T* __restrict__ inv,
-                                            int* __restrict__ fail, RhsArgs rhs) {
+                                            int* __restrict__ fail, RhsArgs rhs, SnapArgs snap) {
   using acc_t = typename Mfma<T>::acc_t;
   const TileWork w = work[blockIdx.x];
   // uniform per work-group: this item completes and factors the next diagonal tile
@@ -699,6 +699,18 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 
   // MODE 0, and lookahead items (flags & 2) of a column-step launch: update only
   if (MODE == 0 || (MODE == 1 && (w.flags & 2))) {  // uniform per workgroup
+    // MODE 0 with flags & 128: also the S_DD snapshot (see SnapArgs)
+    const bool to_p = MODE == 0 && (w.flags & 128) && snap.P;  // uniform per work-group
+    int64_t pbase = 0;
+    if (to_p) {
+      const DevSub S = snap.subs[w.mat];
+      const int o = S.o_d >> 6, tI = w.I - o, tJ = w.J - o;
+      pbase = S.p_off + ((int64_t)tI * (tI + 1) / 2 + tJ) * kTileElems;
+    }
+    auto put = [&](int64_t k, T v) {
+      if (snap.p32) static_cast<float*>(snap.P)[k] = (float)v;
+      else static_cast<double*>(snap.P)[k] = (double)v;
+    };
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -706,7 +718,12 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int j = px[x] + Mfma<T>::row(lk, r), i = qb0 + 16 * y + lr;
-          C[j * 64 + i] = updated(x, y, r, j, i);
+          const T v = updated(x, y, r, j, i);
+          C[j * 64 + i] = v;
+          if (to_p && (w.I != w.J || i >= j)) {  // element (i, j); a diagonal tile mirrors
+            put(pbase + i * 64 + j, v);           // its lower half
+            if (w.I == w.J && i != j) put(pbase + j * 64 + i, v);
+          }
         }
     return;
   }
@@ -832,8 +849,8 @@ __global__ __launch_bounds__(256, 4) void k_tile_update(const TileWork* __restri
                                                         const int64_t* __restrict__ rowabs,
                                                         const int64_t* __restrict__ inv_base,
                                                         T* __restrict__ pool, T* __restrict__ inv,
-                                                        int* __restrict__ fail, RhsArgs rhs) {
-  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+                                                        int* __restrict__ fail, RhsArgs rhs, SnapArgs snap) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
 }
 // Same code with the compiler's own register allocation (no occupancy floor): the
 // autotuner's alternative for the throughput modes.
@@ -843,8 +860,8 @@ __global__ __launch_bounds__(256) void k_tile_update_free(const TileWork* __rest
                                                            const int64_t* __restrict__ rowabs,
                                                            const int64_t* __restrict__ inv_base,
                                                            T* __restrict__ pool, T* __restrict__ inv,
-                                                           int* __restrict__ fail, RhsArgs rhs) {
-  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+                                                           int* __restrict__ fail, RhsArgs rhs, SnapArgs snap) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
 }
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_tile_factor(const TileWork* __restrict__ work,
@@ -852,33 +869,34 @@ __global__ __launch_bounds__(256) void k_tile_factor(const TileWork* __restrict_
                                                      const int64_t* __restrict__ rowabs,
                                                      const int64_t* __restrict__ inv_base,
                                                      T* __restrict__ pool, T* __restrict__ inv,
-                                                     int* __restrict__ fail, RhsArgs rhs) {
-  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+                                                     int* __restrict__ fail, RhsArgs rhs, SnapArgs snap) {
+  tile_update<T, MODE>(work, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
 }
 
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,
                       const int64_t* rowabs, const int64_t* inv_base, T* pool, T* inv, int* fail,
-                      hipStream_t st, int waves, RhsArgs rhs, int lds_pad) {
+                      hipStream_t st, int waves, RhsArgs rhs, int lds_pad, SnapArgs snap) {
   if (n <= 0) return;
   if (mode == 0) rhs = RhsArgs();  // Schur updates do not touch the right-hand side
+  else snap = SnapArgs();          // only Schur updates take the snapshot
   if (mode == 0 && waves == 0)
-    hipLaunchKernelGGL((k_tile_update_free<T, 0>), dim3(n), dim3(256), lds_pad, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+    hipLaunchKernelGGL((k_tile_update_free<T, 0>), dim3(n), dim3(256), lds_pad, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
   else if (mode == 1 && waves == 0)
-    hipLaunchKernelGGL((k_tile_update_free<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+    hipLaunchKernelGGL((k_tile_update_free<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
   else if (mode == 0)
-    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), lds_pad, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+    hipLaunchKernelGGL((k_tile_update<T, 0>), dim3(n), dim3(256), lds_pad, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
   else if (mode == 1)
-    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+    hipLaunchKernelGGL((k_tile_update<T, 1>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_tile_factor<T, 2>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+    hipLaunchKernelGGL((k_tile_factor<T, 2>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
   else
-    hipLaunchKernelGGL((k_tile_factor<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs);
+    hipLaunchKernelGGL((k_tile_factor<T, 3>), dim3(n), dim3(256), 0, st, w, rowbase, rowabs, inv_base, pool, inv, fail, rhs, snap);
 }
 template void launch_tile_mode<double>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                       const int64_t*, double*, double*, int*, hipStream_t, int, RhsArgs, int);
+                                       const int64_t*, double*, double*, int*, hipStream_t, int, RhsArgs, int, SnapArgs);
 template void launch_tile_mode<float>(int, const TileWork*, int64_t, const int64_t*, const int64_t*,
-                                      const int64_t*, float*, float*, int*, hipStream_t, int, RhsArgs, int);
+                                      const int64_t*, float*, float*, int*, hipStream_t, int, RhsArgs, int, SnapArgs);
 
 // ---------------------------------------------------------------------------
 // Operator extraction after the partial factorization

@@ -666,6 +666,9 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
     // on a second stream), the rest follow the last column.
     constexpr int kSideFill = 1024;
     constexpr bool kSplitSchur2 = true;
+    // The Schur update of a D-block tile (flags 128) also writes its result, S_DD, into
+    // the preconditioner store P; the snapshot launches copy only the D-block tiles that
+    // no Schur item covers (no interior coupling: their assembled values, or zeros).
     auto schur1_of = [&](int64_t s, std::vector<TileWork>& out) {
       const SubLayout& L = T.sub[s];
       if (L.Ti == 0) return;
@@ -673,13 +676,18 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
       for (int J = L.Ti; J < L.Tc; ++J)
         for (int I = J; I < L.Tc; ++I) {
           const int k0 = std::max(ft[I], ft[J]);
-          if (k0 < L.Ti) out.push_back(TileWork{int32_t(s), I, J, k0, L.Ti, 0});
+          if (k0 < L.Ti) out.push_back(TileWork{int32_t(s), I, J, k0, L.Ti, I < L.Tr ? 128 : 0});
         }
     };
     std::vector<uint8_t> schur1_done(size_t(ns), 0);
     auto add_snap = [&](int64_t s) {
-      for (int I = 0; I < T.sub[s].ndp / kTile; ++I)
-        for (int J = 0; J <= I; ++J) P.snap_tiles.push_back(SymTile{int32_t(s), I, J});
+      const SubLayout& L = T.sub[s];
+      const int32_t* ft = &P.ftile[P.ftile_off[s]];
+      const int o = L.o_d / kTile;
+      for (int I = 0; I < L.ndp / kTile; ++I)
+        for (int J = 0; J <= I; ++J)
+          if (L.Ti == 0 || std::max(ft[o + I], ft[o + J]) >= L.Ti)
+            P.snap_tiles.push_back(SymTile{int32_t(s), I, J});
     };
     for (int j = 0; j < maxTi; ++j) {
       std::vector<TileWork> diag, next, below;
