@@ -218,8 +218,10 @@ def cpu_baseline(prob, system, sample_iters):
     gold = json.load(open(gpath)) if os.path.exists(gpath) else None
     rp, col, val, f = system
     n = f.size
-    if sample_iters <= 0:  # about 15 s: ~21 ms per iteration per 753,552 rows on one core
-        sample_iters = int(max(20, min(600, 15.0 / (21e-3 * n / 753552))))
+    if sample_iters <= 0:
+        # about 15 s: one iteration costs ~21 ms per 753,552 rows on one core of the build
+        # container and about half that on the GPU hosts
+        sample_iters = int(max(20, min(1200, 30.0 / (21e-3 * n / 753552))))
     lib_path = os.path.join(ROOT, "oracle", "_ref", "libref_solvers.so")
     if os.path.exists(lib_path):
         lib = C.CDLL(lib_path)
