@@ -702,6 +702,7 @@ struct fetidp_solver {
   // chain: two of its work-groups (33.3 KB + 24 KB each) per CU leave a chain work-group
   // (40 KB) room on every CU of the 160 KB, and fewer Schur waves on the chain's SIMDs
   static constexpr int kTailLdsPad = 24576;
+  void* w_store = nullptr;  // W of the factorization in progress (written by the Schur updates)
   template <typename T>
   void run_schedule(const TileSchedule& S, const DBuf<TileWork>& items, const DBuf<int64_t>& rowbase,
                     const DBuf<int64_t>& rabs, const DBuf<int64_t>& inv_base, T* pool, T* inv,
@@ -711,7 +712,8 @@ struct fetidp_solver {
     bool early = false;           // an early Schur launch (mode 4) is pending on st3
     bool tail = false;            // ... and only Schur launches (mode 0) follow it
     // the local Schur updates write the S_DD snapshot of their D-block tiles (flags 128)
-    const SnapArgs snap = local ? SnapArgs{devsub.p, p64 ? (void*)Pd.p : (void*)Pf.p, p64 ? 0 : 1} : SnapArgs();
+    const SnapArgs snap = local ? SnapArgs{devsub.p, p64 ? (void*)Pd.p : (void*)Pf.p, p64 ? 0 : 1, w_store}
+                                : SnapArgs();
     for (size_t li = 0; li < S.launches.size(); ++li) {
       const TileLaunch& L = S.launches[li];
       if (L.mode == 4) {
@@ -909,10 +911,10 @@ struct fetidp_solver {
     }
     // Only tiles with assembled entries are cleared: the schedule marks each item whose
     // tile (or diagonal tile) nothing has written yet, and the kernels read those as zero.
-    // Outside profiled runs the clearing (st3) overlaps the element integration, and the
-    // global CSR values for the residual check follow the W extraction on st3, beside the
-    // coarse factorization (which leaves most of the GPU idle); st meets st3 again before
-    // the factor ends.
+    // Outside profiled runs the clearing (st3) overlaps the element integration, the
+    // global CSR values for the residual check follow on st3 beside the first column
+    // launches, and the W extraction runs there beside the coarse factorization (which
+    // leaves most of the GPU idle); st meets st3 again before the factor ends.
     const bool side = !profiling();
     hipStream_t sz = side ? st3 : st;
     if (side) {
@@ -940,24 +942,34 @@ struct fetidp_solver {
     });
     launch_scaling(topo.dvec_elems, topo.n_lam, rho_pos.p, pool.p, sizeof(T) == 4, rho.p, lam_pos.p,
                    d_sgn.p, d_sgnD.p, d_half.p, st);
+    if (side) {
+      // the global CSR values (needed first by the residual after the first PCPG solve) on
+      // st3 beside the first column launches, which leave HBM bandwidth to spare
+      HIPCHECK(hipEventRecord(ev_main, st));
+      HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
+      launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st3);
+    }
     fail.zero(st);
     // the forward sweep of f_glob rides along with the column steps (RhsArgs)
     launch_split_rhs(topo.vec_elems, vec_dof.p, vec_w.p, f_glob.p, vec.p, st);
     fwd_valid = false;
     const RhsArgs rhs{vec.p, yvec.p, local_vec_off.p};
+    w_store = W.p;  // the D' Schur updates write W (SnapArgs)
     if constexpr (sizeof(T) == 8)
       run_schedule<T>(plans.local_sched, local_items, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, true, rhs);
     else
       run_schedule<T>(plans.local_sched_ll, local_items_ll, local_rowbase, rowabs, local_inv_base, pool.p, inv.p, fail.p, true, rhs);
     // W extraction (st3) and the coarse problem (st, st2) only read the local
     // factors, so they overlap; st waits for st3 before the end of the factor.
+    // (the extraction only when the Schur updates do not cover all of W)
     if (profiling()) {
-      timed(5, [&] { launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st); });
+      if (!plans.w_fused)
+        timed(5, [&] { launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st); });
     } else {
       HIPCHECK(hipEventRecord(ev_main, st));
       HIPCHECK(hipStreamWaitEvent(st3, ev_main, 0));
-      launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st3);
-      launch_assemble_f64(int64_t(plans.csr.col.size()), nullptr, csr_ptr.p, csr_src.p, Ae.p, csr_val.p, st3);
+      if (!plans.w_fused)
+        launch_extract_W<T>(int64_t(plans.sym_tiles.size()), sym_tiles.p, topo.n_sub, plans.max_ncp, plans.max_ndp, devsub.p, pool.p, rowabs.p, ftile.p, (sizeof(T) == 8 ? tw64.p : tw32.p), W.p, st3);
       HIPCHECK(hipEventRecord(ev_extract, st3));
       if (deluxe) {
         // the face factorizations only read the Dirichlet blocks: beside the coarse chain

@@ -43,14 +43,18 @@ struct RhsArgs {
   double* y = nullptr;
   const int64_t* off = nullptr;
 };
-// S_DD snapshot written by the Schur updates themselves: a MODE-0 item with flags & 128
+// Operator stores written by the Schur updates themselves.  A MODE-0 item with flags & 128
 // (a tile of the D block, after its last interior product) also stores its result as
 // tile (I - o_d/64, J - o_d/64) of the subdomain's lower-tile store P (row-major inside a
-// tile, diagonal tiles from their lower half), FP32 when p32.
+// tile, diagonal tiles from their lower half), FP32 when p32.  One with flags & 32 (a
+// tile of a D' row, after its last dual product) stores the negated result into W: a D'D'
+// tile as lower tile (I - o_a/64, J - o_a/64) of Z, a D'c tile as the 64x64 block
+// (J - o_c/64, I - o_a/64) of the row-major Phi^T that follows Z.
 struct SnapArgs {
   const DevSub* subs = nullptr;
   void* P = nullptr;
   int p32 = 0;
+  void* W = nullptr;
 };
 template <typename T>
 void launch_tile_mode(int mode, const TileWork* w, int64_t n, const int64_t* rowbase,

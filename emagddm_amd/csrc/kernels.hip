@@ -221,6 +221,12 @@ void launch_zero_tiles(int64_t n, const int32_t* list, T* pool, hipStream_t st) 
 template void launch_zero_tiles<double>(int64_t, const int32_t*, double*, hipStream_t);
 template void launch_zero_tiles<float>(int64_t, const int32_t*, float*, hipStream_t);
 
+// elements of a lower-tile symmetric store of order n (device copy of sym_tile_elems)
+__device__ __forceinline__ int64_t sym_elems(int64_t n) {
+  const int64_t t = n >> 6;
+  return t * (t + 1) / 2 * kTileElems;
+}
+
 // ---------------------------------------------------------------------------
 // Batched tile Cholesky on MFMA
 // ---------------------------------------------------------------------------
@@ -699,18 +705,35 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
 
   // MODE 0, and lookahead items (flags & 2) of a column-step launch: update only
   if (MODE == 0 || (MODE == 1 && (w.flags & 2))) {  // uniform per workgroup
-    // MODE 0 with flags & 128: also the S_DD snapshot (see SnapArgs)
+    // MODE 0 with flags & 128 / 32: also the S_DD snapshot / the W store (see SnapArgs)
     const bool to_p = MODE == 0 && (w.flags & 128) && snap.P;  // uniform per work-group
-    int64_t pbase = 0;
+    const bool to_w = MODE == 0 && (w.flags & 32) && snap.W;   // uniform per work-group
+    int64_t pbase = 0, wbase = 0;
+    bool phi = false;
+    int64_t ldw = 64;
     if (to_p) {
       const DevSub S = snap.subs[w.mat];
       const int o = S.o_d >> 6, tI = w.I - o, tJ = w.J - o;
       pbase = S.p_off + ((int64_t)tI * (tI + 1) / 2 + tJ) * kTileElems;
     }
+    if (to_w) {
+      const DevSub S = snap.subs[w.mat];
+      const int oa = S.o_a >> 6, kI = w.I - oa;
+      phi = w.J < oa;  // a D'c tile: block (J - o_c/64, I - o_a/64) of Phi^T
+      if (phi) {
+        const int cJ = w.J - (S.o_c >> 6);
+        wbase = S.w_off + sym_elems(S.ndp) + (int64_t)(cJ * 64) * S.ndp + kI * 64;
+        ldw = S.ndp;
+      } else {
+        const int kJ = w.J - oa;
+        wbase = S.w_off + ((int64_t)kI * (kI + 1) / 2 + kJ) * kTileElems;
+      }
+    }
     auto put = [&](int64_t k, T v) {
       if (snap.p32) static_cast<float*>(snap.P)[k] = (float)v;
       else static_cast<double*>(snap.P)[k] = (double)v;
     };
+    T* Wd = static_cast<T*>(snap.W);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -723,6 +746,14 @@ __device__ __forceinline__ void tile_update(const TileWork* __restrict__ work,
           if (to_p && (w.I != w.J || i >= j)) {  // element (i, j); a diagonal tile mirrors
             put(pbase + i * 64 + j, v);           // its lower half
             if (w.I == w.J && i != j) put(pbase + j * 64 + i, v);
+          }
+          if (to_w) {
+            if (phi) {
+              Wd[wbase + (int64_t)j * ldw + i] = -v;  // Phi^T row j of the block, column i
+            } else if (w.I != w.J || i >= j) {
+              Wd[wbase + i * 64 + j] = -v;
+              if (w.I == w.J && i != j) Wd[wbase + j * 64 + i] = -v;
+            }
           }
         }
     return;
@@ -909,12 +940,6 @@ __device__ __forceinline__ T pool_sym(const T* P, const int64_t* ra, const int32
   const int hi = a > b ? a : b, lo = a > b ? b : a;
   if ((lo >> 6) < ft[hi >> 6]) return T(0);
   return P[ra[hi >> 6] + (int64_t)(lo >> 6) * kTileElems + (lo & 63) * 64 + (hi & 63)];
-}
-
-// elements of a lower-tile symmetric store of order n (device copy of sym_tile_elems)
-__device__ __forceinline__ int64_t sym_elems(int64_t n) {
-  const int64_t t = n >> 6;
-  return t * (t + 1) / 2 * kTileElems;
 }
 
 // (row, column) of element e of a lower-tile symmetric store (tile (I, J <= I) at

@@ -762,7 +762,8 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
       for (int J = L.Tr; J < L.T; ++J)
         for (int I = J; I < L.T; ++I) {
           const int k0 = std::max(L.Ti, std::max(ft[I], ft[J]));
-          if (k0 < L.Tr) out.push_back(TileWork{int32_t(s), I, J, k0, L.Tr, 0});
+          // flags 32: a D' row's Schur update also writes its tile of W (Z or Phi^T)
+          if (k0 < L.Tr) out.push_back(TileWork{int32_t(s), I, J, k0, L.Tr, I >= L.Tc ? 32 : 0});
         }
     };
     std::vector<uint8_t> schur2_done(size_t(ns), 0);
@@ -826,6 +827,19 @@ void build_plans(const Topology& T, Plans& P, int coarse_group_tiles) {
     } else {
       xcd_grouped(schur2, true);
       add_tiles(S, 0, schur2);
+    }
+  }
+
+  // ---- W written by the Schur updates: complete when every Z tile and Phi^T block of
+  // every subdomain has a flags-32 item in both schedules (else the extraction runs) -------
+  P.w_fused = true;
+  for (const TileSchedule* S : {&P.local_sched, &P.local_sched_ll}) {
+    std::vector<int64_t> cnt(size_t(ns), 0);
+    for (const TileWork& w : S->items)
+      if (w.flags & 32) ++cnt[size_t(w.mat)];
+    for (int64_t s = 0; s < ns; ++s) {
+      const int64_t td = T.sub[s].ndp / kTile, tc = T.sub[s].ncp / kTile;
+      if (cnt[size_t(s)] != td * (td + 1) / 2 + tc * td) P.w_fused = false;
     }
   }
 

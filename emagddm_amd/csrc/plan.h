@@ -98,6 +98,7 @@ struct Plans {
   int max_ndp = 0, max_ncp = 0, max_m = 0, max_wrows = 0;
   int max_T = 0, max_Tr = 0, max_Ti = 0;
   int max_dual = 0, max_after = 0;  // max over subdomains of Tr - Ti and of T - Ti
+  bool w_fused = false;  // the Schur updates write all of W (no extraction launch)
   std::vector<int64_t> inv_off;  // per subdomain: offset of its inverse diagonal tiles
   std::vector<int32_t> bwd_width;  // per interior tile row J: max over subdomains of J - f_J
                                    // (rows a backward interior step touches)
