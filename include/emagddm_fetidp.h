@@ -83,7 +83,10 @@ typedef struct fetidp_stats {
   int32_t status;
   int32_t iterations;       /* total PCPG iterations (all refinement steps) */
   int32_t refinements;      /* outer refinement steps taken */
-  int32_t fell_back;        /* 1: FP32 factor was not positive definite, refactored in FP64 */
+  int32_t fell_back;        /* FP32_REFINE only, 1: the answer comes from an FP64 factor
+                               (the accuracy estimate nu/(kappa h^2) 2^-24 exceeded 5e-3,
+                               the FP32 factor was not positive definite, or the outer
+                               refinement stalled) */
   double residual_norm;     /* final ||f - K u||_2 / ||f||_2 (FP64, assembled system) */
   double dual_residual;     /* final PCPG ||r|| / ||d|| */
   double initial_dual_residual; /* ||d - F lambda0|| / ||d|| */
