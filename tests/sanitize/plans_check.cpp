@@ -64,10 +64,16 @@ int main(int argc, char** argv) {
     cg_tiles += (long long)(P.cg_fwd_tiles.size() + P.cg_bwd_tiles.size()) / 2;
   }
   emag::build_plans(T, P);
+  // D-block tiles the Schur updates do not write into P themselves (copied by the
+  // snapshot launches) and their total; whether the Schur updates write all of W
+  long long d_tiles = 0;
+  for (const emag::SubLayout& L : T.sub) d_tiles += (long long)(L.ndp / 64) * (L.ndp / 64 + 1) / 2;
   std::printf("{\"n_edges\": %lld, \"n_free\": %lld, \"n_lambda\": %lld, \"n_primal\": %lld, "
-              "\"n_sub\": %d, \"pool_elems\": %lld, \"csr_nnz\": %zu, \"grouped_panel_tiles\": %lld}\n",
+              "\"n_sub\": %d, \"pool_elems\": %lld, \"csr_nnz\": %zu, \"grouped_panel_tiles\": %lld, "
+              "\"snap_tiles\": %zu, \"d_tiles\": %lld, \"w_fused\": %d}\n",
               (long long)T.n_edges, (long long)T.n_free, (long long)T.n_lam, (long long)T.n_c,
-              T.n_sub, (long long)P.pool_elems, P.csr.col.size(), cg_tiles);
+              T.n_sub, (long long)P.pool_elems, P.csr.col.size(), cg_tiles, P.snap_tiles.size(), d_tiles,
+              int(P.w_fused));
   emag_problem_free(&p);
   return 0;
 }

@@ -60,3 +60,11 @@ def test_plans_clean_under_sanitizers(checker, tmp_path, name):
     assert (got["n_edges"], got["n_free"], got["n_lambda"], got["n_primal"], got["n_sub"]) == \
         (z["n_edges"], z["n_free"], z["n_lambda"], z["n_primal"], z["n_sub"])
     assert got["pool_elems"] == z["pool_elems"]
+    # the Schur updates write S_DD and W themselves: on box partitions every D-block tile
+    # has interior coupling (no snapshot copies); on Voronoi subdomains (one without
+    # interior edges) some D-block tiles are copied by the snapshot launches
+    assert got["w_fused"] == 1
+    if name == "voronoi":
+        assert 0 < got["snap_tiles"] < got["d_tiles"]
+    elif name in ("box", "jittered", "contrast"):
+        assert got["snap_tiles"] == 0 and got["d_tiles"] > 0
