@@ -38,11 +38,14 @@ def test_cube48j_matches_reference_llt(gpu_available):
     p = W.make_workload("cube48j")
     print(f"[configs] cube48j tets={p.n_tets}", flush=True)
     assert np.isclose(p.jq.sum(), g["jq_sum"], rtol=1e-12, atol=1e-12)
-    s = FETIDPSolver().setup_problem(p)
+    # the solution-accuracy claim (rel. L2 error vs the reference's direct solution) is
+    # made at a residual target below the 1e-8 stopping rule of the benchmark: the error
+    # can exceed the residual by up to cond(K) ~ nu / (kappa h^2) = 2.3e5 here
+    s = FETIDPSolver(tolerance=1e-10).setup_problem(p)
     s.set_rhs_tet_J(p.jq)
     s.factor()
     r = s.solve()
-    assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-8, r
+    assert r.status == SolverStatus.SUCCESS and r.residual_norm <= 1e-10, r
     res = _host_residual(s, r.x)
     n = r.x.size
     assert n == int(g["n_free"])
@@ -55,7 +58,7 @@ def test_cube48j_matches_reference_llt(gpu_available):
                 / (np.sqrt(n) * xn) for k, pk in enumerate(g["proj"]))
     print(f"[configs] cube48j its={r.iterations} res={r.residual_norm:.2e} host={res:.2e} "
           f"dnorm={dnorm:.1e} dsample={dsamp:.1e} dproj={dproj:.1e}", flush=True)
-    assert res <= 1e-8
+    assert res <= 1.01e-10
     assert dnorm <= 1e-8 and dsamp <= 1e-8 and dproj <= 1e-8
 
 
